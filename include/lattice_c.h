@@ -1,0 +1,151 @@
+/*
+ * lattice_c.h -- C ABI of the MI355X (gfx950) lattice evolution library.
+ *
+ * This is the drop-in boundary for the reference's Fortran evolution path
+ * (jonathanbraden/lattice-dev).  Every entry point is a plain C function
+ * taking plain pointers and sizes; status is an int (0 = ok, < 0 = error,
+ * message via lat_last_error).  An opaque lat_ctx owns all device memory,
+ * FFT twiddle tables and one HIP stream.  Host pointers are borrowed for the
+ * duration of a call only.  A context is not thread-safe; use one per thread.
+ *
+ * Layout at this boundary is the reference's: Fortran column-major, x
+ * fastest.  A real field f(nx,ny,nz) is f[i + nx*(j + ny*k)].  A half-complex
+ * transform is FFTW's r2c layout Fk(nx/2+1, ny, nz) of interleaved doubles.
+ * Field sets passed to lat_set_state/lat_get_state are the reference's AoS
+ * field-fastest arrays fld(nfld, [0:]nx[+1], ...) with an optional ghost
+ * layer of width `ghost` (0 or 1) on every side.
+ *
+ * Each declaration names the reference interface it replaces (file:line,
+ * relative to the reference repository root).
+ */
+#ifndef LATTICE_C_H
+#define LATTICE_C_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lat_ctx lat_ctx;
+
+enum lat_precision { LAT_F64 = 0, LAT_MIXED = 1 };
+enum lat_laplacian { LAT_LAP_SPECTRAL = 0, LAT_LAP_STENCIL = 1 };
+enum lat_timevar { LAT_CONFORMAL = 0, LAT_COSMIC = 1 };
+enum lat_model { LAT_MODEL_PHI4 = 0, LAT_MODEL_TWOFIELD = 1 };
+
+/* Energy/Friedmann diagnostics slots of lat_energies(). */
+enum lat_energy_slot {
+    LAT_E_A = 0,      /* scale factor a                                   */
+    LAT_E_HUBBLE,     /* H                                                */
+    LAT_E_RHO,        /* rho = KE + PE + GE                               */
+    LAT_E_KE,
+    LAT_E_PE,
+    LAT_E_GE,
+    LAT_E_GRAV,       /* -3 H^2                                           */
+    LAT_E_RESID,      /* (rho - 3H^2) / rho                               */
+    LAT_E_PMOM,       /* -p^2/12/a^4 (conformal) or -3 p_y^2/16/a^2       */
+    LAT_E_MEAN1,      /* <phi_1>                                          */
+    LAT_E_MEAN2,      /* <phi_2>                                          */
+    LAT_E_GEGRAD,     /* FD gradient energy, GRAD2 form (stencil only)    */
+    LAT_NE
+};
+
+/* Runtime configuration (replaces the reference's compile-time params.f90,
+ * macros.h and the HAM= model selection of Pseudospec/mpi_dev/Makefile:29-35). */
+typedef struct lat_config {
+    int nx, ny, nz;      /* lattice (params.f90:6); each must be 2^a * 3^b, 8..1024 */
+    int nfld;            /* fields: 1 (phi4) or 2 (twofield)                     */
+    double len;          /* box side L: dx = L/nx, dk = 2 pi / L (params.f90:11-13) */
+    int precision;       /* LAT_F64 or LAT_MIXED                                  */
+    int laplacian;       /* LAT_LAP_SPECTRAL (SPECTRAL) or LAT_LAP_STENCIL (DISCRETE) */
+    double stencil[5];   /* c0,c1,c2,c3,cc (hamiltonian_conformal.f90:41)          */
+    int time_var;        /* LAT_CONFORMAL (hamiltonian_conformal.f90) or LAT_COSMIC */
+    int model;           /* LAT_MODEL_*                                           */
+    double model_params[4]; /* phi4: lambda; twofield: m^2, g^2                  */
+    int symp6_tree;      /* symp6 w1 constant: 0 openmp_dev, 1 mpi_dev            */
+    int reserved[7];
+} lat_config;
+
+/* ---- lifetime ---- */
+int  lat_create(const lat_config *cfg, lat_ctx **out);
+void lat_destroy(lat_ctx *ctx);
+const char *lat_last_error(const lat_ctx *ctx);
+int  lat_synchronize(lat_ctx *ctx);
+
+/* ---- spectral operators (module fftw3) ---- */
+/* Replaces laplacian_3d(n1,n2,n3,f,Fk,dk,planf,planb),
+ * Pseudospec/openmp_dev/fftw_mod.f90:177-207 (generic laplacian_spectral :13-15).
+ * f (host, n1*n2*n3 doubles) is overwritten in place with its Laplacian.  The
+ * reference's Fk scratch and FFTW plans are owned by the context. */
+int  lat_laplacian_3d(lat_ctx *ctx, int n1, int n2, int n3, double *f, double dk);
+
+/* Replaces gradient_squared_3d_spectral,
+ * Pseudospec/mpi_dev/fftw_mod_mpi.f90:279-327 (correct-sign form). */
+int  lat_gradient_squared_3d(lat_ctx *ctx, const double *f, double *grad2, double dk);
+
+/* Replaces fftw_execute_dft_r2c(planf, f, Fk) as called at
+ * Pseudospec/openmp_dev/fftw_mod.f90:191 and analysis_spec.f90:77, and
+ * p3dfft_ftran_r2c (Discrete/Entropy/p3dfft/build/wrap.f:1-29): unnormalised
+ * forward R2C into FFTW layout. */
+int  lat_fft_r2c_3d(lat_ctx *ctx, const double *f, double *Fk);
+/* Replaces fftw_execute_dft_c2r(planb, Fk, f) (fftw_mod.f90:203) and
+ * p3dfft_btran_c2r: unnormalised backward C2R.  Fk is not modified. */
+int  lat_fft_c2r_3d(lat_ctx *ctx, const double *Fk, double *f);
+
+/* Finite-difference Laplacian STENCIL(c,LAPLACIAN)/(cc*dx^2) with periodic
+ * wrap: Pseudospec/openmp_dev/macros.h:26-33 with wrap_fields
+ * (hamiltonian_conformal.f90:309-330). */
+int  lat_stencil_laplacian(lat_ctx *ctx, const double *f, double *lap);
+
+/* ---- spectra (module analysis) ---- */
+/* Replaces init_spectrum_3d(nx,ny,nz), analysis_spec.f90:48-57: returns ns. */
+int  lat_num_bins(int nx, int ny, int nz);
+/* Replaces spectrum_3d(S, f, Fk, plan), analysis_spec.f90:59-102.  S has ns
+ * entries; Fk (nullable) receives the R2C of f in FFTW layout, as the
+ * reference leaves it in its Fk argument. */
+int  lat_spectrum_3d(lat_ctx *ctx, double *S, const double *f, double *Fk);
+/* Replaces crossspec_3d(Fk1, Fk2, S_real, S_imag), analysis_spec.f90:145-178.
+ * Fk1, Fk2 in FFTW layout. */
+int  lat_crossspec_3d(lat_ctx *ctx, const double *Fk1, const double *Fk2, double *S_re, double *S_im);
+/* Device-resident output spectra of make_output (evolve_spectral.f90:523-551):
+ * out[0..ns) S_phi1, [ns..2ns) S_pi1, [2ns..3ns) Re S(pi1,phi1), [3ns..4ns) Im. */
+int  lat_output_spectra(lat_ctx *ctx, int field, double *out);
+
+/* ---- state (module Hamiltonian globals fld, fldp, yscl, ysclp,
+ *      hamiltonian_conformal.f90:22-24; hamiltonian_cosmic.f90:27,31) ---- */
+int  lat_set_state(lat_ctx *ctx, const double *fld, const double *fldp, int ghost,
+                   double yscl, double ysclp);
+int  lat_get_state(lat_ctx *ctx, double *fld, double *fldp, int ghost,
+                   double *yscl, double *ysclp);
+int  lat_get_scalars(lat_ctx *ctx, double *yscl, double *ysclp);
+int  lat_set_scalars(lat_ctx *ctx, double yscl, double ysclp);
+
+/* ---- evolution (module Hamiltonian / integrator) ---- */
+/* Replaces Hamiltonian_Split(dt, hamind): hamiltonian_conformal.f90:66-81
+ * (1 drift, 2 grav, 3 kick) / hamiltonian_cosmic.f90:74-89 (1 grav, 2 drift). */
+int  lat_split(lat_ctx *ctx, double dt, int hamind);
+/* Replaces symp_o2step(dt, w1, w2): hamiltonian_conformal.f90:50-64. */
+int  lat_symp_o2step(lat_ctx *ctx, double dt, double w1, double w2);
+/* Replaces symp2/symp4/symp6/symp8(dt, nsteps): Pseudospec/mpi_dev/integrator.f90:11-110
+ * and openmp_dev/evolve_spectral.f90:129-227.  State stays on the device. */
+int  lat_symp(lat_ctx *ctx, int order, double dt, int nsteps);
+/* Replaces calc_metric(): hamiltonian_conformal.f90:263-304 / hamiltonian_cosmic.f90:351-403. */
+int  lat_calc_metric(lat_ctx *ctx);
+/* Replaces dump_rho (openmp_dev/evolve_spectral.f90:305-415) / scalar_rho
+ * (mpi_dev/hamiltonian_cosmic.f90:268-349) and the get_scale_factor,
+ * get_hubble, grav_energy functions: out[LAT_NE]. */
+int  lat_energies(lat_ctx *ctx, double *out);
+
+/* ---- device pointers for in-process callers (state stays resident) ---- */
+/* Returns device addresses of the SoA state arrays (fld, fldp: nfld*nvol
+ * elements of double for LAT_F64, float for LAT_MIXED fluctuations). */
+int  lat_device_state(lat_ctx *ctx, void **fld, void **fldp);
+/* Kernel time accounting: milliseconds spent in the last lat_symp call
+ * (HIP events on the context stream), and launch counts per stage. */
+int  lat_last_symp_ms(lat_ctx *ctx, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

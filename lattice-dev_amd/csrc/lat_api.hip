@@ -1,0 +1,849 @@
+// lat_api.hip -- C ABI (include/lattice_c.h): context, state transfer, the
+// drop-in operators, and the device-resident symplectic executor.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "lat_internal.h"
+
+namespace lat {
+
+int hip_fail(Ctx *c, hipError_t e, const char *where)
+{
+    if (c) {
+        c->last_error = std::string(where) + ": " + hipGetErrorString(e);
+    }
+    return -3;
+}
+
+size_t elem_size(const Ctx &c) { return c.prec == LAT_F64 ? sizeof(double) : sizeof(float); }
+
+int launch_stencil_io(Ctx &c, const double *f, double *out);
+
+static bool size_ok(int n)
+{
+    static const int ok[] = {8, 16, 32, 64, 128, 256, 512, 1024, 48, 96, 192};
+    for (int v : ok)
+        if (v == n) return true;
+    return false;
+}
+
+template <typename T>
+static cpx<T> *make_twiddles(Ctx &c, int n, int *rc)
+{
+    std::vector<cpx<T>> h(n);
+    const long double twopi = 6.283185307179586476925286766559005768L;
+    for (int m = 0; m < n; ++m) {
+        long double a = -twopi * (long double)m / (long double)n;
+        h[m].x = (T)cosl(a);
+        h[m].y = (T)sinl(a);
+    }
+    cpx<T> *d = nullptr;
+    hipError_t e = hipMalloc(&d, sizeof(cpx<T>) * n);
+    if (e != hipSuccess) { *rc = hip_fail(&c, e, "hipMalloc twiddles"); return nullptr; }
+    e = hipMemcpy(d, h.data(), sizeof(cpx<T>) * n, hipMemcpyHostToDevice);
+    if (e != hipSuccess) { *rc = hip_fail(&c, e, "hipMemcpy twiddles"); return nullptr; }
+    return d;
+}
+
+static int ensure(Ctx &c, void **p, size_t bytes, const char *what)
+{
+    if (*p) return 0;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return hip_fail(&c, e, what);
+    return 0;
+}
+
+static int sync(Ctx &c)
+{
+    LAT_HIP(&c, hipStreamSynchronize(c.stream));
+    return 0;
+}
+
+static int pull_scalars(Ctx &c)
+{
+    LAT_HIP(&c, hipMemcpyAsync(&c.host_sc, c.dsc, sizeof(DevScalars), hipMemcpyDeviceToHost, c.stream));
+    return sync(c);
+}
+
+static int push_scalars(Ctx &c)
+{
+    LAT_HIP(&c, hipMemcpyAsync(c.dsc, &c.host_sc, sizeof(DevScalars), hipMemcpyHostToDevice, c.stream));
+    return sync(c);
+}
+
+// ---------------------------------------------------------------- Laplacian pipeline
+// Forward R2C (x, y, z) of the nfld state fields into Fk; drift fused if asked.
+template <typename T>
+static int forward_state(Ctx &c, bool drift, int zmode)
+{
+    const cpx<T> *twx = (const cpx<T> *)(sizeof(T) == 8 ? (void *)c.twx64 : (void *)c.twx32);
+    const cpx<T> *twy = (const cpx<T> *)(sizeof(T) == 8 ? (void *)c.twy64 : (void *)c.twy32);
+    const cpx<T> *twz = (const cpx<T> *)(sizeof(T) == 8 ? (void *)c.twz64 : (void *)c.twz32);
+    T *fld = (T *)c.fld;
+    const T *fldp = (const T *)c.fldp;
+    cpx<T> *Fk = (cpx<T> *)c.Fk;
+    int rc;
+    for (int l = 0; l < c.nfld; ++l) {
+        rc = launch_x_r2c<T>(c, twx, fld + (size_t)l * c.nvol, fldp + (size_t)l * c.nvol, Fk + (size_t)l * c.nk,
+                             drift ? 1 : 0);
+        if (rc) return rc;
+    }
+    if ((rc = launch_col<T>(c, twy, Fk, 1, 0, c.nfld))) return rc;
+    if ((rc = launch_col<T>(c, twz, Fk, 2, zmode, c.nfld))) return rc;
+    return 0;
+}
+
+// Laplacian of the state into the C2R epilogue: EPI_KICK or EPI_REDUCE.
+// Returns the number of reduction partials (>0) or an error (<0).
+template <typename T>
+static int state_force(Ctx &c, bool drift, int epi)
+{
+    if (c.cfg.laplacian == LAT_LAP_STENCIL) {
+        if (drift) {
+            int rc = launch_drift<T>(c);
+            if (rc) return rc;
+        }
+        return launch_stencil<T>(c, epi);
+    }
+    int rc = forward_state<T>(c, drift, 2);
+    if (rc) return rc;
+    const cpx<T> *twx = (const cpx<T> *)(sizeof(T) == 8 ? (void *)c.twx64 : (void *)c.twx32);
+    const cpx<T> *twy = (const cpx<T> *)(sizeof(T) == 8 ? (void *)c.twy64 : (void *)c.twy32);
+    if ((rc = launch_col<T>(c, twy, (cpx<T> *)c.Fk, 1, 1, c.nfld))) return rc;
+    return launch_x_c2r<T>(c, twx, (cpx<T> *)c.Fk, c.nfld, epi, nullptr, (const T *)c.fld, (T *)c.fldp);
+}
+
+static int force(Ctx &c, bool drift, int epi)
+{
+    return c.prec == LAT_F64 ? state_force<double>(c, drift, epi) : state_force<float>(c, drift, epi);
+}
+
+// ---------------------------------------------------------------- op executor
+struct Op {
+    int type; // OP_DRIFT, OP_GRAV, or 99 = kick
+    double h;
+};
+constexpr int OP_KICK = 99;
+
+static void map_split(const Ctx &c, std::vector<Op> &ops, double h, int hamind)
+{
+    // hamiltonian_conformal.f90:70-80 / hamiltonian_cosmic.f90:78-88
+    const bool conf = c.cfg.time_var == LAT_CONFORMAL;
+    if (hamind == 3) ops.push_back({OP_KICK, h});
+    else if ((hamind == 1) == conf) ops.push_back({OP_DRIFT, h});
+    else ops.push_back({OP_GRAV, h});
+}
+
+static void o2step(const Ctx &c, std::vector<Op> &ops, double dt, double w1, double w2)
+{
+    // symp_o2step, hamiltonian_conformal.f90:50-64 (n_Hamiltonian_terms = 3)
+    map_split(c, ops, w1 * dt / 2.0, 2);
+    map_split(c, ops, w1 * dt, 3);
+    map_split(c, ops, w1 * dt / 2.0, 2);
+    map_split(c, ops, (w1 + w2) * dt / 2.0, 1);
+}
+
+static int build_symp(const Ctx &c, std::vector<Op> &ops, int order, double dt, int nsteps)
+{
+    if (nsteps <= 0) return 0;
+    if (order == 2) { // mpi_dev/integrator.f90:11-22
+        map_split(c, ops, dt / 2.0, 1);
+        for (int j = 1; j <= nsteps - 1; ++j) o2step(c, ops, dt, 1.0, 1.0);
+        o2step(c, ops, dt, 1.0, 0.0);
+    } else if (order == 4) { // integrator.f90:24-44
+        const double w1 = 1.0 / (2.0 - std::pow(2.0, 1.0 / 3.0));
+        const double w0 = 1.0 - 2.0 * w1;
+        map_split(c, ops, w1 * dt / 2.0, 1);
+        for (int i = 1; i <= nsteps; ++i) {
+            o2step(c, ops, dt, w1, w0);
+            o2step(c, ops, dt, w0, w1);
+            o2step(c, ops, dt, w1, i == nsteps ? 0.0 : w1);
+        }
+    } else if (order == 6) { // integrator.f90:46-71 / evolve_spectral.f90:169-194
+        const double w1 = c.cfg.symp6_tree ? -1.17767998417887 : -1.177679984177887;
+        const double w2 = 0.235573213359357, w3 = 0.784513610477560;
+        const double w0 = 1.0 - 2.0 * (w1 + w2 + w3);
+        map_split(c, ops, w3 * dt / 2.0, 1);
+        for (int j = 1; j <= nsteps; ++j) {
+            o2step(c, ops, dt, w3, w2);
+            o2step(c, ops, dt, w2, w1);
+            o2step(c, ops, dt, w1, w0);
+            o2step(c, ops, dt, w0, w1);
+            o2step(c, ops, dt, w1, w2);
+            o2step(c, ops, dt, w2, w3);
+            o2step(c, ops, dt, w3, j == nsteps ? 0.0 : w3);
+        }
+    } else if (order == 8) { // integrator.f90:73-110
+        const double w1 = 0.74167036435061295344822780, w2 = -0.40910082580003159399730010,
+                     w3 = 0.19075471029623837995387626, w4 = -0.57386247111608226665638733,
+                     w5 = 0.29906418130365592384446354, w6 = 0.33462491824529818378495798,
+                     w7 = 0.31529309239676659663205666;
+        const double w0 = 1.0 - 2.0 * (w1 + w2 + w3 + w4 + w5 + w6 + w7);
+        const double seq[16] = {w7, w6, w5, w4, w3, w2, w1, w0, w1, w2, w3, w4, w5, w6, w7, 0.0};
+        map_split(c, ops, w7 * dt / 2.0, 1);
+        for (int j = 1; j <= nsteps; ++j) {
+            for (int q = 0; q < 14; ++q) o2step(c, ops, dt, seq[q], seq[q + 1]);
+            o2step(c, ops, dt, w7, j == nsteps ? 0.0 : w7);
+        }
+    } else {
+        return -1;
+    }
+    return 0;
+}
+
+// Runs a linear sequence of split operators on the device.  Between two
+// kicks, the scalar kernel applies (in order) the pending kick's p-update, the
+// drifts' and grav steps' scalar parts, and computes the next kick's
+// coefficients; the drifts' field updates are fused into the next x-R2C sweep
+// (spectral) or run as one drift sweep (stencil, end of sequence).
+static int run_ops(Ctx &c, const std::vector<Op> &ops)
+{
+    int npart = 0;
+    if (!c.ke_valid) {
+        npart = c.prec == LAT_F64 ? launch_reduce_state<double>(c) : launch_reduce_state<float>(c);
+        if (npart < 0) return npart;
+    }
+    ScalarOps so{};
+    int ndrift = 0;
+    bool kick_pending = false;
+    auto flush_drifts = [&]() -> int {
+        // more drifts than the fused sweep applies: settle them now
+        int rc = launch_scalar(c, so, npart);
+        if (rc) return rc;
+        npart = 0;
+        so = ScalarOps{};
+        kick_pending = false;
+        rc = c.prec == LAT_F64 ? launch_drift<double>(c) : launch_drift<float>(c);
+        ndrift = 0;
+        return rc;
+    };
+    auto begin_seg = [&]() {
+        so = ScalarOps{};
+        if (kick_pending) {
+            so.type[so.n] = OP_FINISH_KICK;
+            so.h[so.n++] = 0.0;
+        }
+    };
+    begin_seg();
+    for (const Op &op : ops) {
+        if (op.type == OP_KICK) {
+            so.type[so.n] = OP_PREP_KICK;
+            so.h[so.n++] = op.h;
+            int rc = launch_scalar(c, so, npart);
+            if (rc) return rc;
+            int g = force(c, ndrift > 0, EPI_KICK);
+            if (g < 0) return g;
+            npart = g;
+            ndrift = 0;
+            kick_pending = true;
+            begin_seg();
+        } else {
+            if (op.type == OP_DRIFT && ndrift == 2) {
+                int rc = flush_drifts();
+                if (rc) return rc;
+                begin_seg();
+            }
+            if (so.n >= MAX_OPS - 1) {
+                c.last_error = "op segment overflow";
+                return -1;
+            }
+            so.type[so.n] = op.type;
+            so.h[so.n++] = op.h;
+            if (op.type == OP_DRIFT) ++ndrift;
+        }
+    }
+    if (so.n > 0 || npart > 0) {
+        int rc = launch_scalar(c, so, npart);
+        if (rc) return rc;
+    }
+    if (ndrift > 0) {
+        int rc = c.prec == LAT_F64 ? launch_drift<double>(c) : launch_drift<float>(c);
+        if (rc) return rc;
+    }
+    c.ke_valid = true;
+    return 0;
+}
+
+// Reduce GE (with the Laplacian), PE, KE and field means of the current state.
+static int state_sums(Ctx &c)
+{
+    int g = force(c, false, EPI_REDUCE);
+    if (g < 0) return g;
+    ScalarOps none{};
+    int rc = launch_scalar(c, none, g);
+    if (rc) return rc;
+    c.ke_valid = true;
+    return pull_scalars(c);
+}
+
+static double scale_factor(const Ctx &c, double y)
+{
+    return c.cfg.time_var == LAT_CONFORMAL ? y : std::pow(y, 2.0 / 3.0);
+}
+
+// ---------------------------------------------------------------- host transforms (FP64)
+static int ensure_io(Ctx &c)
+{
+    int rc = ensure(c, (void **)&c.io64, sizeof(double) * c.nvol * 2, "hipMalloc io64");
+    if (rc) return rc;
+    if (c.prec == LAT_F64) {
+        c.Fk64 = c.Fk;
+        return 0;
+    }
+    return ensure(c, &c.Fk64, sizeof(cpx<double>) * c.nk * 2, "hipMalloc Fk64");
+}
+
+static int host_r2c(Ctx &c, const double *f_host)
+{
+    int rc = ensure_io(c);
+    if (rc) return rc;
+    LAT_HIP(&c, hipMemcpyAsync(c.io64, f_host, sizeof(double) * c.nvol, hipMemcpyHostToDevice, c.stream));
+    cpx<double> *F = (cpx<double> *)c.Fk64;
+    if ((rc = launch_x_r2c<double>(c, c.twx64, c.io64, nullptr, F, 0))) return rc;
+    if ((rc = launch_col<double>(c, c.twy64, F, 1, 0, 1))) return rc;
+    return launch_col<double>(c, c.twz64, F, 2, 0, 1);
+}
+
+static int fk_to_host(Ctx &c, const cpx<double> *F, double *Fk_host)
+{
+    LAT_HIP(&c, hipMemcpy2DAsync(Fk_host, sizeof(cpx<double>) * c.nxh, F, sizeof(cpx<double>) * c.P,
+                                 sizeof(cpx<double>) * c.nxh, (size_t)c.ny * c.nz, hipMemcpyDeviceToHost, c.stream));
+    return 0;
+}
+
+static int fk_from_host(Ctx &c, cpx<double> *F, const double *Fk_host)
+{
+    LAT_HIP(&c, hipMemcpy2DAsync(F, sizeof(cpx<double>) * c.P, Fk_host, sizeof(cpx<double>) * c.nxh,
+                                 sizeof(cpx<double>) * c.nxh, (size_t)c.ny * c.nz, hipMemcpyHostToDevice, c.stream));
+    return 0;
+}
+
+// derivative multiply: G = i k_axis dk * F / nvol (signed wavenumbers)
+__global__ void k_deriv_mult(const cpx<double> *__restrict__ F, cpx<double> *__restrict__ G, int nxh, int P, int ny,
+                             int nz, int axis, double fac)
+{
+    const size_t n = (size_t)P * ny * nz;
+    for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < n; id += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(id % P);
+        const int j = (int)((id / P) % ny);
+        const int k = (int)(id / ((size_t)P * ny));
+        if (i >= nxh) continue;
+        int w;
+        if (axis == 0) w = i;
+        else if (axis == 1) w = (j <= ny / 2) ? j : j - ny;
+        else w = (k <= nz / 2) ? k : k - nz;
+        const double m = (double)w * fac;
+        const cpx<double> a = F[id];
+        G[id] = {-m * a.y, m * a.x};
+    }
+}
+
+__global__ void k_sq_acc(const double *__restrict__ d, double *__restrict__ acc, size_t n, int first)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        acc[i] = first ? d[i] * d[i] : acc[i] + d[i] * d[i];
+}
+
+// ---------------------------------------------------------------- mixed-form helpers
+template <typename T>
+static int upload_fields(Ctx &c, void *dst, const double *soa, size_t n)
+{
+    if (sizeof(T) == 8) {
+        LAT_HIP(&c, hipMemcpy(dst, soa, sizeof(double) * n, hipMemcpyHostToDevice));
+        return 0;
+    }
+    std::vector<float> tmp(n);
+    for (size_t i = 0; i < n; ++i) tmp[i] = (float)soa[i];
+    LAT_HIP(&c, hipMemcpy(dst, tmp.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    return 0;
+}
+
+template <typename T>
+static int download_fields(Ctx &c, double *soa, const void *src, size_t n)
+{
+    if (sizeof(T) == 8) {
+        LAT_HIP(&c, hipMemcpy(soa, src, sizeof(double) * n, hipMemcpyDeviceToHost));
+        return 0;
+    }
+    std::vector<float> tmp(n);
+    LAT_HIP(&c, hipMemcpy(tmp.data(), src, sizeof(float) * n, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) soa[i] = tmp[i];
+    return 0;
+}
+
+} // namespace lat
+
+using namespace lat;
+
+#define CTX_OR_FAIL(ctx)            \
+    if (!(ctx)) return -1;          \
+    Ctx &c = (ctx)->c;              \
+    LAT_HIP(&c, hipSetDevice(c.device))
+
+extern "C" {
+
+int lat_create(const lat_config *cfg, lat_ctx **out)
+{
+    if (!cfg || !out) return -1;
+    *out = nullptr;
+    if (!size_ok(cfg->nx) || !size_ok(cfg->ny) || !size_ok(cfg->nz)) return -2;
+    if (cfg->nfld < 1 || cfg->nfld > 2) return -2;
+    if (cfg->model == LAT_MODEL_TWOFIELD && cfg->nfld != 2) return -2;
+    if (cfg->model == LAT_MODEL_PHI4 && cfg->nfld != 1) return -2;
+    if (cfg->precision != LAT_F64 && cfg->precision != LAT_MIXED) return -2;
+    if (!(cfg->len > 0.0)) return -2;
+    if (cfg->laplacian == LAT_LAP_STENCIL && cfg->stencil[4] == 0.0) return -2;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return -4;
+    lat_ctx *h = new lat_ctx();
+    Ctx &c = h->c;
+    c.cfg = *cfg;
+    hipGetDevice(&c.device);
+    c.nx = cfg->nx;
+    c.ny = cfg->ny;
+    c.nz = cfg->nz;
+    c.nxh = c.nx / 2 + 1;
+    c.P = (c.nxh + 7) / 8 * 8;
+    c.nvol = (size_t)c.nx * c.ny * c.nz;
+    c.nk = (size_t)c.P * c.ny * c.nz;
+    c.dx = cfg->len / c.nx;
+    c.dk = 6.2831853071795864769252867665590 / cfg->len;
+    c.prec = cfg->precision;
+    c.nfld = cfg->nfld;
+    c.ns = lat_num_bins(c.nx, c.ny, c.nz);
+    int rc = 0;
+    hipError_t e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete h; return -3; }
+    const size_t es = elem_size(c);
+    rc |= ensure(c, &c.fld, es * c.nfld * c.nvol, "hipMalloc fld");
+    rc |= ensure(c, &c.fldp, es * c.nfld * c.nvol, "hipMalloc fldp");
+    if (cfg->laplacian == LAT_LAP_SPECTRAL || c.prec == LAT_F64)
+        rc |= ensure(c, &c.Fk, 2 * es * (c.nfld > 2 ? c.nfld : 2) * c.nk, "hipMalloc Fk");
+    rc |= ensure(c, (void **)&c.dsc, sizeof(DevScalars), "hipMalloc dsc");
+    rc |= ensure(c, (void **)&c.partials, sizeof(double) * MAX_GRID * NRED, "hipMalloc partials");
+    if (rc) { lat_destroy(h); return -3; }
+    c.twx64 = make_twiddles<double>(c, c.nx, &rc);
+    c.twy64 = make_twiddles<double>(c, c.ny, &rc);
+    c.twz64 = make_twiddles<double>(c, c.nz, &rc);
+    c.twx32 = make_twiddles<float>(c, c.nx, &rc);
+    c.twy32 = make_twiddles<float>(c, c.ny, &rc);
+    c.twz32 = make_twiddles<float>(c, c.nz, &rc);
+    if (rc) { lat_destroy(h); return -3; }
+    memset(&c.host_sc, 0, sizeof(DevScalars));
+    c.host_sc.yscl = 1.0;
+    if (push_scalars(c)) { lat_destroy(h); return -3; }
+    hipMemset(c.fld, 0, es * c.nfld * c.nvol);
+    hipMemset(c.fldp, 0, es * c.nfld * c.nvol);
+    hipMemset(c.partials, 0, sizeof(double) * MAX_GRID * NRED);
+    c.ke_valid = false;
+    *out = h;
+    return 0;
+}
+
+void lat_destroy(lat_ctx *h)
+{
+    if (!h) return;
+    Ctx &c = h->c;
+    hipSetDevice(c.device);
+    if (c.stream) hipStreamSynchronize(c.stream);
+    void *ptrs[] = {c.fld, c.fldp, c.Fk, c.Fk2, c.io64, (c.Fk64 != c.Fk) ? c.Fk64 : nullptr, c.dsc, c.partials,
+                    c.spec_part, c.spec_out, c.twx64, c.twy64, c.twz64, c.twx32, c.twy32, c.twz32};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    if (c.stream) hipStreamDestroy(c.stream);
+    delete h;
+}
+
+const char *lat_last_error(const lat_ctx *h) { return h ? h->c.last_error.c_str() : "null context"; }
+
+int lat_synchronize(lat_ctx *h)
+{
+    CTX_OR_FAIL(h);
+    return sync(c);
+}
+
+int lat_num_bins(int nx, int ny, int nz)
+{
+    // analysis_spec.f90:55-56
+    long ns = (long)(nx / 2 + 1) * (nx / 2 + 1) + (long)(ny / 2 + 1) * (ny / 2 + 1) + (long)(nz / 2 + 1) * (nz / 2 + 1);
+    return (int)std::floor(std::sqrt((double)ns + 1.0));
+}
+
+int lat_fft_r2c_3d(lat_ctx *h, const double *f, double *Fk)
+{
+    CTX_OR_FAIL(h);
+    if (!f || !Fk) return -1;
+    int rc = host_r2c(c, f);
+    if (rc) return rc;
+    if ((rc = fk_to_host(c, (const cpx<double> *)c.Fk64, Fk))) return rc;
+    return sync(c);
+}
+
+int lat_fft_c2r_3d(lat_ctx *h, const double *Fk, double *f)
+{
+    CTX_OR_FAIL(h);
+    if (!f || !Fk) return -1;
+    int rc = ensure_io(c);
+    if (rc) return rc;
+    cpx<double> *F = (cpx<double> *)c.Fk64;
+    if ((rc = fk_from_host(c, F, Fk))) return rc;
+    if ((rc = launch_col<double>(c, c.twz64, F, 2, 1, 1))) return rc;
+    if ((rc = launch_col<double>(c, c.twy64, F, 1, 1, 1))) return rc;
+    rc = launch_x_c2r<double>(c, c.twx64, F, 1, EPI_STORE, c.io64, nullptr, nullptr);
+    if (rc < 0) return rc;
+    LAT_HIP(&c, hipMemcpyAsync(f, c.io64, sizeof(double) * c.nvol, hipMemcpyDeviceToHost, c.stream));
+    return sync(c);
+}
+
+int lat_laplacian_3d(lat_ctx *h, int n1, int n2, int n3, double *f, double dk)
+{
+    CTX_OR_FAIL(h);
+    if (!f) return -1;
+    if (n1 != c.nx || n2 != c.ny || n3 != c.nz) {
+        c.last_error = "lattice size differs from the context";
+        return -2;
+    }
+    int rc = ensure_io(c);
+    if (rc) return rc;
+    cpx<double> *F = (cpx<double> *)c.Fk64;
+    LAT_HIP(&c, hipMemcpyAsync(c.io64, f, sizeof(double) * c.nvol, hipMemcpyHostToDevice, c.stream));
+    if ((rc = launch_x_r2c<double>(c, c.twx64, c.io64, nullptr, F, 0))) return rc;
+    if ((rc = launch_col<double>(c, c.twy64, F, 1, 0, 1))) return rc;
+    const double saved = c.dk;
+    c.dk = dk;
+    rc = launch_col<double>(c, c.twz64, F, 2, 2, 1);
+    c.dk = saved;
+    if (rc) return rc;
+    if ((rc = launch_col<double>(c, c.twy64, F, 1, 1, 1))) return rc;
+    rc = launch_x_c2r<double>(c, c.twx64, F, 1, EPI_STORE, c.io64, nullptr, nullptr);
+    if (rc < 0) return rc;
+    LAT_HIP(&c, hipMemcpyAsync(f, c.io64, sizeof(double) * c.nvol, hipMemcpyDeviceToHost, c.stream));
+    return sync(c);
+}
+
+int lat_gradient_squared_3d(lat_ctx *h, const double *f, double *grad2, double dk)
+{
+    CTX_OR_FAIL(h);
+    if (!f || !grad2) return -1;
+    int rc = host_r2c(c, f);
+    if (rc) return rc;
+    // second FP64 half-complex buffer: upper half of the Fk64 allocation
+    cpx<double> *F = (cpx<double> *)c.Fk64;
+    cpx<double> *G = F + c.nk;
+    double *acc = c.io64 + c.nvol;
+    for (int ax = 2; ax >= 0; --ax) {
+        hipLaunchKernelGGL(k_deriv_mult, dim3(4096), dim3(256), 0, c.stream, F, G, c.nxh, c.P, c.ny, c.nz, ax,
+                           dk / (double)c.nvol);
+        LAT_HIP(&c, hipGetLastError());
+        if ((rc = launch_col<double>(c, c.twz64, G, 2, 1, 1))) return rc;
+        if ((rc = launch_col<double>(c, c.twy64, G, 1, 1, 1))) return rc;
+        rc = launch_x_c2r<double>(c, c.twx64, G, 1, EPI_STORE, c.io64, nullptr, nullptr);
+        if (rc < 0) return rc;
+        hipLaunchKernelGGL(k_sq_acc, dim3(4096), dim3(256), 0, c.stream, c.io64, acc, c.nvol, ax == 2 ? 1 : 0);
+        LAT_HIP(&c, hipGetLastError());
+    }
+    LAT_HIP(&c, hipMemcpyAsync(grad2, acc, sizeof(double) * c.nvol, hipMemcpyDeviceToHost, c.stream));
+    return sync(c);
+}
+
+int lat_stencil_laplacian(lat_ctx *h, const double *f, double *lap)
+{
+    CTX_OR_FAIL(h);
+    if (!f || !lap) return -1;
+    if (c.cfg.stencil[4] == 0.0) {
+        c.last_error = "stencil coefficients not set (cc = 0)";
+        return -2;
+    }
+    int rc = ensure_io(c);
+    if (rc) return rc;
+    LAT_HIP(&c, hipMemcpyAsync(c.io64, f, sizeof(double) * c.nvol, hipMemcpyHostToDevice, c.stream));
+    if ((rc = launch_stencil_io(c, c.io64, c.io64 + c.nvol))) return rc;
+    LAT_HIP(&c, hipMemcpyAsync(lap, c.io64 + c.nvol, sizeof(double) * c.nvol, hipMemcpyDeviceToHost, c.stream));
+    return sync(c);
+}
+
+static int ensure_spec(Ctx &c)
+{
+    int rc = ensure(c, (void **)&c.spec_part, sizeof(double) * 1024 * 5 * c.ns, "hipMalloc spec_part");
+    rc |= ensure(c, (void **)&c.spec_out, sizeof(double) * 5 * c.ns, "hipMalloc spec_out");
+    return rc;
+}
+
+static void finish_spectra(const Ctx &c, const std::vector<double> &raw, int nch, double *const *outs)
+{
+    const int ns = c.ns;
+    const double *W = raw.data() + (size_t)nch * ns;
+    for (int ch = 0; ch < nch; ++ch)
+        for (int b = 0; b < ns; ++b) {
+            double s = raw[(size_t)ch * ns + b];
+            if (W[b] != 0.0) s = s / W[b]; // analysis_spec.f90:101
+            outs[ch][b] = s;
+        }
+}
+
+int lat_spectrum_3d(lat_ctx *h, double *S, const double *f, double *Fk)
+{
+    CTX_OR_FAIL(h);
+    if (!S || !f) return -1;
+    int rc = ensure_spec(c);
+    if (rc) return rc;
+    if ((rc = host_r2c(c, f))) return rc;
+    const cpx<double> *F = (const cpx<double> *)c.Fk64;
+    if ((rc = launch_bin<double>(c, F, F, 0)) < 0) return rc;
+    std::vector<double> raw(2 * c.ns);
+    LAT_HIP(&c, hipMemcpyAsync(raw.data(), c.spec_out, sizeof(double) * 2 * c.ns, hipMemcpyDeviceToHost, c.stream));
+    if (Fk && (rc = fk_to_host(c, F, Fk))) return rc;
+    if ((rc = sync(c))) return rc;
+    double *outs[1] = {S};
+    finish_spectra(c, raw, 1, outs);
+    return 0;
+}
+
+int lat_crossspec_3d(lat_ctx *h, const double *Fk1, const double *Fk2, double *S_re, double *S_im)
+{
+    CTX_OR_FAIL(h);
+    if (!Fk1 || !Fk2 || !S_re || !S_im) return -1;
+    int rc = ensure_spec(c);
+    if (rc) return rc;
+    if ((rc = ensure_io(c))) return rc;
+    cpx<double> *F1 = (cpx<double> *)c.Fk64;
+    cpx<double> *F2 = F1 + c.nk;
+    if ((rc = fk_from_host(c, F1, Fk1))) return rc;
+    if ((rc = fk_from_host(c, F2, Fk2))) return rc;
+    if ((rc = launch_bin<double>(c, F1, F2, 1)) < 0) return rc;
+    std::vector<double> raw(3 * c.ns);
+    LAT_HIP(&c, hipMemcpyAsync(raw.data(), c.spec_out, sizeof(double) * 3 * c.ns, hipMemcpyDeviceToHost, c.stream));
+    if ((rc = sync(c))) return rc;
+    double *outs[2] = {S_re, S_im};
+    finish_spectra(c, raw, 2, outs);
+    return 0;
+}
+
+} // extern "C"
+
+template <typename T>
+static int output_spectra_t(Ctx &c, int field, double *out)
+{
+    const cpx<T> *twx = (const cpx<T> *)(sizeof(T) == 8 ? (void *)c.twx64 : (void *)c.twx32);
+    const cpx<T> *twy = (const cpx<T> *)(sizeof(T) == 8 ? (void *)c.twy64 : (void *)c.twy32);
+    const cpx<T> *twz = (const cpx<T> *)(sizeof(T) == 8 ? (void *)c.twz64 : (void *)c.twz32);
+    int rc = ensure(c, &c.Fk, 2 * sizeof(cpx<T>) * c.nk, "hipMalloc Fk");
+    if (rc) return rc;
+    cpx<T> *Fphi = (cpx<T> *)c.Fk;
+    cpx<T> *Fpi = Fphi + c.nk;
+    T *fld = (T *)c.fld + (size_t)field * c.nvol;
+    T *fldp = (T *)c.fldp + (size_t)field * c.nvol;
+    if ((rc = launch_x_r2c<T>(c, twx, fld, nullptr, Fphi, 0))) return rc;
+    if ((rc = launch_x_r2c<T>(c, twx, fldp, nullptr, Fpi, 0))) return rc;
+    if ((rc = launch_col<T>(c, twy, Fphi, 1, 0, 2))) return rc;
+    if ((rc = launch_col<T>(c, twz, Fphi, 2, 0, 2))) return rc;
+    // crossspec_3d(Fk = pi^, Fk2 = phi^) as called at evolve_spectral.f90:534-538
+    if ((rc = launch_bin<T>(c, Fpi, Fphi, 2)) < 0) return rc;
+    std::vector<double> raw(5 * c.ns);
+    LAT_HIP(&c, hipMemcpyAsync(raw.data(), c.spec_out, sizeof(double) * 5 * c.ns, hipMemcpyDeviceToHost, c.stream));
+    if ((rc = sync(c))) return rc;
+    double *outs[4] = {out, out + c.ns, out + 2 * c.ns, out + 3 * c.ns};
+    finish_spectra(c, raw, 4, outs);
+    return 0;
+}
+
+extern "C" {
+
+int lat_output_spectra(lat_ctx *h, int field, double *out)
+{
+    CTX_OR_FAIL(h);
+    if (!out || field < 0 || field >= c.nfld) return -1;
+    int rc = ensure_spec(c);
+    if (rc) return rc;
+    return c.prec == LAT_F64 ? output_spectra_t<double>(c, field, out) : output_spectra_t<float>(c, field, out);
+}
+
+int lat_set_state(lat_ctx *h, const double *fld, const double *fldp, int ghost, double yscl, double ysclp)
+{
+    CTX_OR_FAIL(h);
+    if (!fld || !fldp || ghost < 0 || ghost > 1) return -1;
+    const int g = ghost, nf = c.nfld;
+    const size_t sx = c.nx + 2 * g, sy = c.ny + 2 * g;
+    std::vector<double> soa((size_t)nf * c.nvol), soap((size_t)nf * c.nvol);
+    // fld carries the ghost layer (hamiltonian_conformal.f90:22 SIRANGE), fldp never does (:23 IRANGE)
+    for (int k = 0; k < c.nz; ++k)
+        for (int j = 0; j < c.ny; ++j)
+            for (int i = 0; i < c.nx; ++i) {
+                const size_t s = (size_t)i + (size_t)c.nx * ((size_t)j + (size_t)c.ny * k);
+                const size_t gs = (size_t)(i + g) + sx * ((size_t)(j + g) + sy * (size_t)(k + g));
+                for (int l = 0; l < nf; ++l) {
+                    soa[(size_t)l * c.nvol + s] = fld[(size_t)l + (size_t)nf * gs];
+                    soap[(size_t)l * c.nvol + s] = fldp[(size_t)l + (size_t)nf * s];
+                }
+            }
+    int rc = c.prec == LAT_F64 ? upload_fields<double>(c, c.fld, soa.data(), soa.size())
+                               : upload_fields<float>(c, c.fld, soa.data(), soa.size());
+    if (rc) return rc;
+    rc = c.prec == LAT_F64 ? upload_fields<double>(c, c.fldp, soap.data(), soap.size())
+                           : upload_fields<float>(c, c.fldp, soap.data(), soap.size());
+    if (rc) return rc;
+    if ((rc = pull_scalars(c))) return rc;
+    c.host_sc.yscl = yscl;
+    c.host_sc.ysclp = ysclp;
+    c.host_sc.nd = 0;
+    c.ke_valid = false;
+    return push_scalars(c);
+}
+
+int lat_get_state(lat_ctx *h, double *fld, double *fldp, int ghost, double *yscl, double *ysclp)
+{
+    CTX_OR_FAIL(h);
+    if (ghost < 0 || ghost > 1) return -1;
+    int rc = sync(c);
+    if (rc) return rc;
+    const int g = ghost, nf = c.nfld;
+    const size_t sx = c.nx + 2 * g, sy = c.ny + 2 * g, sz = c.nz + 2 * g;
+    std::vector<double> soa((size_t)nf * c.nvol), soap((size_t)nf * c.nvol);
+    if (fld) {
+        rc = c.prec == LAT_F64 ? download_fields<double>(c, soa.data(), c.fld, soa.size())
+                               : download_fields<float>(c, soa.data(), c.fld, soa.size());
+        if (rc) return rc;
+        for (size_t kk = 0; kk < sz; ++kk)
+            for (size_t jj = 0; jj < sy; ++jj)
+                for (size_t ii = 0; ii < sx; ++ii) {
+                    // periodic image for ghost cells (wrap_fields, hamiltonian_conformal.f90:309-330)
+                    const int i = (int)((ii - g + c.nx) % c.nx), j = (int)((jj - g + c.ny) % c.ny),
+                              k = (int)((kk - g + c.nz) % c.nz);
+                    const size_t s = (size_t)i + (size_t)c.nx * ((size_t)j + (size_t)c.ny * k);
+                    const size_t gs = ii + sx * (jj + sy * kk);
+                    for (int l = 0; l < nf; ++l) fld[(size_t)l + (size_t)nf * gs] = soa[(size_t)l * c.nvol + s];
+                }
+    }
+    if (fldp) {
+        rc = c.prec == LAT_F64 ? download_fields<double>(c, soap.data(), c.fldp, soap.size())
+                               : download_fields<float>(c, soap.data(), c.fldp, soap.size());
+        if (rc) return rc;
+        for (size_t s = 0; s < c.nvol; ++s)
+            for (int l = 0; l < nf; ++l) fldp[(size_t)l + (size_t)nf * s] = soap[(size_t)l * c.nvol + s];
+    }
+    if ((rc = pull_scalars(c))) return rc;
+    if (yscl) *yscl = c.host_sc.yscl;
+    if (ysclp) *ysclp = c.host_sc.ysclp;
+    return 0;
+}
+
+int lat_get_scalars(lat_ctx *h, double *yscl, double *ysclp)
+{
+    CTX_OR_FAIL(h);
+    int rc = pull_scalars(c);
+    if (rc) return rc;
+    if (yscl) *yscl = c.host_sc.yscl;
+    if (ysclp) *ysclp = c.host_sc.ysclp;
+    return 0;
+}
+
+int lat_set_scalars(lat_ctx *h, double yscl, double ysclp)
+{
+    CTX_OR_FAIL(h);
+    int rc = pull_scalars(c);
+    if (rc) return rc;
+    c.host_sc.yscl = yscl;
+    c.host_sc.ysclp = ysclp;
+    return push_scalars(c);
+}
+
+int lat_split(lat_ctx *h, double dt, int hamind)
+{
+    CTX_OR_FAIL(h);
+    if (hamind < 1 || hamind > 3) {
+        c.last_error = "Undefined Hamiltonian term"; // hamiltonian_conformal.f90:78
+        return -1;
+    }
+    std::vector<Op> ops;
+    map_split(c, ops, dt, hamind);
+    return run_ops(c, ops);
+}
+
+int lat_symp_o2step(lat_ctx *h, double dt, double w1, double w2)
+{
+    CTX_OR_FAIL(h);
+    std::vector<Op> ops;
+    o2step(c, ops, dt, w1, w2);
+    return run_ops(c, ops);
+}
+
+int lat_symp(lat_ctx *h, int order, double dt, int nsteps)
+{
+    CTX_OR_FAIL(h);
+    std::vector<Op> ops;
+    if (build_symp(c, ops, order, dt, nsteps)) {
+        c.last_error = "symp order must be 2, 4, 6 or 8";
+        return -1;
+    }
+    return run_ops(c, ops);
+}
+
+int lat_calc_metric(lat_ctx *h)
+{
+    CTX_OR_FAIL(h);
+    int rc = state_sums(c);
+    if (rc) return rc;
+    const double nv = (double)c.nvol, y = c.host_sc.yscl;
+    const double a = scale_factor(c, y);
+    const bool conf = c.cfg.time_var == LAT_CONFORMAL;
+    const double knorm = conf ? 1.0 / std::pow(y, 6) : 1.0 / std::pow(y, 4);
+    const double KE = 0.5 * c.host_sc.red[RC_KE] * knorm / nv;
+    const double PE = c.host_sc.red[RC_PE] / nv;
+    const double GE = 0.5 * c.host_sc.red[RC_GE] / (a * a) / nv;
+    const double rho = KE + PE + GE;
+    c.host_sc.ysclp = conf ? -std::sqrt(std::pow(y, 4) * 12.0 * rho) : -a * std::sqrt(rho * 16.0 / 3.0);
+    return push_scalars(c);
+}
+
+int lat_energies(lat_ctx *h, double *out)
+{
+    CTX_OR_FAIL(h);
+    if (!out) return -1;
+    int rc = state_sums(c);
+    if (rc) return rc;
+    const DevScalars &s = c.host_sc;
+    const double nv = (double)c.nvol, y = s.yscl;
+    const double a = scale_factor(c, y);
+    const bool conf = c.cfg.time_var == LAT_CONFORMAL;
+    const double knorm = conf ? 1.0 / std::pow(y, 6) : 1.0 / std::pow(y, 4);
+    const double hub = conf ? -s.ysclp / 6.0 / (y * y) : -s.ysclp / y / 4.0;
+    const double KE = 0.5 * s.red[RC_KE] * knorm / nv;
+    const double PE = s.red[RC_PE] / nv;
+    const double GE = 0.5 * s.red[RC_GE] / nv / (a * a);
+    const double rho = KE + PE + GE;
+    const double grav = -3.0 * hub * hub;
+    out[LAT_E_A] = a;
+    out[LAT_E_HUBBLE] = hub;
+    out[LAT_E_RHO] = rho;
+    out[LAT_E_KE] = KE;
+    out[LAT_E_PE] = PE;
+    out[LAT_E_GE] = GE;
+    out[LAT_E_GRAV] = grav;
+    out[LAT_E_RESID] = (rho + grav) / rho;
+    out[LAT_E_PMOM] = conf ? -s.ysclp * s.ysclp / 12.0 / std::pow(a, 4) : -s.ysclp * s.ysclp * 3.0 / 16.0 / (a * a);
+    out[LAT_E_MEAN1] = s.red[RC_M1] / nv;
+    out[LAT_E_MEAN2] = c.nfld > 1 ? s.red[RC_M2] / nv : 0.0;
+    out[LAT_E_GEGRAD] = 0.0;
+    return 0;
+}
+
+int lat_device_state(lat_ctx *h, void **fld, void **fldp)
+{
+    CTX_OR_FAIL(h);
+    if (fld) *fld = c.fld;
+    if (fldp) *fldp = c.fldp;
+    c.ke_valid = false; // the caller may write the state
+    return 0;
+}
+
+int lat_last_symp_ms(lat_ctx *h, double *ms)
+{
+    CTX_OR_FAIL(h);
+    if (ms) *ms = 0.0;
+    return 0;
+}
+
+} // extern "C"

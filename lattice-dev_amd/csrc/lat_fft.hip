@@ -1,0 +1,434 @@
+// lat_fft.hip -- 3-D R2C/C2R pipeline of the spectral Laplacian on gfx950.
+//
+// The Laplacian of laplacian_3d (Pseudospec/openmp_dev/fftw_mod.f90:177-207)
+// runs as five HBM sweeps per field:
+//   1. x: R2C of each contiguous real row (N/2-point complex FFT of the row
+//      packed as z[n] = f[2n] + i f[2n+1], then the real-FFT split), with the
+//      pending drifts phi += c*pi fused in front (the drift sweep disappears);
+//   2. y: forward complex FFT of tiles of TC adjacent kx columns;
+//   3. z: forward FFT, multiply by -dk^2 (i^2+jj^2+kk^2)/(nx ny nz), inverse FFT;
+//   4. y: inverse;
+//   5. x: C2R of every field's rows, then the kick of all fields at each site
+//      (V' needs all fields) with the GE/PE/KE reductions fused in.
+// Every 1-D transform is LDS-resident (fft_core.h).  The k-space array is
+// FFTW's half-complex layout with the row pitch padded to P >= nx/2+1.
+#include "lat_internal.h"
+
+namespace lat {
+
+// ---------------------------------------------------------------- tiling
+// Grid-stride loop over T tiles with an XCD-aware assignment: hardware hands
+// consecutive workgroup ids to the 8 XCDs round-robin, so block b runs on XCD
+// b%8.  Each XCD takes one contiguous eighth of the tiles, so tiles that share
+// cache lines are processed by the same XCD (same L2) at the same time.
+__device__ __forceinline__ void tile_range(int T, int &start, int &end, int &step)
+{
+    const int G = gridDim.x, b = blockIdx.x;
+    if ((G & 7) == 0 && T >= G) {
+        const int xcd = b & 7, slot = b >> 3, G8 = G >> 3;
+        const int C = (T + 7) >> 3;
+        start = xcd * C + slot;
+        end = min((xcd + 1) * C, T);
+        step = G8;
+    } else {
+        start = b;
+        end = T;
+        step = G;
+    }
+}
+
+int grid_for(int tiles)
+{
+    int g = tiles < MAX_GRID ? tiles : MAX_GRID;
+    if (g >= 8) g &= ~7;
+    return g < 1 ? 1 : g;
+}
+
+__device__ __forceinline__ int wmag(int j0, int n) { return (j0 <= n / 2) ? j0 : n - j0; }
+
+// ---------------------------------------------------------------- block reduction
+template <int NCH>
+__device__ __forceinline__ void block_reduce_store(double (&v)[NCH], double *out)
+{
+    __shared__ double red[NCH][TPB / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        double x = v[c];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        if (lane == 0) red[c][wid] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < NCH) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < TPB / 64; ++w) s += red[threadIdx.x][w];
+        out[threadIdx.x] = s;
+    }
+}
+
+// ---------------------------------------------------------------- model
+template <typename T>
+__device__ __forceinline__ void model_dv(const ModelDev &m, const T *f, T *dv)
+{
+    if (m.model == LAT_MODEL_PHI4) {
+        dv[0] = T(m.p0) * f[0] * f[0] * f[0];
+    } else {
+        dv[0] = T(m.p0) * f[0] + T(m.p1) * f[0] * (f[1] * f[1]);
+        dv[1] = T(m.p1) * (f[0] * f[0]) * f[1];
+    }
+}
+template <typename T>
+__device__ __forceinline__ double model_v(const ModelDev &m, const T *f)
+{
+    const double a = f[0];
+    if (m.model == LAT_MODEL_PHI4) return 0.25 * m.p0 * (a * a) * (a * a);
+    const double b = f[1];
+    return 0.5 * m.p0 * (a * a) + 0.5 * m.p1 * (a * a) * (b * b);
+}
+
+// ---------------------------------------------------------------- x R2C
+template <typename T, int N, int L, bool DRIFT>
+__global__ __launch_bounds__(TPB) void k_x_r2c(T *__restrict__ f, const T *__restrict__ pi,
+                                               cpx<T> *__restrict__ Fk, const cpx<T> *__restrict__ tw,
+                                               int rows, int P, const DevScalars *__restrict__ dsc)
+{
+    constexpr int M = N / 2;
+    constexpr int LS = M + 1;
+    __shared__ cpx<T> buf[L * LS];
+    int nd = 0;
+    T dc0 = 0, dc1 = 0;
+    if constexpr (DRIFT) {
+        nd = dsc->nd;
+        dc0 = T(dsc->dc[0]);
+        dc1 = T(dsc->dc[1]);
+    }
+    int t0, t1, ts;
+    tile_range((rows + L - 1) / L, t0, t1, ts);
+    for (int t = t0; t < t1; t += ts) {
+        const int r0 = t * L;
+        for (int idx = threadIdx.x; idx < L * M; idx += TPB) {
+            const int line = idx / M, n = idx - line * M;
+            const int row = r0 + line;
+            cpx<T> v = {0, 0};
+            if (row < rows) {
+                cpx<T> *src = reinterpret_cast<cpx<T> *>(f + (size_t)row * N) + n;
+                v = *src;
+                if constexpr (DRIFT) {
+                    if (nd > 0) {
+                        const cpx<T> p = reinterpret_cast<const cpx<T> *>(pi + (size_t)row * N)[n];
+                        v.x = v.x + dc0 * p.x;
+                        v.y = v.y + dc0 * p.y;
+                        if (nd > 1) {
+                            v.x = v.x + dc1 * p.x;
+                            v.y = v.y + dc1 * p.y;
+                        }
+                        *src = v;
+                    }
+                }
+            }
+            buf[line * LS + n] = v;
+        }
+        __syncthreads();
+        lds_fft<T, M, -1, L, TPB, false, 2>(buf, LS, 1, tw);
+        // real-FFT split: X[k] = (Z[k] + Z*[M-k])/2 - i W^k (Z[k] - Z*[M-k])/2
+        for (int idx = threadIdx.x; idx < L * (M / 2 + 1); idx += TPB) {
+            const int line = idx / (M / 2 + 1), k = idx - line * (M / 2 + 1);
+            cpx<T> *z = buf + line * LS;
+            const int km = (M - k) % M;
+            const cpx<T> a = z[k], b = z[km];
+            {
+                const cpx<T> w = tw[k];
+                const cpx<T> s = {T(0.5) * (a.x + b.x), T(0.5) * (a.y - b.y)};
+                const cpx<T> d = {T(0.5) * (a.x - b.x), T(0.5) * (a.y + b.y)};
+                const cpx<T> wd = cmul(w, d);
+                z[k] = {s.x + wd.y, s.y - wd.x};
+            }
+            if (k != M - k) {
+                const int k2 = M - k;
+                const cpx<T> w = tw[k2];
+                const cpx<T> s = {T(0.5) * (b.x + a.x), T(0.5) * (b.y - a.y)};
+                const cpx<T> d = {T(0.5) * (b.x - a.x), T(0.5) * (b.y + a.y)};
+                const cpx<T> wd = cmul(w, d);
+                z[k2] = {s.x + wd.y, s.y - wd.x};
+            }
+        }
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < L * (M + 1); idx += TPB) {
+            const int line = idx / (M + 1), k = idx - line * (M + 1);
+            const int row = r0 + line;
+            if (row < rows) Fk[(size_t)row * P + k] = buf[line * LS + k];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- y / z columns
+// MODE 0 forward, 1 inverse, 2 forward * (-k^2 kfac) inverse (z axis only).
+template <typename T, int N, int TC, int MODE>
+__global__ __launch_bounds__(TPB) void k_col(cpx<T> *__restrict__ Fk, const cpx<T> *__restrict__ tw, int nxh,
+                                             int n_other, size_t s_axis, size_t s_other, size_t s_arr,
+                                             int n_arr, int ny, double kfac)
+{
+    __shared__ cpx<T> buf[TC * N];
+    const int nct = (nxh + TC - 1) / TC;
+    int t0, t1, ts;
+    tile_range(nct * n_other * n_arr, t0, t1, ts);
+    for (int t = t0; t < t1; t += ts) {
+        const int ct = t % nct;
+        const int rest = t / nct;
+        const int o = rest % n_other, arr = rest / n_other;
+        cpx<T> *base = Fk + (size_t)arr * s_arr + (size_t)o * s_other + (size_t)ct * TC;
+        const int ncol = min(TC, nxh - ct * TC);
+        for (int idx = threadIdx.x; idx < TC * N; idx += TPB) {
+            const int c = idx % TC, n = idx / TC;
+            cpx<T> v = {0, 0};
+            if (c < ncol) v = base[(size_t)n * s_axis + c];
+            buf[idx] = v;
+        }
+        __syncthreads();
+        if constexpr (MODE == 1) {
+            lds_fft<T, N, +1, TC, TPB, true, 1>(buf, 1, TC, tw);
+        } else {
+            lds_fft<T, N, -1, TC, TPB, true, 1>(buf, 1, TC, tw);
+        }
+        if constexpr (MODE == 2) {
+            const int jj = wmag(o, ny);
+            for (int idx = threadIdx.x; idx < TC * N; idx += TPB) {
+                const int c = idx % TC, n = idx / TC;
+                const int ii = ct * TC + c;
+                const int kk = wmag(n, N);
+                const double rad2 = double(ii * ii) + double(jj * jj) + double(kk * kk);
+                const T m = T(-rad2 * kfac);
+                cpx<T> v = buf[idx];
+                buf[idx] = {v.x * m, v.y * m};
+            }
+            __syncthreads();
+            lds_fft<T, N, +1, TC, TPB, true, 1>(buf, 1, TC, tw);
+        }
+        for (int idx = threadIdx.x; idx < TC * N; idx += TPB) {
+            const int c = idx % TC, n = idx / TC;
+            if (c < ncol) base[(size_t)n * s_axis + c] = buf[idx];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- x C2R + epilogue
+template <typename T, int N, int L, int EPI>
+__global__ __launch_bounds__(TPB) void k_x_c2r(const cpx<T> *__restrict__ Fk, const cpx<T> *__restrict__ tw,
+                                               int rows, int P, size_t nk, size_t nvol, int nf, T *__restrict__ out,
+                                               const T *__restrict__ fld, T *__restrict__ fldp,
+                                               const DevScalars *__restrict__ dsc, ModelDev md,
+                                               double *__restrict__ partials)
+{
+    constexpr int M = N / 2;
+    constexpr int LS = M + 1;
+    constexpr int MAXF = 2;
+    __shared__ cpx<T> buf[MAXF * L * LS];
+    double acc[NRED] = {0, 0, 0, 0, 0, 0, 0, 0}; // RedChan order
+    T hA = 0, hB = 0;
+    if constexpr (EPI == EPI_KICK) {
+        hA = T(dsc->hA);
+        hB = T(dsc->hB);
+    }
+    int t0, t1, ts;
+    tile_range((rows + L - 1) / L, t0, t1, ts);
+    for (int t = t0; t < t1; t += ts) {
+        const int r0 = t * L;
+        for (int idx = threadIdx.x; idx < nf * L * (M + 1); idx += TPB) {
+            const int fl = idx / (M + 1), k = idx - fl * (M + 1);
+            const int fi = fl / L, line = fl - fi * L;
+            const int row = r0 + line;
+            cpx<T> v = {0, 0};
+            if (row < rows) v = Fk[(size_t)fi * nk + (size_t)row * P + k];
+            buf[fl * LS + k] = v;
+        }
+        __syncthreads();
+        // Hermitian pre-split: Z[k] = (X[k] + X*[M-k]) + i e^{+2 pi i k/N} (X[k] - X*[M-k]),
+        // with Im X[0] = Im X[M] = 0 (imaginary parts of the x-DC and x-Nyquist
+        // coefficients are discarded, as numpy irfftn / FFTW c2r rows do).
+        for (int idx = threadIdx.x; idx < nf * L * (M / 2 + 1); idx += TPB) {
+            const int fl = idx / (M / 2 + 1), k = idx - fl * (M / 2 + 1);
+            cpx<T> *z = buf + fl * LS;
+            cpx<T> a = z[k], b = z[M - k];
+            if (k == 0) { a.y = 0; b.y = 0; }
+            {
+                cpx<T> w = tw[k];
+                w.y = -w.y; // e^{+2 pi i k/N}
+                const cpx<T> s = {a.x + b.x, a.y - b.y};
+                const cpx<T> d = {a.x - b.x, a.y + b.y};
+                const cpx<T> wd = cmul(w, d);
+                z[k] = {s.x - wd.y, s.y + wd.x};
+            }
+            if (k != 0 && k != M - k) {
+                const int k2 = M - k;
+                cpx<T> w = tw[k2];
+                w.y = -w.y;
+                const cpx<T> s = {b.x + a.x, b.y - a.y};
+                const cpx<T> d = {b.x - a.x, b.y + a.y};
+                const cpx<T> wd = cmul(w, d);
+                z[k2] = {s.x - wd.y, s.y + wd.x};
+            }
+        }
+        __syncthreads();
+        if (nf == 2) lds_fft<T, M, +1, 2 * L, TPB, false, 2>(buf, LS, 1, tw);
+        else lds_fft<T, M, +1, L, TPB, false, 2>(buf, LS, 1, tw);
+        for (int idx = threadIdx.x; idx < L * N; idx += TPB) {
+            const int line = idx / N, i = idx - line * N;
+            const int row = r0 + line;
+            if (row >= rows) continue;
+            const size_t site = (size_t)row * N + i;
+            T lap[MAXF];
+            for (int fi = 0; fi < nf; ++fi)
+                lap[fi] = reinterpret_cast<const T *>(buf + (fi * L + line) * LS)[i];
+            if constexpr (EPI == EPI_STORE) {
+                for (int fi = 0; fi < nf; ++fi) out[(size_t)fi * nvol + site] = lap[fi];
+            } else {
+                T phi[MAXF] = {0, 0}, dv[MAXF] = {0, 0};
+                for (int fi = 0; fi < nf; ++fi) phi[fi] = fld[(size_t)fi * nvol + site];
+                if constexpr (EPI == EPI_KICK) model_dv(md, phi, dv);
+                acc[RC_PE] += model_v(md, phi);
+                acc[RC_M1] += double(phi[0]);
+                acc[RC_M2] += double(phi[1]);
+                for (int fi = 0; fi < nf; ++fi) {
+                    T p = fldp[(size_t)fi * nvol + site];
+                    if constexpr (EPI == EPI_KICK) {
+                        p = p + (hA * lap[fi] - hB * dv[fi]);
+                        fldp[(size_t)fi * nvol + site] = p;
+                    }
+                    acc[RC_GE] -= double(phi[fi]) * double(lap[fi]);
+                    acc[RC_KE] += double(p) * double(p);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if constexpr (EPI != EPI_STORE) block_reduce_store<NRED>(acc, partials + (size_t)blockIdx.x * NRED);
+}
+
+// ---------------------------------------------------------------- launch helpers
+template <typename T>
+constexpr int elems_budget() { return sizeof(T) == 8 ? 1024 : 2048; }
+
+template <typename T, int N>
+constexpr int row_lines() // rows per workgroup for the x kernels
+{
+    return (elems_budget<T>() / (N / 2)) < 1 ? 1 : (elems_budget<T>() / (N / 2));
+}
+
+template <typename T, int N>
+constexpr int col_tile() // adjacent kx columns per workgroup for the y/z kernels
+{
+    constexpr int lim = (sizeof(T) == 8 ? 4096 : 8192) / N;
+    constexpr int mn = sizeof(T) == 8 ? 4 : 8;
+    return lim < mn ? mn : (lim > 64 ? 64 : lim);
+}
+
+#define LAT_FOR_SIZES(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(48) X(96) X(192)
+
+template <typename T>
+int launch_x_r2c(Ctx &c, const cpx<T> *tw, T *f, const T *pi, cpx<T> *Fk, int drift)
+{
+    const int rows = c.ny * c.nz;
+    switch (c.nx) {
+#define CASE(NN)                                                                                         \
+    case NN: {                                                                                           \
+        constexpr int L = row_lines<T, NN>();                                                            \
+        const int g = grid_for((rows + L - 1) / L);                                                      \
+        if (drift)                                                                                       \
+            hipLaunchKernelGGL((k_x_r2c<T, NN, L, true>), dim3(g), dim3(TPB), 0, c.stream, f, pi, Fk, tw, \
+                               rows, c.P, c.dsc);                                                        \
+        else                                                                                             \
+            hipLaunchKernelGGL((k_x_r2c<T, NN, L, false>), dim3(g), dim3(TPB), 0, c.stream, f, pi, Fk, tw, \
+                               rows, c.P, c.dsc);                                                        \
+        break;                                                                                           \
+    }
+        LAT_FOR_SIZES(CASE)
+#undef CASE
+    default:
+        c.last_error = "unsupported nx";
+        return -2;
+    }
+    LAT_HIP(&c, hipGetLastError());
+    return 0;
+}
+
+template <typename T>
+int launch_col(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int axis, int mode, int n_arrays)
+{
+    const int n = axis == 1 ? c.ny : c.nz;
+    const int n_other = axis == 1 ? c.nz : c.ny;
+    const size_t s_axis = axis == 1 ? (size_t)c.P : (size_t)c.P * c.ny;
+    const size_t s_other = axis == 1 ? (size_t)c.P * c.ny : (size_t)c.P;
+    const double kfac = c.dk * c.dk / (double)c.nvol;
+    switch (n) {
+#define CASE(NN)                                                                                          \
+    case NN: {                                                                                            \
+        constexpr int TC = col_tile<T, NN>();                                                             \
+        const int g = grid_for(((c.nxh + TC - 1) / TC) * n_other * n_arrays);                              \
+        if (mode == 0)                                                                                    \
+            hipLaunchKernelGGL((k_col<T, NN, TC, 0>), dim3(g), dim3(TPB), 0, c.stream, Fk, tw, c.nxh, n_other, \
+                               s_axis, s_other, c.nk, n_arrays, c.ny, kfac);                              \
+        else if (mode == 1)                                                                               \
+            hipLaunchKernelGGL((k_col<T, NN, TC, 1>), dim3(g), dim3(TPB), 0, c.stream, Fk, tw, c.nxh, n_other, \
+                               s_axis, s_other, c.nk, n_arrays, c.ny, kfac);                              \
+        else                                                                                              \
+            hipLaunchKernelGGL((k_col<T, NN, TC, 2>), dim3(g), dim3(TPB), 0, c.stream, Fk, tw, c.nxh, n_other, \
+                               s_axis, s_other, c.nk, n_arrays, c.ny, kfac);                              \
+        break;                                                                                            \
+    }
+        LAT_FOR_SIZES(CASE)
+#undef CASE
+    default:
+        c.last_error = "unsupported ny/nz";
+        return -2;
+    }
+    LAT_HIP(&c, hipGetLastError());
+    return 0;
+}
+
+template <typename T>
+int launch_x_c2r(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int nf, int epi, T *out, const T *fld, T *fldp)
+{
+    const int rows = c.ny * c.nz;
+    ModelDev md{c.cfg.model, c.nfld, c.cfg.model_params[0], c.cfg.model_params[1]};
+    if (nf < 1 || nf > 2) {
+        c.last_error = "nf must be 1 or 2";
+        return -1;
+    }
+    switch (c.nx) {
+#define CASE(NN)                                                                                         \
+    case NN: {                                                                                           \
+        constexpr int L = row_lines<T, NN>();                                                            \
+        const int g = grid_for((rows + L - 1) / L);                                                      \
+        if (epi == EPI_STORE)                                                                            \
+            hipLaunchKernelGGL((k_x_c2r<T, NN, L, EPI_STORE>), dim3(g), dim3(TPB), 0, c.stream, Fk, tw, rows, \
+                               c.P, c.nk, c.nvol, nf, out, fld, fldp, c.dsc, md, c.partials);           \
+        else if (epi == EPI_KICK)                                                                        \
+            hipLaunchKernelGGL((k_x_c2r<T, NN, L, EPI_KICK>), dim3(g), dim3(TPB), 0, c.stream, Fk, tw, rows, \
+                               c.P, c.nk, c.nvol, nf, out, fld, fldp, c.dsc, md, c.partials);           \
+        else                                                                                             \
+            hipLaunchKernelGGL((k_x_c2r<T, NN, L, EPI_REDUCE>), dim3(g), dim3(TPB), 0, c.stream, Fk, tw, rows, \
+                               c.P, c.nk, c.nvol, nf, out, fld, fldp, c.dsc, md, c.partials);           \
+        LAT_HIP(&c, hipGetLastError());                                                                  \
+        return g;                                                                                        \
+    }
+        LAT_FOR_SIZES(CASE)
+#undef CASE
+    default:
+        c.last_error = "unsupported nx";
+        return -2;
+    }
+}
+
+template int launch_x_r2c<double>(Ctx &, const cpx<double> *, double *, const double *, cpx<double> *, int);
+template int launch_x_r2c<float>(Ctx &, const cpx<float> *, float *, const float *, cpx<float> *, int);
+template int launch_col<double>(Ctx &, const cpx<double> *, cpx<double> *, int, int, int);
+template int launch_col<float>(Ctx &, const cpx<float> *, cpx<float> *, int, int, int);
+template int launch_x_c2r<double>(Ctx &, const cpx<double> *, cpx<double> *, int, int, double *, const double *,
+                                  double *);
+template int launch_x_c2r<float>(Ctx &, const cpx<float> *, cpx<float> *, int, int, float *, const float *, float *);
+
+} // namespace lat

@@ -1,0 +1,143 @@
+// lat_internal.h -- shared definitions of the HIP library (not a public header).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/lattice_c.h"
+#include "fft_core.h"
+
+namespace lat {
+
+constexpr int TPB = 256;       // threads per workgroup for all lattice kernels
+constexpr int MAX_GRID = 4096; // cap of grid-stride launches (= max reduction partials)
+constexpr int NRED = 8;        // reduction channels per partial
+
+// Reduction channels written by the sweep kernels (FP64 partial per block).
+enum RedChan {
+    RC_GE = 0,   // sum over fields and sites of -phi * lap_raw (lap without a factors)
+    RC_PE = 1,   // sum of V(phi) (mixed form: sum of V(phi) - V(mean))
+    RC_KE = 2,   // sum over fields of pi^2 (mixed form: of dpi^2)
+    RC_S1 = 3,   // mixed form: sum of dpi for field 0
+    RC_S2 = 4,   // mixed form: sum of dpi for field 1
+    RC_M1 = 5,   // sum of phi (or dphi) field 0
+    RC_M2 = 6,   // sum of phi (or dphi) field 1
+    RC_X = 7,    // auxiliary
+};
+
+// Background (FRW) scalars and step coefficients, resident in device memory.
+struct DevScalars {
+    double yscl, ysclp;       // a (conformal) or y = a^{3/2} (cosmic), and its momentum
+    double red[NRED];         // last completed reduction (sums, unnormalised)
+    double kick_h;            // h of the kick whose p-update is pending
+    double hA, hB;            // next kick: pi += hA*lap_raw - hB*V'
+    double dc[4];             // pending drift coefficients phi += dc[q]*pi, in order
+    int nd;                   // number of pending drifts
+    int pad_;
+    double mean_phi[4];       // mixed form: FP64 homogeneous parts of phi, pi
+    double mean_pi[4];
+    double dv_mean[4];        // mixed form: V'(mean) used by the last kick
+};
+
+enum ScalarOpType {
+    OP_NONE = 0,
+    OP_FINISH_KICK = 1, // p -= ... using red[], kick_h
+    OP_DRIFT = 2,       // record drift coefficient, p += h K2 / y^3
+    OP_GRAV = 3,        // y -= ...
+    OP_PREP_KICK = 4,   // hA, hB for a kick of size h; kick_h = h
+    OP_CLEAR_DRIFT = 5, // nd = 0
+};
+
+constexpr int MAX_OPS = 12;
+struct ScalarOps {
+    int n;
+    int type[MAX_OPS];
+    double h[MAX_OPS];
+};
+
+struct ModelDev {
+    int model;  // LAT_MODEL_*
+    int nfld;
+    double p0, p1; // phi4: lambda; twofield: m2, g2
+};
+
+struct Ctx {
+    lat_config cfg;
+    int nx, ny, nz, nxh, P; // P = pitch (complex elements) of a half-complex row
+    size_t nvol, nk;        // nk = P*ny*nz
+    double dx, dk;
+    int prec;               // LAT_F64 or LAT_MIXED
+    int nfld;
+    hipStream_t stream;
+    int device;
+
+    void *fld, *fldp;       // nfld * nvol of T (state)
+    void *Fk;               // nfld * nk of cpx<T>
+    void *Fk2;              // nk of cpx<T>, spectra second operand
+    double *io64;           // nvol doubles: host-interface staging (FP64)
+    void *Fk64;             // nk of cpx<double>: FP64 drop-in transforms
+    DevScalars *dsc;
+    double *partials;       // MAX_GRID * NRED
+    double *spec_part;      // spectrum block partials
+    double *spec_out;       // 5 * ns reduced spectrum channels
+    int ns;
+    int spec_blocks;
+
+    // twiddle tables w[m] = exp(-2 pi i m / n) for n = nx, ny, nz
+    cpx<double> *twx64, *twy64, *twz64;
+    cpx<float> *twx32, *twy32, *twz32;
+
+    // host mirror (valid after sync_scalars_to_host)
+    DevScalars host_sc;
+    bool ke_valid; // red[RC_KE] holds sum pi^2 of the current state
+    std::string last_error;
+};
+
+size_t elem_size(const Ctx &c);
+
+// ---- FFT stage launchers (lat_fft.hip). T = double or float. ----
+// x-direction R2C of each real row of `f` (nvol), into half-complex rows of Fk.
+// If drift != 0, first applies the pending drifts of dsc to f in place using pi.
+template <typename T>
+int launch_x_r2c(Ctx &c, const cpx<T> *tw, T *f, const T *pi, cpx<T> *Fk, int drift);
+// Column FFT along y (axis 1) or z (axis 2) of a half-complex array.
+// mode 0: forward; 1: inverse; 2: forward, multiply by -k^2 dk^2/nvol, inverse.
+template <typename T>
+int launch_col(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int axis, int mode, int n_arrays);
+// x-direction C2R of nfld half-complex arrays followed by an epilogue:
+//   EPI_STORE:  write the real rows into out (one array per field)
+//   EPI_KICK:   kick pi with the device coefficients; reduce GE, PE, KE'
+//   EPI_REDUCE: reduce GE, PE, KE (no state change)
+enum { EPI_STORE = 0, EPI_KICK = 1, EPI_REDUCE = 2 };
+template <typename T>
+int launch_x_c2r(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int nf, int epi, T *out, const T *fld, T *fldp);
+
+// ---- step kernels (lat_step.hip) ----
+template <typename T>
+int launch_drift(Ctx &c);              // apply pending drifts of dsc to all fields
+template <typename T>
+int launch_reduce_state(Ctx &c);        // partials: PE, KE, means (no Laplacian)
+template <typename T>
+int launch_stencil(Ctx &c, int epi);    // FD Laplacian + kick / reduce epilogue
+int launch_scalar(Ctx &c, const ScalarOps &ops, int npart);
+int grid_for(int tiles);
+
+// ---- spectra (lat_spec.hip) ----
+template <typename T>
+int launch_bin(Ctx &c, const cpx<T> *F1, const cpx<T> *F2, int nch);
+
+// ---- helpers ----
+int hip_fail(Ctx *c, hipError_t e, const char *where);
+#define LAT_HIP(ctx, call)                                                     \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess) return hip_fail((ctx), e_, #call);               \
+    } while (0)
+
+} // namespace lat
+
+struct lat_ctx {
+    lat::Ctx c;
+};

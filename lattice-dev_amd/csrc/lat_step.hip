@@ -1,0 +1,355 @@
+// lat_step.hip -- drift, finite-difference kick, reductions and the device
+// update of the FRW background scalars.
+//
+// Every split sub-step of the reference ends in a lattice-wide FP64 reduction
+// feeding a scalar update that the next sub-step reads
+// (hamiltonian_conformal.f90:116-117, 203-205; hamiltonian_cosmic.f90:125-126,
+// 209-211).  Here the sweep kernels write per-workgroup FP64 partials and one
+// single-workgroup kernel (k_scalar) reduces them in a fixed order and applies
+// the scalar part of the split operators between two kicks, so a whole
+// Yoshida step runs without a host round trip.
+#include "lat_internal.h"
+
+namespace lat {
+
+template <int NCH>
+__device__ __forceinline__ void block_reduce_store2(double (&v)[NCH], double *out)
+{
+    __shared__ double red[NCH][TPB / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        double x = v[c];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        if (lane == 0) red[c][wid] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < NCH) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < TPB / 64; ++w) s += red[threadIdx.x][w];
+        out[threadIdx.x] = s;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void mdv(const ModelDev &m, const T *f, T *dv)
+{
+    if (m.model == LAT_MODEL_PHI4) {
+        dv[0] = T(m.p0) * f[0] * f[0] * f[0];
+    } else {
+        dv[0] = T(m.p0) * f[0] + T(m.p1) * f[0] * (f[1] * f[1]);
+        dv[1] = T(m.p1) * (f[0] * f[0]) * f[1];
+    }
+}
+template <typename T>
+__device__ __forceinline__ double mv(const ModelDev &m, const T *f)
+{
+    const double a = f[0];
+    if (m.model == LAT_MODEL_PHI4) return 0.25 * m.p0 * (a * a) * (a * a);
+    const double b = f[1];
+    return 0.5 * m.p0 * (a * a) + 0.5 * m.p1 * (a * a) * (b * b);
+}
+
+// ---------------------------------------------------------------- drift
+// phi_l += dc[q] * pi_l for the pending drifts q < nd (hamiltonian_conformal.f90:98,
+// hamiltonian_cosmic.f90:107); the p-update was done by k_scalar.
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_drift(T *__restrict__ f, const T *__restrict__ p, size_t n,
+                                               const DevScalars *__restrict__ dsc)
+{
+    const int nd = dsc->nd;
+    T dc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) dc[q] = T(dsc->dc[q]);
+    for (size_t i = blockIdx.x * (size_t)TPB + threadIdx.x; i < n; i += (size_t)gridDim.x * TPB) {
+        T x = f[i];
+        const T pi = p[i];
+        for (int q = 0; q < nd; ++q) x = x + dc[q] * pi;
+        f[i] = x;
+    }
+}
+
+template <typename T>
+int launch_drift(Ctx &c)
+{
+    const size_t n = (size_t)c.nfld * c.nvol;
+    int g = (int)((n + TPB - 1) / TPB);
+    if (g > 8192) g = 8192;
+    hipLaunchKernelGGL((k_drift<T>), dim3(g), dim3(TPB), 0, c.stream, (T *)c.fld, (const T *)c.fldp, n, c.dsc);
+    LAT_HIP(&c, hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------------------- state reductions
+// partials: PE = sum V, KE = sum pi^2, M1/M2 = sum phi_l (no Laplacian).
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_reduce_state(const T *__restrict__ f, const T *__restrict__ p, size_t nvol,
+                                                      int nf, ModelDev md, double *__restrict__ partials)
+{
+    double acc[NRED] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (size_t i = blockIdx.x * (size_t)TPB + threadIdx.x; i < nvol; i += (size_t)gridDim.x * TPB) {
+        T phi[2] = {0, 0};
+        for (int l = 0; l < nf; ++l) {
+            phi[l] = f[(size_t)l * nvol + i];
+            const double pp = p[(size_t)l * nvol + i];
+            acc[RC_KE] += pp * pp;
+        }
+        acc[RC_PE] += mv(md, phi);
+        acc[RC_M1] += double(phi[0]);
+        acc[RC_M2] += double(phi[1]);
+    }
+    block_reduce_store2<NRED>(acc, partials + (size_t)blockIdx.x * NRED);
+}
+
+template <typename T>
+int launch_reduce_state(Ctx &c)
+{
+    ModelDev md{c.cfg.model, c.nfld, c.cfg.model_params[0], c.cfg.model_params[1]};
+    const size_t blocks = (c.nvol + TPB - 1) / TPB;
+    int g = grid_for(blocks > (size_t)MAX_GRID ? MAX_GRID : (int)blocks);
+    hipLaunchKernelGGL((k_reduce_state<T>), dim3(g), dim3(TPB), 0, c.stream, (const T *)c.fld, (const T *)c.fldp,
+                       c.nvol, c.nfld, md, c.partials);
+    LAT_HIP(&c, hipGetLastError());
+    return g;
+}
+
+// ---------------------------------------------------------------- FD stencil
+// lap_raw = STENCIL(c)/(cc dx^2) with periodic indexing (macros.h:26-33 and
+// wrap_fields, hamiltonian_conformal.f90:309-330); the a-dependence of the
+// reference's b-coefficients (hamiltonian_conformal.f90:177-179,
+// hamiltonian_cosmic.f90:185-187) lives in the kick coefficients hA.
+// EPI_KICK: pi += hA lap - hB V', reduce GE, PE, KE'.  EPI_REDUCE: reduce only.
+// EPI_STORE: write lap_raw of field 0 into out.
+struct StencilC {
+    double c0, c1, c2, c3, elap;
+};
+
+template <typename T, int EPI>
+__global__ __launch_bounds__(TPB) void k_stencil(const T *__restrict__ fld, T *__restrict__ fldp, T *__restrict__ out,
+                                                 int nx, int ny, int nz, int nf, StencilC sc, ModelDev md,
+                                                 const DevScalars *__restrict__ dsc, double *__restrict__ partials)
+{
+    const size_t nvol = (size_t)nx * ny * nz;
+    const T c0 = T(sc.c0), c1 = T(sc.c1), c2 = T(sc.c2), c3 = T(sc.c3), el = T(sc.elap);
+    const bool has2 = sc.c2 != 0.0, has3 = sc.c3 != 0.0;
+    T hA = 0, hB = 0;
+    if constexpr (EPI == EPI_KICK) {
+        hA = T(dsc->hA);
+        hB = T(dsc->hB);
+    }
+    double acc[NRED] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (size_t s = blockIdx.x * (size_t)TPB + threadIdx.x; s < nvol; s += (size_t)gridDim.x * TPB) {
+        const int i = (int)(s % nx);
+        const int j = (int)((s / nx) % ny);
+        const int k = (int)(s / ((size_t)nx * ny));
+        const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
+        const int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
+        const int km = k == 0 ? nz - 1 : k - 1, kp = k == nz - 1 ? 0 : k + 1;
+        const int xs[3] = {im, i, ip}, ys[3] = {jm, j, jp}, zs[3] = {km, k, kp};
+        T lap[2] = {0, 0}, phi[2] = {0, 0};
+        for (int l = 0; l < nf; ++l) {
+            const T *f = fld + (size_t)l * nvol;
+#define AT(a, b, cc_) f[(size_t)xs[a] + (size_t)nx * ((size_t)ys[b] + (size_t)ny * zs[cc_])]
+            const T f0 = AT(1, 1, 1);
+            const T r1 = AT(0, 1, 1) + AT(2, 1, 1) + AT(1, 0, 1) + AT(1, 2, 1) + AT(1, 1, 0) + AT(1, 1, 2);
+            T v = c0 * f0 + c1 * r1;
+            if (has2) {
+                const T r2 = AT(0, 0, 1) + AT(2, 0, 1) + AT(0, 2, 1) + AT(2, 2, 1) + AT(0, 1, 0) + AT(2, 1, 0) +
+                             AT(0, 1, 2) + AT(2, 1, 2) + AT(1, 0, 0) + AT(1, 2, 0) + AT(1, 0, 2) + AT(1, 2, 2);
+                v = v + c2 * r2;
+            }
+            if (has3) {
+                const T r3 = AT(0, 0, 0) + AT(2, 0, 0) + AT(0, 2, 0) + AT(2, 2, 0) + AT(0, 0, 2) + AT(2, 0, 2) +
+                             AT(0, 2, 2) + AT(2, 2, 2);
+                v = v + c3 * r3;
+            }
+#undef AT
+            lap[l] = v * el;
+            phi[l] = f0;
+        }
+        if constexpr (EPI == EPI_STORE) {
+            for (int l = 0; l < nf; ++l) out[(size_t)l * nvol + s] = lap[l];
+        } else {
+            T dv[2] = {0, 0};
+            if constexpr (EPI == EPI_KICK) mdv(md, phi, dv);
+            acc[RC_PE] += mv(md, phi);
+            for (int l = 0; l < nf; ++l) {
+                T p = fldp[(size_t)l * nvol + s];
+                if constexpr (EPI == EPI_KICK) {
+                    p = p + (hA * lap[l] - hB * dv[l]);
+                    fldp[(size_t)l * nvol + s] = p;
+                }
+                acc[RC_GE] -= double(phi[l]) * double(lap[l]);
+                acc[RC_KE] += double(p) * double(p);
+            }
+            acc[RC_M1] += double(phi[0]);
+            acc[RC_M2] += double(phi[1]);
+        }
+    }
+    if constexpr (EPI != EPI_STORE) block_reduce_store2<NRED>(acc, partials + (size_t)blockIdx.x * NRED);
+}
+
+template <typename T>
+int launch_stencil(Ctx &c, int epi)
+{
+    const double *cs = c.cfg.stencil;
+    StencilC sc{cs[0], cs[1], cs[2], cs[3], 1.0 / (c.dx * c.dx) / cs[4]};
+    ModelDev md{c.cfg.model, c.nfld, c.cfg.model_params[0], c.cfg.model_params[1]};
+    size_t blocks = (c.nvol + TPB - 1) / TPB;
+    int g = grid_for(blocks > (size_t)MAX_GRID ? MAX_GRID : (int)blocks);
+    T *fld = (T *)c.fld, *fldp = (T *)c.fldp;
+    if (epi == EPI_KICK)
+        hipLaunchKernelGGL((k_stencil<T, EPI_KICK>), dim3(g), dim3(TPB), 0, c.stream, fld, fldp, (T *)nullptr, c.nx,
+                           c.ny, c.nz, c.nfld, sc, md, c.dsc, c.partials);
+    else if (epi == EPI_REDUCE)
+        hipLaunchKernelGGL((k_stencil<T, EPI_REDUCE>), dim3(g), dim3(TPB), 0, c.stream, fld, fldp, (T *)nullptr,
+                           c.nx, c.ny, c.nz, c.nfld, sc, md, c.dsc, c.partials);
+    else
+        hipLaunchKernelGGL((k_stencil<T, EPI_STORE>), dim3(g), dim3(TPB), 0, c.stream, fld, fldp, (T *)c.io64, c.nx,
+                           c.ny, c.nz, 1, sc, md, c.dsc, c.partials);
+    LAT_HIP(&c, hipGetLastError());
+    return g;
+}
+
+// FP64 stencil of one host-interface field (io64 -> io64 + nvol).
+__global__ __launch_bounds__(TPB) void k_stencil_io(const double *__restrict__ f, double *__restrict__ out, int nx,
+                                                    int ny, int nz, StencilC sc)
+{
+    const size_t nvol = (size_t)nx * ny * nz;
+    for (size_t s = blockIdx.x * (size_t)TPB + threadIdx.x; s < nvol; s += (size_t)gridDim.x * TPB) {
+        const int i = (int)(s % nx);
+        const int j = (int)((s / nx) % ny);
+        const int k = (int)(s / ((size_t)nx * ny));
+        const int xs[3] = {i == 0 ? nx - 1 : i - 1, i, i == nx - 1 ? 0 : i + 1};
+        const int ys[3] = {j == 0 ? ny - 1 : j - 1, j, j == ny - 1 ? 0 : j + 1};
+        const int zs[3] = {k == 0 ? nz - 1 : k - 1, k, k == nz - 1 ? 0 : k + 1};
+        double r[4] = {0, 0, 0, 0};
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                for (int cc = 0; cc < 3; ++cc) {
+                    const int rank = (a != 1) + (b != 1) + (cc != 1);
+                    r[rank] += f[(size_t)xs[a] + (size_t)nx * ((size_t)ys[b] + (size_t)ny * zs[cc])];
+                }
+        out[s] = (sc.c0 * r[0] + sc.c1 * r[1] + sc.c2 * r[2] + sc.c3 * r[3]) * sc.elap;
+    }
+}
+
+int launch_stencil_io(Ctx &c, const double *f, double *out)
+{
+    const double *cs = c.cfg.stencil;
+    StencilC sc{cs[0], cs[1], cs[2], cs[3], 1.0 / (c.dx * c.dx) / cs[4]};
+    size_t blocks = (c.nvol + TPB - 1) / TPB;
+    int g = blocks > 8192 ? 8192 : (int)blocks;
+    hipLaunchKernelGGL(k_stencil_io, dim3(g), dim3(TPB), 0, c.stream, f, out, c.nx, c.ny, c.nz, sc);
+    LAT_HIP(&c, hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------------------- scalars
+struct ScalarEnv {
+    int time_var;
+    double nvol;
+};
+
+// One workgroup: ordered FP64 reduction of `npart` partials (Neumaier
+// compensated per channel), then the split operators' scalar parts in order.
+__global__ __launch_bounds__(TPB) void k_scalar(DevScalars *__restrict__ sc, const double *__restrict__ partials,
+                                                int npart, ScalarOps ops, ScalarEnv env)
+{
+    __shared__ double sh[NRED][32];
+    const int ch = threadIdx.x >> 5, lane = threadIdx.x & 31; // 8 channels x 32 lanes
+    if (npart > 0) {
+        double s = 0.0, comp = 0.0;
+        for (int g = lane; g < npart; g += 32) {
+            const double x = partials[(size_t)g * NRED + ch];
+            const double t = s + x;
+            comp += (fabs(s) >= fabs(x)) ? (s - t) + x : (x - t) + s;
+            s = t;
+        }
+        sh[ch][lane] = s + comp;
+        __syncthreads();
+        if (lane == 0) {
+            double tot = 0.0, c2 = 0.0;
+            for (int q = 0; q < 32; ++q) {
+                const double x = sh[ch][q];
+                const double t = tot + x;
+                c2 += (fabs(tot) >= fabs(x)) ? (tot - t) + x : (x - t) + tot;
+                tot = t;
+            }
+            sc->red[ch] = tot + c2;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    sc->nd = 0;
+    double y = sc->yscl, p = sc->ysclp;
+    const bool conf = env.time_var == LAT_CONFORMAL;
+    for (int q = 0; q < ops.n; ++q) {
+        const double h = ops.h[q];
+        switch (ops.type[q]) {
+        case OP_FINISH_KICK: {
+            // hamiltonian_conformal.f90:172,203-205; hamiltonian_cosmic.f90:153,209-211
+            const double kh = sc->kick_h;
+            const double PE = sc->red[RC_PE] / env.nvol;
+            if (conf) {
+                const double GE2 = sc->red[RC_GE] / (y * y) / env.nvol;
+                p = p - (y * y * y) * (GE2 + 4.0 * PE) * kh;
+            } else {
+                const double a = pow(y, 2.0 / 3.0);
+                const double GE2 = sc->red[RC_GE] / (a * a) / env.nvol;
+                p = p - y * ((1.0 / 3.0) * GE2 + 2.0 * PE) * kh;
+            }
+            break;
+        }
+        case OP_DRIFT: {
+            // hamiltonian_conformal.f90:98-99,116-117 / hamiltonian_cosmic.f90:107-108,125-126
+            if (sc->nd < 4) sc->dc[sc->nd++] = h / (y * y);
+            const double K2 = sc->red[RC_KE] / env.nvol;
+            p = p + h * K2 / (y * y * y);
+            break;
+        }
+        case OP_GRAV:
+            // hamiltonian_conformal.f90:89 / hamiltonian_cosmic.f90:97
+            y = conf ? y - h * p / 6.0 : y - h * p * (3.0 / 8.0);
+            break;
+        case OP_PREP_KICK: {
+            sc->kick_h = h;
+            if (conf) {
+                sc->hA = h * (y * y);
+                sc->hB = h * (y * y) * (y * y);
+            } else {
+                const double a = pow(y, 2.0 / 3.0);
+                sc->hA = h * (y * y) / (a * a);
+                sc->hB = h * (y * y);
+            }
+            break;
+        }
+        case OP_CLEAR_DRIFT:
+            sc->nd = 0;
+            break;
+        default:
+            break;
+        }
+    }
+    sc->yscl = y;
+    sc->ysclp = p;
+}
+
+int launch_scalar(Ctx &c, const ScalarOps &ops, int npart)
+{
+    ScalarEnv env{c.cfg.time_var, (double)c.nvol};
+    hipLaunchKernelGGL(k_scalar, dim3(1), dim3(TPB), 0, c.stream, c.dsc, c.partials, npart, ops, env);
+    LAT_HIP(&c, hipGetLastError());
+    return 0;
+}
+
+template int launch_drift<double>(Ctx &);
+template int launch_drift<float>(Ctx &);
+template int launch_reduce_state<double>(Ctx &);
+template int launch_reduce_state<float>(Ctx &);
+template int launch_stencil<double>(Ctx &, int);
+template int launch_stencil<float>(Ctx &, int);
+
+} // namespace lat

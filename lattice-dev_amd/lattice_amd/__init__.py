@@ -1,0 +1,295 @@
+"""Host-side binding of liblattice_amd.so (the C ABI of include/lattice_c.h).
+
+This mirrors the reference's Fortran module interfaces for the evolution path
+(`module fftw3`, `module analysis`, `module Hamiltonian`, `module integrator`):
+same names, argument meaning and error behaviour (a nonzero status raises
+LatticeError, where the reference prints and stops).  It holds no numerics of
+its own: every call goes through the HIP library, and importing fails loudly
+if that library is missing.
+
+Array convention: a lattice field is a numpy array of shape (nz, ny, nx),
+C-contiguous, i.e. byte-identical to the reference's Fortran f(nx, ny, nz).
+Half-complex transforms are (nz, ny, nx//2+1) complex128 (FFTW r2c layout).
+Field sets are (nfld, nz, ny, nx) here and are converted to/from the
+reference's AoS fld(nfld, nx, ny, nz) layout at the C boundary.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "liblattice_amd.so")
+
+F64, MIXED = 0, 1
+LAP_SPECTRAL, LAP_STENCIL = 0, 1
+CONFORMAL, COSMIC = 0, 1
+MODEL_PHI4, MODEL_TWOFIELD = 0, 1
+E_NAMES = ["a", "hubble", "rho", "ke", "pe", "ge", "grav", "resid", "pmom",
+           "mean1", "mean2", "ge_grad"]
+
+STENCIL_7PT = (-6.0, 1.0, 0.0, 0.0, 1.0)      # Discrete/Entropy/params.f90:63
+STENCIL_27PT = (-128.0, 14.0, 3.0, 1.0, 30.0)  # hamiltonian_conformal.f90:41
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_lib = None
+
+
+class LatticeError(RuntimeError):
+    pass
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("nfld", ctypes.c_int),
+        ("len", ctypes.c_double), ("precision", ctypes.c_int), ("laplacian", ctypes.c_int),
+        ("stencil", ctypes.c_double * 5), ("time_var", ctypes.c_int), ("model", ctypes.c_int),
+        ("model_params", ctypes.c_double * 4), ("symp6_tree", ctypes.c_int),
+        ("reserved", ctypes.c_int * 7),
+    ]
+
+
+EXPORTS = [
+    "lat_create", "lat_destroy", "lat_last_error", "lat_synchronize",
+    "lat_laplacian_3d", "lat_gradient_squared_3d", "lat_fft_r2c_3d", "lat_fft_c2r_3d",
+    "lat_stencil_laplacian", "lat_num_bins", "lat_spectrum_3d", "lat_crossspec_3d",
+    "lat_output_spectra", "lat_set_state", "lat_get_state", "lat_get_scalars",
+    "lat_set_scalars", "lat_split", "lat_symp_o2step", "lat_symp", "lat_calc_metric",
+    "lat_energies", "lat_device_state", "lat_last_symp_ms",
+]
+
+
+def lib():
+    """Load the HIP library; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LatticeError(f"{LIB_PATH} not built: run __graft_entry__.build() "
+                               "(make -C lattice-dev_amd)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.lat_last_error.restype = ctypes.c_char_p
+        L.lat_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(ctypes.c_void_p)]
+        L.lat_destroy.argtypes = [ctypes.c_void_p]
+        for name in EXPORTS:
+            fn = getattr(L, name)
+            if name not in ("lat_last_error", "lat_destroy", "lat_create"):
+                fn.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    if not a.flags["C_CONTIGUOUS"]:
+        raise LatticeError("array must be C-contiguous")
+    return a.ctypes.data_as(_dp)
+
+
+def num_bins(nx, ny, nz):
+    """init_spectrum_3d (analysis_spec.f90:48-57): number of shells ns."""
+    return lib().lat_num_bins(nx, ny, nz)
+
+
+class Lattice:
+    """One device context: state, FFT tables, stream (the module globals of
+    hamiltonian_*.f90 and fftw_mod.f90 gathered into an object)."""
+
+    def __init__(self, n, nfld=2, length=10.0, precision=F64, laplacian=LAP_SPECTRAL,
+                 stencil=STENCIL_7PT, time_var=COSMIC, model=MODEL_TWOFIELD, model_params=(1.0, 2.5e5),
+                 symp6_tree=1):
+        if isinstance(n, int):
+            n = (n, n, n)
+        self.nx, self.ny, self.nz = n
+        self.nfld = nfld
+        self.length = length
+        self.dx = length / self.nx
+        self.dk = 2.0 * np.pi / length
+        cfg = Config()
+        cfg.nx, cfg.ny, cfg.nz, cfg.nfld = self.nx, self.ny, self.nz, nfld
+        cfg.len, cfg.precision, cfg.laplacian = length, precision, laplacian
+        for i, v in enumerate(stencil):
+            cfg.stencil[i] = v
+        cfg.time_var, cfg.model = time_var, model
+        for i, v in enumerate(model_params):
+            cfg.model_params[i] = v
+        cfg.symp6_tree = symp6_tree
+        self.cfg = cfg
+        h = ctypes.c_void_p()
+        rc = lib().lat_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc != 0:
+            raise LatticeError(f"lat_create failed with status {rc}")
+        self._h = h
+        self.ns = num_bins(self.nx, self.ny, self.nz)
+
+    # ---------------------------------------------------------------- plumbing
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lat_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = lib().lat_last_error(self._h)
+            raise LatticeError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def _field(self, f):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        if f.shape != (self.nz, self.ny, self.nx):
+            raise LatticeError(f"field shape {f.shape} != {(self.nz, self.ny, self.nx)}")
+        return f
+
+    def synchronize(self):
+        self._check(lib().lat_synchronize(self._h), "lat_synchronize")
+
+    # ---------------------------------------------------------------- module fftw3
+    def laplacian_3d(self, f, dk=None):
+        """laplacian_3d (fftw_mod.f90:177-207); returns a new array."""
+        f = self._field(f).copy()
+        dk = self.dk if dk is None else dk
+        self._check(lib().lat_laplacian_3d(self._h, self.nx, self.ny, self.nz, _p(f), ctypes.c_double(dk)),
+                    "laplacian_3d")
+        return f
+
+    def gradient_squared_3d(self, f, dk=None):
+        f = self._field(f)
+        out = np.empty_like(f)
+        dk = self.dk if dk is None else dk
+        self._check(lib().lat_gradient_squared_3d(self._h, _p(f), _p(out), ctypes.c_double(dk)),
+                    "gradient_squared_3d")
+        return out
+
+    def fft_r2c_3d(self, f):
+        f = self._field(f)
+        F = np.empty((self.nz, self.ny, self.nx // 2 + 1), np.complex128)
+        self._check(lib().lat_fft_r2c_3d(self._h, _p(f), _p(F.view(np.float64))), "fft_r2c_3d")
+        return F
+
+    def fft_c2r_3d(self, F):
+        F = np.ascontiguousarray(F, dtype=np.complex128)
+        f = np.empty((self.nz, self.ny, self.nx))
+        self._check(lib().lat_fft_c2r_3d(self._h, _p(F.view(np.float64)), _p(f)), "fft_c2r_3d")
+        return f
+
+    def stencil_laplacian(self, f):
+        f = self._field(f)
+        out = np.empty_like(f)
+        self._check(lib().lat_stencil_laplacian(self._h, _p(f), _p(out)), "stencil_laplacian")
+        return out
+
+    # ---------------------------------------------------------------- module analysis
+    def spectrum_3d(self, f, return_fk=False):
+        """spectrum_3d (analysis_spec.f90:59-102)."""
+        f = self._field(f)
+        S = np.empty(self.ns)
+        F = np.empty((self.nz, self.ny, self.nx // 2 + 1), np.complex128) if return_fk else None
+        self._check(lib().lat_spectrum_3d(self._h, _p(S), _p(f), _p(F.view(np.float64)) if return_fk else None),
+                    "spectrum_3d")
+        return (S, F) if return_fk else S
+
+    def crossspec_3d(self, Fk1, Fk2):
+        """crossspec_3d (analysis_spec.f90:145-178) -> (S_real, S_imag)."""
+        Fk1 = np.ascontiguousarray(Fk1, dtype=np.complex128)
+        Fk2 = np.ascontiguousarray(Fk2, dtype=np.complex128)
+        sre, sim = np.empty(self.ns), np.empty(self.ns)
+        self._check(lib().lat_crossspec_3d(self._h, _p(Fk1.view(np.float64)), _p(Fk2.view(np.float64)),
+                                           _p(sre), _p(sim)), "crossspec_3d")
+        return sre, sim
+
+    def output_spectra(self, field=0):
+        """make_output spectra (evolve_spectral.f90:531-538) of the resident state:
+        returns (S_phi, S_pi, Re S_pi_phi, Im S_pi_phi)."""
+        out = np.empty(4 * self.ns)
+        self._check(lib().lat_output_spectra(self._h, field, _p(out)), "output_spectra")
+        return out.reshape(4, self.ns)
+
+    # ---------------------------------------------------------------- state
+    def set_state(self, fld, fldp, yscl=1.0, ysclp=0.0):
+        """fld, fldp: (nfld, nz, ny, nx) SoA arrays; converted to the reference AoS."""
+        fld = np.asarray(fld, np.float64)
+        fldp = np.asarray(fldp, np.float64)
+        aos = np.ascontiguousarray(np.moveaxis(fld, 0, -1))
+        aosp = np.ascontiguousarray(np.moveaxis(fldp, 0, -1))
+        self._check(lib().lat_set_state(self._h, _p(aos), _p(aosp), 0, ctypes.c_double(yscl),
+                                        ctypes.c_double(ysclp)), "set_state")
+
+    def set_state_aos(self, fld_aos, fldp_aos, ghost, yscl, ysclp):
+        """Reference layout: fld(nfld, 0:nx+1, ...) (ghost=1) and fldp(nfld, 1:nx, ...)."""
+        fld_aos = np.ascontiguousarray(fld_aos, np.float64)
+        fldp_aos = np.ascontiguousarray(fldp_aos, np.float64)
+        self._check(lib().lat_set_state(self._h, _p(fld_aos), _p(fldp_aos), ghost, ctypes.c_double(yscl),
+                                        ctypes.c_double(ysclp)), "set_state")
+
+    def get_state(self):
+        aos = np.empty((self.nz, self.ny, self.nx, self.nfld))
+        aosp = np.empty_like(aos)
+        y, p = ctypes.c_double(), ctypes.c_double()
+        self._check(lib().lat_get_state(self._h, _p(aos), _p(aosp), 0, ctypes.byref(y), ctypes.byref(p)),
+                    "get_state")
+        return np.moveaxis(aos, -1, 0).copy(), np.moveaxis(aosp, -1, 0).copy(), y.value, p.value
+
+    def get_state_aos(self, ghost):
+        g = ghost
+        aos = np.empty((self.nz + 2 * g, self.ny + 2 * g, self.nx + 2 * g, self.nfld))
+        aosp = np.empty((self.nz, self.ny, self.nx, self.nfld))
+        y, p = ctypes.c_double(), ctypes.c_double()
+        self._check(lib().lat_get_state(self._h, _p(aos), _p(aosp), g, ctypes.byref(y), ctypes.byref(p)),
+                    "get_state")
+        return aos, aosp, y.value, p.value
+
+    @property
+    def scalars(self):
+        y, p = ctypes.c_double(), ctypes.c_double()
+        self._check(lib().lat_get_scalars(self._h, ctypes.byref(y), ctypes.byref(p)), "get_scalars")
+        return y.value, p.value
+
+    def set_scalars(self, yscl, ysclp):
+        self._check(lib().lat_set_scalars(self._h, ctypes.c_double(yscl), ctypes.c_double(ysclp)), "set_scalars")
+
+    # ---------------------------------------------------------------- module Hamiltonian / integrator
+    def Hamiltonian_Split(self, dt, hamind):
+        self._check(lib().lat_split(self._h, ctypes.c_double(dt), hamind), "Hamiltonian_Split")
+
+    def symp_o2step(self, dt, w1, w2):
+        self._check(lib().lat_symp_o2step(self._h, ctypes.c_double(dt), ctypes.c_double(w1), ctypes.c_double(w2)),
+                    "symp_o2step")
+
+    def symp(self, order, dt, nsteps):
+        self._check(lib().lat_symp(self._h, order, ctypes.c_double(dt), nsteps), f"symp{order}")
+
+    def symp2(self, dt, nsteps):
+        self.symp(2, dt, nsteps)
+
+    def symp4(self, dt, nsteps):
+        self.symp(4, dt, nsteps)
+
+    def symp6(self, dt, nsteps):
+        self.symp(6, dt, nsteps)
+
+    def symp8(self, dt, nsteps):
+        self.symp(8, dt, nsteps)
+
+    def calc_metric(self):
+        self._check(lib().lat_calc_metric(self._h), "calc_metric")
+
+    def energies(self):
+        out = np.empty(len(E_NAMES))
+        self._check(lib().lat_energies(self._h, _p(out)), "energies")
+        return dict(zip(E_NAMES, out))
+
+    def device_state(self):
+        f, p = ctypes.c_void_p(), ctypes.c_void_p()
+        self._check(lib().lat_device_state(self._h, ctypes.byref(f), ctypes.byref(p)), "device_state")
+        return f.value, p.value

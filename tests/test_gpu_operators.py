@@ -1,0 +1,117 @@
+"""GPU parity of the single operators (FFT, Laplacians, spectra) against the
+CPU oracle and numpy.  Calls go through the C ABI (lattice_amd -> liblattice_amd.so)."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(8, 8, 8), (16, 16, 16), (32, 32, 32), (64, 64, 64), (16, 32, 8), (48, 48, 48), (96, 32, 16),
+         (192, 16, 8), (128, 16, 16), (256, 8, 8), (1024, 8, 8), (8, 512, 8), (8, 8, 1024)]
+
+
+def _lat(la, shape, **kw):
+    nx, ny, nz = shape
+    kw.setdefault("nfld", 2)
+    return la.Lattice((nx, ny, nz), length=10.0, **kw)
+
+
+@pytest.mark.parametrize("shape", SIZES, ids=lambda s: "x".join(map(str, s)))
+def test_r2c_matches_numpy(gpu, shape):
+    print("case r2c", shape, flush=True)
+    nx, ny, nz = shape
+    rng = np.random.default_rng(nx * 7 + ny * 3 + nz)
+    f = rng.standard_normal((nz, ny, nx))
+    with _lat(gpu, shape) as lat:
+        F = lat.fft_r2c_3d(f)
+        back = lat.fft_c2r_3d(F)
+    G = np.fft.rfftn(f)
+    assert np.abs(F - G).max() <= 1e-13 * np.abs(G).max() * np.log2(f.size)
+    assert np.abs(back / f.size - f).max() <= 1e-13 * np.log2(f.size)
+
+
+@pytest.mark.parametrize("shape", SIZES, ids=lambda s: "x".join(map(str, s)))
+def test_c2r_matches_numpy_nonhermitian(gpu, shape):
+    """C2R of an arbitrary half-complex array: imaginary parts of the x-DC and
+    x-Nyquist coefficients are discarded after the y/z passes (numpy irfftn)."""
+    print("case c2r", shape, flush=True)
+    nx, ny, nz = shape
+    rng = np.random.default_rng(3 + nx)
+    F = rng.standard_normal((nz, ny, nx // 2 + 1)) + 1j * rng.standard_normal((nz, ny, nx // 2 + 1))
+    with _lat(gpu, shape) as lat:
+        f = lat.fft_c2r_3d(F)
+    ref = np.fft.irfftn(F, s=(nz, ny, nx), axes=(0, 1, 2)) * (nx * ny * nz)
+    assert np.abs(f - ref).max() <= 1e-13 * np.abs(ref).max() * np.log2(f.size)
+
+
+@pytest.mark.parametrize("shape", SIZES, ids=lambda s: "x".join(map(str, s)))
+def test_laplacian_matches_oracle(gpu, shape):
+    print("case laplacian", shape, flush=True)
+    nx, ny, nz = shape
+    rng = np.random.default_rng(11 + nx)
+    f = rng.standard_normal((nz, ny, nx))
+    dk = 2 * np.pi / 10.0
+    with _lat(gpu, shape) as lat:
+        g = lat.laplacian_3d(f, dk)
+    r = O.laplacian_3d(f, dk)
+    assert np.abs(g - r).max() <= 1e-13 * np.abs(r).max()
+
+
+def test_laplacian_plane_wave_closed_form(gpu):
+    n, L = 32, 10.0
+    dk = 2 * np.pi / L
+    x = np.arange(n) * L / n
+    z3, y3, x3 = np.meshgrid(x, x, x, indexing="ij")
+    f = np.cos(3 * dk * x3 + 5 * dk * y3 - 7 * dk * z3) + np.cos(16 * dk * x3)  # incl. x-Nyquist
+    expect = -(9 + 25 + 49) * dk**2 * np.cos(3 * dk * x3 + 5 * dk * y3 - 7 * dk * z3) \
+        - (16 * dk) ** 2 * np.cos(16 * dk * x3)
+    with gpu.Lattice(n, nfld=2, length=L) as lat:
+        g = lat.laplacian_3d(f)
+    assert np.abs(g - expect).max() < 1e-11
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 8), (32, 32, 32), (64, 64, 64), (16, 32, 8), (48, 48, 48),
+                                   (192, 16, 8)], ids=lambda s: "x".join(map(str, s)))
+def test_spectra_match_oracle(gpu, shape):
+    print("case spectra", shape, flush=True)
+    nx, ny, nz = shape
+    rng = np.random.default_rng(5 + nx)
+    f1 = rng.standard_normal((nz, ny, nx))
+    f2 = rng.standard_normal((nz, ny, nx))
+    with _lat(gpu, shape) as lat:
+        S1, F1 = lat.spectrum_3d(f1, return_fk=True)
+        S2, F2 = lat.spectrum_3d(f2, return_fk=True)
+        cre, cim = lat.crossspec_3d(F2, F1)
+        # repeatability: deterministic binning
+        S1b = lat.spectrum_3d(f1)
+    R1 = O.r2c_3d(f1)
+    R2 = O.r2c_3d(f2)
+    assert np.abs(F1 - R1).max() <= 1e-13 * np.abs(R1).max()
+    o1 = O.spectrum_3d(R1, nx)
+    ore, oim = O.crossspec_3d(R2, R1, nx)
+    assert np.allclose(S1, o1, rtol=1e-12, atol=0)
+    assert np.allclose(cre, ore, rtol=1e-10, atol=1e-12 * np.abs(ore).max())
+    assert np.allclose(cim, oim, rtol=1e-10, atol=1e-12 * np.abs(ore).max())
+    assert np.array_equal(S1, S1b)
+
+
+@pytest.mark.parametrize("coef", [(-6.0, 1.0, 0.0, 0.0, 1.0), (-128.0, 14.0, 3.0, 1.0, 30.0)], ids=["7pt", "27pt"])
+def test_stencil_matches_oracle(gpu, coef):
+    n, L = 32, 10.0
+    rng = np.random.default_rng(2)
+    f = rng.standard_normal((n, n, n))
+    with gpu.Lattice(n, nfld=2, length=L, laplacian=gpu.LAP_STENCIL, stencil=coef) as lat:
+        g = lat.stencil_laplacian(f)
+    r = O.stencil_laplacian(f, coef, L / n)
+    assert np.abs(g - r).max() <= 1e-12 * np.abs(r).max()
+
+
+def test_gradient_squared_matches_oracle(gpu):
+    n, L = 32, 10.0
+    rng = np.random.default_rng(4)
+    f = rng.standard_normal((n, n, n))
+    with gpu.Lattice(n, nfld=2, length=L) as lat:
+        g = lat.gradient_squared_3d(f)
+    r = O.gradient_squared_3d(f, 2 * np.pi / L)
+    assert np.abs(g - r).max() <= 1e-12 * np.abs(r).max()
