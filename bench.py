@@ -1,0 +1,232 @@
+#!/usr/bin/env python3
+"""Throughput of the lattice evolution hot path on one MI355X.
+
+Workload: two-field preheating, V = m^2 phi^2/2 + g^2 phi^2 chi^2/2 with
+g^2/m^2 = 2.5e5, cosmic-time FRW background, pseudospectral Laplacian,
+Yoshida-4 (symp4), L = 10/m, dt = dx/40, on a 1024^3 lattice (1.07e9
+sites per field).  Initial conditions: homogeneous background
+(hamiltonian_cosmic.f90:18-22 values) plus seeded vacuum fluctuations from
+the device sampler.  The state is resident in HBM before timing starts.
+
+One "step" = one Yoshida-4 step (3 force evaluations) of every site; the
+metric is site-steps/s = N^3 * steps / s, output spectra of both fields
+included at the configured cadence.  Both forms run from identical IC:
+  plain FP64, and mixed (FP32 fields/FFT, FP64 background and reductions);
+`value` is the mixed form, the FP64 form and the mixed-vs-FP64 spectrum
+error are reported beside it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "lattice-dev_amd"))
+
+METRIC = "site-steps/s at 1024^3, mixed vs plain FP64, at stated energy/spectrum error"
+PEAK_HBM_GBS = 8000.0
+
+PHI0 = 1.0093430384226378929425913902459
+DPHI0 = -0.7137133070120812430962278466136
+MPL = 2.0e5
+G2 = 2.5e5
+
+
+def make_lattice(la, n, precision):
+    return la.Lattice(n, nfld=2, length=10.0, precision=precision, laplacian=la.LAP_SPECTRAL,
+                      time_var=la.COSMIC, model=la.MODEL_TWOFIELD, model_params=(1.0, G2))
+
+
+def init_state(lat, seed=20240601):
+    """Vacuum fluctuations (gamma = -1/4 fields, +1/4 momenta) on the background."""
+    m2 = [1.0, G2 * PHI0 ** 2]
+    lat.sample_field(0, 0, PHI0, -0.25, m2[0], MPL, seed, 0)
+    lat.sample_field(0, 1, DPHI0, 0.25, m2[0], MPL, seed, 1)
+    lat.sample_field(1, 0, 0.0, -0.25, m2[1], MPL, seed, 2)
+    lat.sample_field(1, 1, 0.0, 0.25, m2[1], MPL, seed, 3)
+    lat.set_scalars(1.0, 0.0)
+    lat.calc_metric()
+
+
+def algorithmic_bytes(stage, n, nf, w):
+    """Fused-minimum HBM bytes per launch of each stage (w = bytes per real)."""
+    nvol = float(n) ** 3
+    wc = w * (1.0 + 2.0 / n)  # half-complex bytes per real site
+    return {
+        "x_r2c": nvol * (3 * w + wc),       # read phi, pi; write phi; write F (one field)
+        "y_fwd": nf * nvol * 2 * wc,         # read + write F (all fields)
+        "z_lap": nf * nvol * 2 * wc,
+        "y_inv": nf * nvol * 2 * wc,
+        "x_c2r": nvol * nf * (wc + 3 * w),   # read F, phi; read + write pi
+    }.get(stage)
+
+
+def run_form(la, n, precision, steps, warmup, out_every, dt):
+    lat = make_lattice(la, n, precision)
+    t0 = time.time()
+    init_state(lat)
+    lat.synchronize()
+    t_init = time.time() - t0
+    lat.symp4(dt, warmup)
+    lat.synchronize()
+    lat.profile(True)
+    done = 0
+    spectra = None
+    t0 = time.perf_counter()
+    while done < steps:
+        k = min(out_every, steps - done)
+        lat.symp4(dt, k)
+        done += k
+        spectra = [lat.output_spectra(0), lat.output_spectra(1)]
+    lat.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = lat.profile_read()
+    lat.profile(False)
+    energies = lat.energies()
+    lat.close()
+    return dict(elapsed=elapsed, prof=prof, spectra=spectra, energies=energies, t_init=t_init)
+
+
+def roofline(prof, n, nf, w):
+    best = None
+    for stage, (ms, cnt) in prof.items():
+        b = algorithmic_bytes(stage, n, nf, w)
+        if b is None or cnt == 0:
+            continue
+        if best is None or ms > best[1]:
+            best = (stage, ms, cnt, b)
+    if best is None:
+        return None
+    stage, ms, cnt, b = best
+    avg_s = ms / cnt / 1e3
+    achieved = b / avg_s / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{stage}_{n}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+    return {"bound": "hbm", "kernel": stage, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+            "algorithmic_bytes_per_launch": b, "avg_launch_ms": round(ms / cnt, 4), "launches": cnt}
+
+
+def cpu_baseline(n_sample, steps_sample):
+    """The oracle (C restatement, OpenMP) on a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    dt = (10.0 / n_sample) / 40.0
+    sim = O.Simulation(n_sample, 2, 10.0, O.COSMIC, model=O.MODEL_TWOFIELD, mp=(1.0, G2))
+    rng = np.random.default_rng(1)
+    sim.fld[0] = PHI0 + 1e-6 * rng.standard_normal(sim.fld[0].shape)
+    sim.fld[1] = 1e-6 * rng.standard_normal(sim.fld[1].shape)
+    sim.fldp[0] = DPHI0
+    sim.yscl = 1.0
+    sim.calc_metric()
+    t0 = time.perf_counter()
+    sim.symp(4, dt, steps_sample)
+    el = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    return {"value": n_sample ** 3 * steps_sample / el, "unit": "site-steps/s", "cores": threads, "kind": "port",
+            "sample": f"oracle/liblatoracle.so FP64, {n_sample}^3 two-field cosmic spectral symp4, "
+                      f"{steps_sample} steps, {el:.1f} s"}
+
+
+def spectrum_error(sa, sb, floor):
+    """max |a-b|/|b| of S_phi, S_pi of both fields over k > 0 bins carrying at
+    least `floor` of the peak bin power of the FP64 spectrum."""
+    errs = []
+    for a, b in zip(sa, sb):
+        for ch in (0, 1):
+            x, y = a[ch][1:], b[ch][1:]
+            m = np.abs(y) >= floor * np.abs(y).max()
+            m &= np.abs(y) > 0
+            errs.append(float(np.max(np.abs(x[m] - y[m]) / np.abs(y[m]))))
+    return max(errs)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--output-every", type=int, default=20)
+    ap.add_argument("--forms", default="f64,mixed")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=12)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design note, "
+              "not a measured configuration (use --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import lattice_amd as la
+
+    n = args.n
+    dt = (10.0 / n) / 40.0
+    forms = args.forms.split(",")
+    res = {}
+    for form in forms:
+        prec = la.F64 if form == "f64" else la.MIXED
+        print(f"[bench] {form} {n}^3: warmup {args.warmup}, timing {args.steps} steps", file=sys.stderr, flush=True)
+        res[form] = run_form(la, n, prec, args.steps, args.warmup, args.output_every, dt)
+        r = res[form]
+        print(f"[bench] {form}: {r['elapsed']:.3f} s, {n ** 3 * args.steps / r['elapsed']:.4e} site-steps/s "
+              f"(init {r['t_init']:.1f} s)", file=sys.stderr, flush=True)
+    head = "mixed" if "mixed" in res else forms[0]
+    r = res[head]
+    value = n ** 3 * args.steps / r["elapsed"]
+    w = 4 if head == "mixed" else 8
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "site-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * r["elapsed"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 fields/FFT + f64 background/reductions" if head == "mixed" else "f64",
+        "data": "synthetic: seeded vacuum fluctuations (device Philox sampler) on the homogeneous two-field background",
+        "config": {"workload": f"{n}^3 two-field preheating g^2/m^2=2.5e5, cosmic FRW, spectral Laplacian, "
+                               f"Yoshida-4, L=10/m, dt=dx/40, spectra every {args.output_every} steps",
+                   "form": head},
+        "roofline": roofline(r["prof"], n, 2, w),
+    }
+    if "f64" in res:
+        f = res["f64"]
+        v64 = n ** 3 * args.steps / f["elapsed"]
+        line["fp64"] = {"value": v64, "ms_per_step": 1e3 * f["elapsed"] / args.steps,
+                        "roofline": roofline(f["prof"], n, 2, 8)}
+        if head == "mixed":
+            line["mixed_over_fp64"] = value / v64
+            line["spectrum_rel_err_vs_fp64"] = {
+                "bins_ge_1e-3_peak": spectrum_error(r["spectra"], f["spectra"], 1e-3),
+                "bins_ge_1e-6_peak": spectrum_error(r["spectra"], f["spectra"], 1e-6),
+                "all_bins": spectrum_error(r["spectra"], f["spectra"], 0.0),
+                "after_steps": args.warmup + args.steps}
+            line["energy_rel_err_vs_fp64"] = abs(r["energies"]["rho"] - f["energies"]["rho"]) / abs(
+                f["energies"]["rho"])
+            line["friedmann_resid"] = {"mixed": r["energies"]["resid"], "f64": f["energies"]["resid"]}
+    for form in res:
+        line.setdefault("stages_ms", {})[form] = {k: round(v[0], 3) for k, v in res[form]["prof"].items() if v[1]}
+    if not args.no_cpu:
+        print("[bench] cpu baseline (oracle)", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_steps)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

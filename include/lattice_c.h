@@ -137,13 +137,39 @@ int  lat_calc_metric(lat_ctx *ctx);
  * get_hubble, grav_energy functions: out[LAT_NE]. */
 int  lat_energies(lat_ctx *ctx, double *out);
 
+/* ---- initial conditions ---- */
+/* Replaces sample(gamma, m2eff) + the assignment in init_fields
+ * (Pseudospec/openmp_dev/evolve_spectral.f90:262-303,421-506), on the device:
+ * field `field` of fld (momentum = 0) or fldp (momentum = 1) becomes
+ * mean + sample.  The compiler RNG of the reference is replaced by Philox-4x32-10
+ * keyed by (seed, stream); mode (i,j,k) uses counter i + (nx/2+1)(j + ny k). */
+int  lat_sample_field(lat_ctx *ctx, int field, int momentum, double mean, double gamma, double m2eff,
+                      double mpl, double kcut_fac, unsigned long long seed, unsigned long long stream);
+
 /* ---- device pointers for in-process callers (state stays resident) ---- */
 /* Returns device addresses of the SoA state arrays (fld, fldp: nfld*nvol
  * elements of double for LAT_F64, float for LAT_MIXED fluctuations). */
 int  lat_device_state(lat_ctx *ctx, void **fld, void **fldp);
-/* Kernel time accounting: milliseconds spent in the last lat_symp call
- * (HIP events on the context stream), and launch counts per stage. */
-int  lat_last_symp_ms(lat_ctx *ctx, double *ms);
+
+/* ---- measurement (no counterpart in the reference, which only prints
+ *      wall-clock times at evolve_spectral.f90:79,99) ---- */
+enum lat_stage {
+    LAT_ST_X_R2C = 0, /* x-direction R2C (drift fused)            */
+    LAT_ST_Y_FWD,     /* y forward column FFT                      */
+    LAT_ST_Z_LAP,     /* z forward * -k^2 * z inverse              */
+    LAT_ST_Y_INV,     /* y inverse column FFT                      */
+    LAT_ST_X_C2R,     /* x-direction C2R + kick + reductions       */
+    LAT_ST_SCALAR,    /* background scalar update                  */
+    LAT_ST_DRIFT,     /* standalone drift sweep                    */
+    LAT_ST_STENCIL,   /* FD stencil + kick                         */
+    LAT_ST_REDUCE,    /* state reductions                          */
+    LAT_NSTAGE
+};
+/* enable = 1 starts per-launch HIP-event timing on the context stream (and
+ * clears the accumulators); 0 stops it. */
+int  lat_profile(lat_ctx *ctx, int enable);
+/* Accumulated kernel milliseconds and launch counts per stage. */
+int  lat_profile_read(lat_ctx *ctx, double *ms, long long *launches);
 
 #ifdef __cplusplus
 }

@@ -74,6 +74,50 @@ static int push_scalars(Ctx &c)
     return sync(c);
 }
 
+// ---------------------------------------------------------------- profiling
+int prof_begin(Ctx &c, int stage)
+{
+    if (!c.prof.on) return -1;
+    const int i = (int)c.prof.used.size() * 2;
+    while ((int)c.prof.pool.size() < i + 2) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return -1;
+        c.prof.pool.push_back(e);
+    }
+    if (hipEventRecord(c.prof.pool[i], c.stream) != hipSuccess) return -1;
+    c.prof.used.push_back({stage, i});
+    return (int)c.prof.used.size() - 1;
+}
+
+void prof_end(Ctx &c, int idx)
+{
+    if (idx < 0) return;
+    (void)hipEventRecord(c.prof.pool[c.prof.used[idx].second + 1], c.stream);
+}
+
+// Folds recorded event pairs into the per-stage accumulators.
+static int prof_collect(Ctx &c)
+{
+    if (c.prof.used.empty()) return 0;
+    int rc = sync(c);
+    if (rc) return rc;
+    for (const auto &u : c.prof.used) {
+        float ms = 0.f;
+        LAT_HIP(&c, hipEventElapsedTime(&ms, c.prof.pool[u.second], c.prof.pool[u.second + 1]));
+        c.prof.ms[u.first] += ms;
+        c.prof.cnt[u.first] += 1;
+    }
+    c.prof.used.clear();
+    return 0;
+}
+
+struct StageTimer {
+    Ctx &c;
+    int idx;
+    StageTimer(Ctx &cc, int stage) : c(cc), idx(prof_begin(cc, stage)) {}
+    ~StageTimer() { prof_end(c, idx); }
+};
+
 // ---------------------------------------------------------------- Laplacian pipeline
 // Forward R2C (x, y, z) of the nfld state fields into Fk; drift fused if asked.
 template <typename T>
@@ -87,11 +131,16 @@ static int forward_state(Ctx &c, bool drift, int zmode)
     cpx<T> *Fk = (cpx<T> *)c.Fk;
     int rc;
     for (int l = 0; l < c.nfld; ++l) {
+        StageTimer st(c, LAT_ST_X_R2C);
         rc = launch_x_r2c<T>(c, twx, fld + (size_t)l * c.nvol, fldp + (size_t)l * c.nvol, Fk + (size_t)l * c.nk,
-                             drift ? 1 : 0);
+                             drift ? 1 : 0, l);
         if (rc) return rc;
     }
-    if ((rc = launch_col<T>(c, twy, Fk, 1, 0, c.nfld))) return rc;
+    {
+        StageTimer st(c, LAT_ST_Y_FWD);
+        if ((rc = launch_col<T>(c, twy, Fk, 1, 0, c.nfld))) return rc;
+    }
+    StageTimer st(c, LAT_ST_Z_LAP);
     if ((rc = launch_col<T>(c, twz, Fk, 2, zmode, c.nfld))) return rc;
     return 0;
 }
@@ -106,13 +155,18 @@ static int state_force(Ctx &c, bool drift, int epi)
             int rc = launch_drift<T>(c);
             if (rc) return rc;
         }
+        StageTimer st(c, LAT_ST_STENCIL);
         return launch_stencil<T>(c, epi);
     }
     int rc = forward_state<T>(c, drift, 2);
     if (rc) return rc;
     const cpx<T> *twx = (const cpx<T> *)(sizeof(T) == 8 ? (void *)c.twx64 : (void *)c.twx32);
     const cpx<T> *twy = (const cpx<T> *)(sizeof(T) == 8 ? (void *)c.twy64 : (void *)c.twy32);
-    if ((rc = launch_col<T>(c, twy, (cpx<T> *)c.Fk, 1, 1, c.nfld))) return rc;
+    {
+        StageTimer st(c, LAT_ST_Y_INV);
+        if ((rc = launch_col<T>(c, twy, (cpx<T> *)c.Fk, 1, 1, c.nfld))) return rc;
+    }
+    StageTimer st(c, LAT_ST_X_C2R);
     return launch_x_c2r<T>(c, twx, (cpx<T> *)c.Fk, c.nfld, epi, nullptr, (const T *)c.fld, (T *)c.fldp);
 }
 
@@ -284,6 +338,45 @@ static double scale_factor(const Ctx &c, double y)
     return c.cfg.time_var == LAT_CONFORMAL ? y : std::pow(y, 2.0 / 3.0);
 }
 
+static double host_v(const Ctx &c, const double *f)
+{
+    const double *mp = c.cfg.model_params;
+    if (c.cfg.model == LAT_MODEL_PHI4) return 0.25 * mp[0] * (f[0] * f[0]) * (f[0] * f[0]);
+    return 0.5 * mp[0] * (f[0] * f[0]) + 0.5 * mp[1] * (f[0] * f[0]) * (f[1] * f[1]);
+}
+
+// Lattice sums of the physical fields after state_sums(): sum pi^2, sum V,
+// sum -phi lap_raw, and the field means.  The mixed form reassembles them
+// from the FP64 homogeneous parts and the FP64-accumulated fluctuation sums.
+struct PhysSums {
+    double pi2, v, ge, mean[2], mean_pi[2];
+};
+
+static PhysSums phys_sums(const Ctx &c)
+{
+    const DevScalars &s = c.host_sc;
+    const double nv = (double)c.nvol;
+    PhysSums r{};
+    r.ge = s.red[RC_GE];
+    if (!s.mixed) {
+        r.pi2 = s.red[RC_KE];
+        r.v = s.red[RC_PE];
+        for (int l = 0; l < 2; ++l) {
+            r.mean[l] = l < c.nfld ? s.red[RC_M1 + l] / nv : 0.0;
+            r.mean_pi[l] = l < c.nfld ? s.red[RC_S1 + l] / nv : 0.0;
+        }
+        return r;
+    }
+    r.pi2 = s.red[RC_KE];
+    for (int l = 0; l < c.nfld; ++l) {
+        r.pi2 += nv * s.mean_pi[l] * s.mean_pi[l] + 2.0 * s.mean_pi[l] * s.red[RC_S1 + l];
+        r.mean[l] = s.mean_phi[l] + s.red[RC_M1 + l] / nv;
+        r.mean_pi[l] = s.mean_pi[l] + s.red[RC_S1 + l] / nv;
+    }
+    r.v = nv * host_v(c, s.mean_phi) + s.red[RC_PE];
+    return r;
+}
+
 // ---------------------------------------------------------------- host transforms (FP64)
 static int ensure_io(Ctx &c)
 {
@@ -434,6 +527,7 @@ int lat_create(const lat_config *cfg, lat_ctx **out)
     if (rc) { lat_destroy(h); return -3; }
     memset(&c.host_sc, 0, sizeof(DevScalars));
     c.host_sc.yscl = 1.0;
+    c.host_sc.mixed = c.prec == LAT_MIXED ? 1 : 0;
     if (push_scalars(c)) { lat_destroy(h); return -3; }
     hipMemset(c.fld, 0, es * c.nfld * c.nvol);
     hipMemset(c.fldp, 0, es * c.nfld * c.nvol);
@@ -452,8 +546,9 @@ void lat_destroy(lat_ctx *h)
     void *ptrs[] = {c.fld, c.fldp, c.Fk, c.Fk2, c.io64, (c.Fk64 != c.Fk) ? c.Fk64 : nullptr, c.dsc, c.partials,
                     c.spec_part, c.spec_out, c.twx64, c.twy64, c.twz64, c.twx32, c.twy32, c.twz32};
     for (void *p : ptrs)
-        if (p) hipFree(p);
-    if (c.stream) hipStreamDestroy(c.stream);
+        if (p) (void)hipFree(p);
+    for (hipEvent_t e : c.prof.pool) (void)hipEventDestroy(e);
+    if (c.stream) (void)hipStreamDestroy(c.stream);
     delete h;
 }
 
@@ -645,6 +740,23 @@ static int output_spectra_t(Ctx &c, int field, double *out)
     std::vector<double> raw(5 * c.ns);
     LAT_HIP(&c, hipMemcpyAsync(raw.data(), c.spec_out, sizeof(double) * 5 * c.ns, hipMemcpyDeviceToHost, c.stream));
     if ((rc = sync(c))) return rc;
+    if (sizeof(T) == 4) {
+        // Mixed form: the FP32 transform holds the fluctuations only.  Bin 0 is
+        // the k = 0 mode alone (weight 1), so it is set from the FP64 means.
+        const int g = launch_reduce_state<T>(c);
+        if (g < 0) return g;
+        ScalarOps none{};
+        if ((rc = launch_scalar(c, none, g))) return rc;
+        if ((rc = pull_scalars(c))) return rc;
+        c.ke_valid = true;
+        const double nv = (double)c.nvol;
+        const double fphi = nv * c.host_sc.mean_phi[field] + c.host_sc.red[RC_M1 + field];
+        const double fpi = nv * c.host_sc.mean_pi[field] + c.host_sc.red[RC_S1 + field];
+        raw[0] = fphi * fphi;
+        raw[(size_t)c.ns] = fpi * fpi;
+        raw[2 * (size_t)c.ns] = fpi * fphi;
+        raw[3 * (size_t)c.ns] = 0.0;
+    }
     double *outs[4] = {out, out + c.ns, out + 2 * c.ns, out + 3 * c.ns};
     finish_spectra(c, raw, 4, outs);
     return 0;
@@ -679,6 +791,23 @@ int lat_set_state(lat_ctx *h, const double *fld, const double *fldp, int ghost, 
                     soap[(size_t)l * c.nvol + s] = fldp[(size_t)l + (size_t)nf * s];
                 }
             }
+    double mphi[2] = {0, 0}, mpi[2] = {0, 0};
+    if (c.prec == LAT_MIXED) {
+        // split into FP64 homogeneous parts and fluctuations (stored in FP32)
+        for (int l = 0; l < nf; ++l) {
+            double a = 0.0, b = 0.0;
+            for (size_t s = 0; s < c.nvol; ++s) {
+                a += soa[(size_t)l * c.nvol + s];
+                b += soap[(size_t)l * c.nvol + s];
+            }
+            mphi[l] = a / (double)c.nvol;
+            mpi[l] = b / (double)c.nvol;
+            for (size_t s = 0; s < c.nvol; ++s) {
+                soa[(size_t)l * c.nvol + s] -= mphi[l];
+                soap[(size_t)l * c.nvol + s] -= mpi[l];
+            }
+        }
+    }
     int rc = c.prec == LAT_F64 ? upload_fields<double>(c, c.fld, soa.data(), soa.size())
                                : upload_fields<float>(c, c.fld, soa.data(), soa.size());
     if (rc) return rc;
@@ -689,6 +818,11 @@ int lat_set_state(lat_ctx *h, const double *fld, const double *fldp, int ghost, 
     c.host_sc.yscl = yscl;
     c.host_sc.ysclp = ysclp;
     c.host_sc.nd = 0;
+    for (int l = 0; l < 2; ++l) {
+        c.host_sc.mean_phi[l] = mphi[l];
+        c.host_sc.mean_pi[l] = mpi[l];
+        c.host_sc.nu[l] = c.host_sc.mu[l] = c.host_sc.sdphi[l] = 0.0;
+    }
     c.ke_valid = false;
     return push_scalars(c);
 }
@@ -702,10 +836,15 @@ int lat_get_state(lat_ctx *h, double *fld, double *fldp, int ghost, double *yscl
     const int g = ghost, nf = c.nfld;
     const size_t sx = c.nx + 2 * g, sy = c.ny + 2 * g, sz = c.nz + 2 * g;
     std::vector<double> soa((size_t)nf * c.nvol), soap((size_t)nf * c.nvol);
+    if ((rc = pull_scalars(c))) return rc;
+    const bool mixed = c.host_sc.mixed != 0;
     if (fld) {
         rc = c.prec == LAT_F64 ? download_fields<double>(c, soa.data(), c.fld, soa.size())
                                : download_fields<float>(c, soa.data(), c.fld, soa.size());
         if (rc) return rc;
+        if (mixed)
+            for (int l = 0; l < nf; ++l)
+                for (size_t s = 0; s < c.nvol; ++s) soa[(size_t)l * c.nvol + s] += c.host_sc.mean_phi[l];
         for (size_t kk = 0; kk < sz; ++kk)
             for (size_t jj = 0; jj < sy; ++jj)
                 for (size_t ii = 0; ii < sx; ++ii) {
@@ -721,6 +860,9 @@ int lat_get_state(lat_ctx *h, double *fld, double *fldp, int ghost, double *yscl
         rc = c.prec == LAT_F64 ? download_fields<double>(c, soap.data(), c.fldp, soap.size())
                                : download_fields<float>(c, soap.data(), c.fldp, soap.size());
         if (rc) return rc;
+        if (mixed)
+            for (int l = 0; l < nf; ++l)
+                for (size_t s = 0; s < c.nvol; ++s) soap[(size_t)l * c.nvol + s] += c.host_sc.mean_pi[l];
         for (size_t s = 0; s < c.nvol; ++s)
             for (int l = 0; l < nf; ++l) fldp[(size_t)l + (size_t)nf * s] = soap[(size_t)l * c.nvol + s];
     }
@@ -790,9 +932,10 @@ int lat_calc_metric(lat_ctx *h)
     const double a = scale_factor(c, y);
     const bool conf = c.cfg.time_var == LAT_CONFORMAL;
     const double knorm = conf ? 1.0 / std::pow(y, 6) : 1.0 / std::pow(y, 4);
-    const double KE = 0.5 * c.host_sc.red[RC_KE] * knorm / nv;
-    const double PE = c.host_sc.red[RC_PE] / nv;
-    const double GE = 0.5 * c.host_sc.red[RC_GE] / (a * a) / nv;
+    const PhysSums ps = phys_sums(c);
+    const double KE = 0.5 * ps.pi2 * knorm / nv;
+    const double PE = ps.v / nv;
+    const double GE = 0.5 * ps.ge / (a * a) / nv;
     const double rho = KE + PE + GE;
     c.host_sc.ysclp = conf ? -std::sqrt(std::pow(y, 4) * 12.0 * rho) : -a * std::sqrt(rho * 16.0 / 3.0);
     return push_scalars(c);
@@ -810,9 +953,10 @@ int lat_energies(lat_ctx *h, double *out)
     const bool conf = c.cfg.time_var == LAT_CONFORMAL;
     const double knorm = conf ? 1.0 / std::pow(y, 6) : 1.0 / std::pow(y, 4);
     const double hub = conf ? -s.ysclp / 6.0 / (y * y) : -s.ysclp / y / 4.0;
-    const double KE = 0.5 * s.red[RC_KE] * knorm / nv;
-    const double PE = s.red[RC_PE] / nv;
-    const double GE = 0.5 * s.red[RC_GE] / nv / (a * a);
+    const PhysSums ps = phys_sums(c);
+    const double KE = 0.5 * ps.pi2 * knorm / nv;
+    const double PE = ps.v / nv;
+    const double GE = 0.5 * ps.ge / nv / (a * a);
     const double rho = KE + PE + GE;
     const double grav = -3.0 * hub * hub;
     out[LAT_E_A] = a;
@@ -824,8 +968,8 @@ int lat_energies(lat_ctx *h, double *out)
     out[LAT_E_GRAV] = grav;
     out[LAT_E_RESID] = (rho + grav) / rho;
     out[LAT_E_PMOM] = conf ? -s.ysclp * s.ysclp / 12.0 / std::pow(a, 4) : -s.ysclp * s.ysclp * 3.0 / 16.0 / (a * a);
-    out[LAT_E_MEAN1] = s.red[RC_M1] / nv;
-    out[LAT_E_MEAN2] = c.nfld > 1 ? s.red[RC_M2] / nv : 0.0;
+    out[LAT_E_MEAN1] = ps.mean[0];
+    out[LAT_E_MEAN2] = c.nfld > 1 ? ps.mean[1] : 0.0;
     out[LAT_E_GEGRAD] = 0.0;
     return 0;
 }
@@ -839,10 +983,52 @@ int lat_device_state(lat_ctx *h, void **fld, void **fldp)
     return 0;
 }
 
-int lat_last_symp_ms(lat_ctx *h, double *ms)
+int lat_sample_field(lat_ctx *h, int field, int momentum, double mean, double gamma, double m2eff, double mpl,
+                     double kcut_fac, unsigned long long seed, unsigned long long stream)
 {
     CTX_OR_FAIL(h);
-    if (ms) *ms = 0.0;
+    if (field < 0 || field >= c.nfld || momentum < 0 || momentum > 1 || !(mpl != 0.0)) return -1;
+    int rc = ensure_io(c);
+    if (rc) return rc;
+    if ((rc = sample_to_io(c, gamma, m2eff, mpl, kcut_fac, seed, stream))) return rc;
+    void *base = momentum ? c.fldp : c.fld;
+    if (c.prec == LAT_F64) rc = store_io_field<double>(c, (double *)base + (size_t)field * c.nvol, mean);
+    else rc = store_io_field<float>(c, (float *)base + (size_t)field * c.nvol, 0.0);
+    if (rc) return rc;
+    c.ke_valid = false;
+    if (c.prec == LAT_MIXED) {
+        // the homogeneous value goes to the FP64 part, the sample is the fluctuation
+        if ((rc = pull_scalars(c))) return rc;
+        (momentum ? c.host_sc.mean_pi : c.host_sc.mean_phi)[field] = mean;
+        return push_scalars(c);
+    }
+    return sync(c);
+}
+
+int lat_profile(lat_ctx *h, int enable)
+{
+    CTX_OR_FAIL(h);
+    int rc = prof_collect(c);
+    if (rc) return rc;
+    if (enable) {
+        for (int s = 0; s < LAT_NSTAGE; ++s) {
+            c.prof.ms[s] = 0.0;
+            c.prof.cnt[s] = 0;
+        }
+    }
+    c.prof.on = enable != 0;
+    return 0;
+}
+
+int lat_profile_read(lat_ctx *h, double *ms, long long *launches)
+{
+    CTX_OR_FAIL(h);
+    int rc = prof_collect(c);
+    if (rc) return rc;
+    for (int s = 0; s < LAT_NSTAGE; ++s) {
+        if (ms) ms[s] = c.prof.ms[s];
+        if (launches) launches[s] = c.prof.cnt[s];
+    }
     return 0;
 }
 

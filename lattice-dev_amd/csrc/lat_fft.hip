@@ -13,6 +13,7 @@
 // Every 1-D transform is LDS-resident (fft_core.h).  The k-space array is
 // FFTW's half-complex layout with the row pitch padded to P >= nx/2+1.
 #include "lat_internal.h"
+#include "model.h"
 
 namespace lat {
 
@@ -68,41 +69,24 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NCH], double *out
     }
 }
 
-// ---------------------------------------------------------------- model
-template <typename T>
-__device__ __forceinline__ void model_dv(const ModelDev &m, const T *f, T *dv)
-{
-    if (m.model == LAT_MODEL_PHI4) {
-        dv[0] = T(m.p0) * f[0] * f[0] * f[0];
-    } else {
-        dv[0] = T(m.p0) * f[0] + T(m.p1) * f[0] * (f[1] * f[1]);
-        dv[1] = T(m.p1) * (f[0] * f[0]) * f[1];
-    }
-}
-template <typename T>
-__device__ __forceinline__ double model_v(const ModelDev &m, const T *f)
-{
-    const double a = f[0];
-    if (m.model == LAT_MODEL_PHI4) return 0.25 * m.p0 * (a * a) * (a * a);
-    const double b = f[1];
-    return 0.5 * m.p0 * (a * a) + 0.5 * m.p1 * (a * a) * (b * b);
-}
-
 // ---------------------------------------------------------------- x R2C
+// With DRIFT: phi += dc0*pi (+ dc1*pi) first, written back (mixed form: on
+// the fluctuations, then minus the re-centering shift nu[field]).
 template <typename T, int N, int L, bool DRIFT>
 __global__ __launch_bounds__(TPB) void k_x_r2c(T *__restrict__ f, const T *__restrict__ pi,
                                                cpx<T> *__restrict__ Fk, const cpx<T> *__restrict__ tw,
-                                               int rows, int P, const DevScalars *__restrict__ dsc)
+                                               int rows, int P, const DevScalars *__restrict__ dsc, int field)
 {
     constexpr int M = N / 2;
     constexpr int LS = M + 1;
     __shared__ cpx<T> buf[L * LS];
     int nd = 0;
-    T dc0 = 0, dc1 = 0;
+    T dc0 = 0, dc1 = 0, nu = 0;
     if constexpr (DRIFT) {
         nd = dsc->nd;
         dc0 = T(dsc->dc[0]);
         dc1 = T(dsc->dc[1]);
+        if (dsc->mixed) nu = T(dsc->nu[field]);
     }
     int t0, t1, ts;
     tile_range((rows + L - 1) / L, t0, t1, ts);
@@ -123,6 +107,10 @@ __global__ __launch_bounds__(TPB) void k_x_r2c(T *__restrict__ f, const T *__res
                         if (nd > 1) {
                             v.x = v.x + dc1 * p.x;
                             v.y = v.y + dc1 * p.y;
+                        }
+                        if constexpr (sizeof(T) == 4) {
+                            v.x = v.x - nu;
+                            v.y = v.y - nu;
                         }
                         *src = v;
                     }
@@ -233,6 +221,16 @@ __global__ __launch_bounds__(TPB) void k_x_c2r(const cpx<T> *__restrict__ Fk, co
         hA = T(dsc->hA);
         hB = T(dsc->hB);
     }
+    float pb32[MAXF] = {0, 0};
+    double pb64[MAXF] = {0, 0};
+    T mu[MAXF] = {0, 0};
+    if constexpr (sizeof(T) == 4 && EPI != EPI_STORE) {
+        for (int fi = 0; fi < MAXF; ++fi) {
+            pb64[fi] = dsc->mean_phi[fi];
+            pb32[fi] = float(pb64[fi]);
+            if constexpr (EPI == EPI_KICK) mu[fi] = T(dsc->mu[fi]);
+        }
+    }
     int t0, t1, ts;
     tile_range((rows + L - 1) / L, t0, t1, ts);
     for (int t = t0; t < t1; t += ts) {
@@ -288,18 +286,26 @@ __global__ __launch_bounds__(TPB) void k_x_c2r(const cpx<T> *__restrict__ Fk, co
             } else {
                 T phi[MAXF] = {0, 0}, dv[MAXF] = {0, 0};
                 for (int fi = 0; fi < nf; ++fi) phi[fi] = fld[(size_t)fi * nvol + site];
-                if constexpr (EPI == EPI_KICK) model_dv(md, phi, dv);
-                acc[RC_PE] += model_v(md, phi);
+                if constexpr (sizeof(T) == 4) {
+                    // mixed form: phi, p hold fluctuations; V' and V as differences
+                    if constexpr (EPI == EPI_KICK) model_ddv(md, pb32, phi, dv);
+                    acc[RC_PE] += model_dvv(md, pb64, phi);
+                } else {
+                    if constexpr (EPI == EPI_KICK) model_dv(md, phi, dv);
+                    acc[RC_PE] += model_v(md, phi);
+                }
                 acc[RC_M1] += double(phi[0]);
                 acc[RC_M2] += double(phi[1]);
                 for (int fi = 0; fi < nf; ++fi) {
                     T p = fldp[(size_t)fi * nvol + site];
                     if constexpr (EPI == EPI_KICK) {
                         p = p + (hA * lap[fi] - hB * dv[fi]);
+                        if constexpr (sizeof(T) == 4) p = p - mu[fi];
                         fldp[(size_t)fi * nvol + site] = p;
                     }
                     acc[RC_GE] -= double(phi[fi]) * double(lap[fi]);
                     acc[RC_KE] += double(p) * double(p);
+                    acc[RC_S1 + fi] += double(p);
                 }
             }
         }
@@ -329,7 +335,7 @@ constexpr int col_tile() // adjacent kx columns per workgroup for the y/z kernel
 #define LAT_FOR_SIZES(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(48) X(96) X(192)
 
 template <typename T>
-int launch_x_r2c(Ctx &c, const cpx<T> *tw, T *f, const T *pi, cpx<T> *Fk, int drift)
+int launch_x_r2c(Ctx &c, const cpx<T> *tw, T *f, const T *pi, cpx<T> *Fk, int drift, int field)
 {
     const int rows = c.ny * c.nz;
     switch (c.nx) {
@@ -339,10 +345,10 @@ int launch_x_r2c(Ctx &c, const cpx<T> *tw, T *f, const T *pi, cpx<T> *Fk, int dr
         const int g = grid_for((rows + L - 1) / L);                                                      \
         if (drift)                                                                                       \
             hipLaunchKernelGGL((k_x_r2c<T, NN, L, true>), dim3(g), dim3(TPB), 0, c.stream, f, pi, Fk, tw, \
-                               rows, c.P, c.dsc);                                                        \
+                               rows, c.P, c.dsc, field);                                                 \
         else                                                                                             \
             hipLaunchKernelGGL((k_x_r2c<T, NN, L, false>), dim3(g), dim3(TPB), 0, c.stream, f, pi, Fk, tw, \
-                               rows, c.P, c.dsc);                                                        \
+                               rows, c.P, c.dsc, field);                                                 \
         break;                                                                                           \
     }
         LAT_FOR_SIZES(CASE)
@@ -423,8 +429,8 @@ int launch_x_c2r(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int nf, int epi, T *out, 
     }
 }
 
-template int launch_x_r2c<double>(Ctx &, const cpx<double> *, double *, const double *, cpx<double> *, int);
-template int launch_x_r2c<float>(Ctx &, const cpx<float> *, float *, const float *, cpx<float> *, int);
+template int launch_x_r2c<double>(Ctx &, const cpx<double> *, double *, const double *, cpx<double> *, int, int);
+template int launch_x_r2c<float>(Ctx &, const cpx<float> *, float *, const float *, cpx<float> *, int, int);
 template int launch_col<double>(Ctx &, const cpx<double> *, cpx<double> *, int, int, int);
 template int launch_col<float>(Ctx &, const cpx<float> *, cpx<float> *, int, int, int);
 template int launch_x_c2r<double>(Ctx &, const cpx<double> *, cpx<double> *, int, int, double *, const double *,

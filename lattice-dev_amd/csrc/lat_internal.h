@@ -5,6 +5,8 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "../../include/lattice_c.h"
 #include "fft_core.h"
@@ -16,15 +18,17 @@ constexpr int MAX_GRID = 4096; // cap of grid-stride launches (= max reduction p
 constexpr int NRED = 8;        // reduction channels per partial
 
 // Reduction channels written by the sweep kernels (FP64 partial per block).
+// FP64 form: sums of the full fields.  Mixed form: sums of the fluctuations
+// (GE uses dphi, PE is the sum of V(phi) - V(mean)).
 enum RedChan {
     RC_GE = 0,   // sum over fields and sites of -phi * lap_raw (lap without a factors)
     RC_PE = 1,   // sum of V(phi) (mixed form: sum of V(phi) - V(mean))
     RC_KE = 2,   // sum over fields of pi^2 (mixed form: of dpi^2)
-    RC_S1 = 3,   // mixed form: sum of dpi for field 0
-    RC_S2 = 4,   // mixed form: sum of dpi for field 1
-    RC_M1 = 5,   // sum of phi (or dphi) field 0
-    RC_M2 = 6,   // sum of phi (or dphi) field 1
-    RC_X = 7,    // auxiliary
+    RC_S1 = 3,   // sum of pi (dpi) field 0
+    RC_S2 = 4,   // sum of pi (dpi) field 1
+    RC_M1 = 5,   // sum of phi (dphi) field 0
+    RC_M2 = 6,   // sum of phi (dphi) field 1
+    RC_X = 7,    // auxiliary (FD: GRAD2 stencil sum)
 };
 
 // Background (FRW) scalars and step coefficients, resident in device memory.
@@ -35,10 +39,13 @@ struct DevScalars {
     double hA, hB;            // next kick: pi += hA*lap_raw - hB*V'
     double dc[4];             // pending drift coefficients phi += dc[q]*pi, in order
     int nd;                   // number of pending drifts
-    int pad_;
-    double mean_phi[4];       // mixed form: FP64 homogeneous parts of phi, pi
-    double mean_pi[4];
-    double dv_mean[4];        // mixed form: V'(mean) used by the last kick
+    int mixed;                // state holds FP32 fluctuations about the FP64 means below
+    // Mixed form: phi_l(x) = mean_phi[l] + dphi_l(x), pi_l(x) = mean_pi[l] + dpi_l(x).
+    double mean_phi[2];
+    double mean_pi[2];
+    double nu[2];             // re-centering: the next drift sweep subtracts nu from dphi
+    double mu[2];             // re-centering: the next kick subtracts mu from dpi
+    double sdphi[2];          // sum of dphi after the pending drifts (before nu)
 };
 
 enum ScalarOpType {
@@ -63,8 +70,18 @@ struct ModelDev {
     double p0, p1; // phi4: lambda; twofield: m2, g2
 };
 
+// Per-stage HIP-event accounting (lat_profile / lat_profile_read).
+struct Prof {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<int, int>> used; // (stage, first event index of the pair)
+    double ms[LAT_NSTAGE] = {};
+    long long cnt[LAT_NSTAGE] = {};
+};
+
 struct Ctx {
     lat_config cfg;
+    Prof prof;
     int nx, ny, nz, nxh, P; // P = pitch (complex elements) of a half-complex row
     size_t nvol, nk;        // nk = P*ny*nz
     double dx, dk;
@@ -101,7 +118,7 @@ size_t elem_size(const Ctx &c);
 // x-direction R2C of each real row of `f` (nvol), into half-complex rows of Fk.
 // If drift != 0, first applies the pending drifts of dsc to f in place using pi.
 template <typename T>
-int launch_x_r2c(Ctx &c, const cpx<T> *tw, T *f, const T *pi, cpx<T> *Fk, int drift);
+int launch_x_r2c(Ctx &c, const cpx<T> *tw, T *f, const T *pi, cpx<T> *Fk, int drift, int field = 0);
 // Column FFT along y (axis 1) or z (axis 2) of a half-complex array.
 // mode 0: forward; 1: inverse; 2: forward, multiply by -k^2 dk^2/nvol, inverse.
 template <typename T>
@@ -127,6 +144,15 @@ int grid_for(int tiles);
 // ---- spectra (lat_spec.hip) ----
 template <typename T>
 int launch_bin(Ctx &c, const cpx<T> *F1, const cpx<T> *F2, int nch);
+
+// ---- initial conditions (lat_ic.hip) ----
+int sample_to_io(Ctx &c, double gamma, double m2eff, double mpl, double kcut_fac, uint64_t seed, uint64_t stream);
+template <typename T>
+int store_io_field(Ctx &c, T *dst, double add);
+
+// ---- profiling ----
+int prof_begin(Ctx &c, int stage); // returns a pair index or -1
+void prof_end(Ctx &c, int idx);
 
 // ---- helpers ----
 int hip_fail(Ctx *c, hipError_t e, const char *where);

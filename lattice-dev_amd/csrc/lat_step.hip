@@ -9,6 +9,7 @@
 // the scalar part of the split operators between two kicks, so a whole
 // Yoshida step runs without a host round trip.
 #include "lat_internal.h"
+#include "model.h"
 
 namespace lat {
 
@@ -33,40 +34,39 @@ __device__ __forceinline__ void block_reduce_store2(double (&v)[NCH], double *ou
     }
 }
 
+// Per-site potential terms of the two forms: full V, V' (FP64 form) or the
+// difference forms about the homogeneous parts (mixed form, T = float).
 template <typename T>
-__device__ __forceinline__ void mdv(const ModelDev &m, const T *f, T *dv)
+__device__ __forceinline__ void site_dv(const ModelDev &m, const float *pb32, const T *f, T *dv)
 {
-    if (m.model == LAT_MODEL_PHI4) {
-        dv[0] = T(m.p0) * f[0] * f[0] * f[0];
-    } else {
-        dv[0] = T(m.p0) * f[0] + T(m.p1) * f[0] * (f[1] * f[1]);
-        dv[1] = T(m.p1) * (f[0] * f[0]) * f[1];
-    }
+    if constexpr (sizeof(T) == 4) model_ddv(m, pb32, f, dv);
+    else model_dv(m, f, dv);
 }
 template <typename T>
-__device__ __forceinline__ double mv(const ModelDev &m, const T *f)
+__device__ __forceinline__ double site_v(const ModelDev &m, const double *pb64, const T *f)
 {
-    const double a = f[0];
-    if (m.model == LAT_MODEL_PHI4) return 0.25 * m.p0 * (a * a) * (a * a);
-    const double b = f[1];
-    return 0.5 * m.p0 * (a * a) + 0.5 * m.p1 * (a * a) * (b * b);
+    if constexpr (sizeof(T) == 4) return model_dvv(m, pb64, f);
+    else return model_v(m, f);
 }
 
 // ---------------------------------------------------------------- drift
 // phi_l += dc[q] * pi_l for the pending drifts q < nd (hamiltonian_conformal.f90:98,
-// hamiltonian_cosmic.f90:107); the p-update was done by k_scalar.
+// hamiltonian_cosmic.f90:107); the p-update was done by k_scalar.  Mixed
+// form: on the fluctuations, minus the re-centering shift nu[l].
 template <typename T>
-__global__ __launch_bounds__(TPB) void k_drift(T *__restrict__ f, const T *__restrict__ p, size_t n,
+__global__ __launch_bounds__(TPB) void k_drift(T *__restrict__ f, const T *__restrict__ p, size_t nvol, int nf,
                                                const DevScalars *__restrict__ dsc)
 {
     const int nd = dsc->nd;
     T dc[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) dc[q] = T(dsc->dc[q]);
+    const size_t n = nvol * nf;
     for (size_t i = blockIdx.x * (size_t)TPB + threadIdx.x; i < n; i += (size_t)gridDim.x * TPB) {
         T x = f[i];
         const T pi = p[i];
         for (int q = 0; q < nd; ++q) x = x + dc[q] * pi;
+        if constexpr (sizeof(T) == 4) x = x - T(dsc->nu[i / nvol]);
         f[i] = x;
     }
 }
@@ -77,7 +77,10 @@ int launch_drift(Ctx &c)
     const size_t n = (size_t)c.nfld * c.nvol;
     int g = (int)((n + TPB - 1) / TPB);
     if (g > 8192) g = 8192;
-    hipLaunchKernelGGL((k_drift<T>), dim3(g), dim3(TPB), 0, c.stream, (T *)c.fld, (const T *)c.fldp, n, c.dsc);
+    const int pi = prof_begin(c, LAT_ST_DRIFT);
+    hipLaunchKernelGGL((k_drift<T>), dim3(g), dim3(TPB), 0, c.stream, (T *)c.fld, (const T *)c.fldp, c.nvol, c.nfld,
+                       c.dsc);
+    prof_end(c, pi);
     LAT_HIP(&c, hipGetLastError());
     return 0;
 }
@@ -86,8 +89,14 @@ int launch_drift(Ctx &c)
 // partials: PE = sum V, KE = sum pi^2, M1/M2 = sum phi_l (no Laplacian).
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_reduce_state(const T *__restrict__ f, const T *__restrict__ p, size_t nvol,
-                                                      int nf, ModelDev md, double *__restrict__ partials)
+                                                      int nf, ModelDev md, const DevScalars *__restrict__ dsc,
+                                                      double *__restrict__ partials)
 {
+    double pb64[2] = {0, 0};
+    if constexpr (sizeof(T) == 4) {
+        pb64[0] = dsc->mean_phi[0];
+        pb64[1] = dsc->mean_phi[1];
+    }
     double acc[NRED] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (size_t i = blockIdx.x * (size_t)TPB + threadIdx.x; i < nvol; i += (size_t)gridDim.x * TPB) {
         T phi[2] = {0, 0};
@@ -95,8 +104,9 @@ __global__ __launch_bounds__(TPB) void k_reduce_state(const T *__restrict__ f, c
             phi[l] = f[(size_t)l * nvol + i];
             const double pp = p[(size_t)l * nvol + i];
             acc[RC_KE] += pp * pp;
+            acc[RC_S1 + l] += pp;
         }
-        acc[RC_PE] += mv(md, phi);
+        acc[RC_PE] += site_v(md, pb64, phi);
         acc[RC_M1] += double(phi[0]);
         acc[RC_M2] += double(phi[1]);
     }
@@ -109,8 +119,10 @@ int launch_reduce_state(Ctx &c)
     ModelDev md{c.cfg.model, c.nfld, c.cfg.model_params[0], c.cfg.model_params[1]};
     const size_t blocks = (c.nvol + TPB - 1) / TPB;
     int g = grid_for(blocks > (size_t)MAX_GRID ? MAX_GRID : (int)blocks);
+    const int pi = prof_begin(c, LAT_ST_REDUCE);
     hipLaunchKernelGGL((k_reduce_state<T>), dim3(g), dim3(TPB), 0, c.stream, (const T *)c.fld, (const T *)c.fldp,
-                       c.nvol, c.nfld, md, c.partials);
+                       c.nvol, c.nfld, md, c.dsc, c.partials);
+    prof_end(c, pi);
     LAT_HIP(&c, hipGetLastError());
     return g;
 }
@@ -138,6 +150,16 @@ __global__ __launch_bounds__(TPB) void k_stencil(const T *__restrict__ fld, T *_
     if constexpr (EPI == EPI_KICK) {
         hA = T(dsc->hA);
         hB = T(dsc->hB);
+    }
+    float pb32[2] = {0, 0};
+    double pb64[2] = {0, 0};
+    T mu[2] = {0, 0};
+    if constexpr (sizeof(T) == 4) {
+        for (int l = 0; l < 2; ++l) {
+            pb64[l] = dsc->mean_phi[l];
+            pb32[l] = float(pb64[l]);
+            if constexpr (EPI == EPI_KICK) mu[l] = T(dsc->mu[l]);
+        }
     }
     double acc[NRED] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (size_t s = blockIdx.x * (size_t)TPB + threadIdx.x; s < nvol; s += (size_t)gridDim.x * TPB) {
@@ -173,16 +195,18 @@ __global__ __launch_bounds__(TPB) void k_stencil(const T *__restrict__ fld, T *_
             for (int l = 0; l < nf; ++l) out[(size_t)l * nvol + s] = lap[l];
         } else {
             T dv[2] = {0, 0};
-            if constexpr (EPI == EPI_KICK) mdv(md, phi, dv);
-            acc[RC_PE] += mv(md, phi);
+            if constexpr (EPI == EPI_KICK) site_dv(md, pb32, phi, dv);
+            acc[RC_PE] += site_v(md, pb64, phi);
             for (int l = 0; l < nf; ++l) {
                 T p = fldp[(size_t)l * nvol + s];
                 if constexpr (EPI == EPI_KICK) {
                     p = p + (hA * lap[l] - hB * dv[l]);
+                    if constexpr (sizeof(T) == 4) p = p - mu[l];
                     fldp[(size_t)l * nvol + s] = p;
                 }
                 acc[RC_GE] -= double(phi[l]) * double(lap[l]);
                 acc[RC_KE] += double(p) * double(p);
+                acc[RC_S1 + l] += double(p);
             }
             acc[RC_M1] += double(phi[0]);
             acc[RC_M2] += double(phi[1]);
@@ -251,6 +275,7 @@ int launch_stencil_io(Ctx &c, const double *f, double *out)
 struct ScalarEnv {
     int time_var;
     double nvol;
+    ModelDev md;
 };
 
 // One workgroup: ordered FP64 reduction of `npart` partials (Neumaier
@@ -284,15 +309,39 @@ __global__ __launch_bounds__(TPB) void k_scalar(DevScalars *__restrict__ sc, con
     }
     if (threadIdx.x != 0) return;
     sc->nd = 0;
+    const bool mixed = sc->mixed != 0;
+    const int nf = env.md.nfld;
+    if (mixed && npart > 0) {
+        sc->sdphi[0] = sc->red[RC_M1];
+        sc->sdphi[1] = sc->red[RC_M2];
+    }
     double y = sc->yscl, p = sc->ysclp;
     const bool conf = env.time_var == LAT_CONFORMAL;
+    // Mixed form: fold the mean of the pending drifted fluctuations into the
+    // FP64 part (the drift sweep subtracts nu).  Must precede any use of
+    // mean_phi by a kick, which sees the shifted split.
+    bool nu_done = false;
+    auto apply_nu = [&]() {
+        if (!mixed || nu_done) return;
+        nu_done = true;
+        for (int l = 0; l < 2; ++l) {
+            double nu = 0.0;
+            if (sc->nd > 0 && l < nf) {
+                nu = (double)(float)(sc->sdphi[l] / env.nvol);
+                sc->mean_phi[l] += nu;
+                sc->sdphi[l] -= nu * env.nvol;
+            }
+            sc->nu[l] = nu;
+        }
+    };
     for (int q = 0; q < ops.n; ++q) {
         const double h = ops.h[q];
         switch (ops.type[q]) {
         case OP_FINISH_KICK: {
             // hamiltonian_conformal.f90:172,203-205; hamiltonian_cosmic.f90:153,209-211
             const double kh = sc->kick_h;
-            const double PE = sc->red[RC_PE] / env.nvol;
+            double PE = sc->red[RC_PE] / env.nvol;
+            if (mixed) PE += model_v64(env.md, sc->mean_phi);
             if (conf) {
                 const double GE2 = sc->red[RC_GE] / (y * y) / env.nvol;
                 p = p - (y * y * y) * (GE2 + 4.0 * PE) * kh;
@@ -305,8 +354,22 @@ __global__ __launch_bounds__(TPB) void k_scalar(DevScalars *__restrict__ sc, con
         }
         case OP_DRIFT: {
             // hamiltonian_conformal.f90:98-99,116-117 / hamiltonian_cosmic.f90:107-108,125-126
-            if (sc->nd < 4) sc->dc[sc->nd++] = h / (y * y);
-            const double K2 = sc->red[RC_KE] / env.nvol;
+            const double cdr = h / (y * y);
+            if (sc->nd < 4) sc->dc[sc->nd++] = cdr;
+            double K2 = sc->red[RC_KE] / env.nvol;
+            if (mixed) {
+                // <pi^2> = sum_l (pb_l^2 + 2 pb_l <dpi_l>) + <dpi^2>
+                double cross = 0.0, hom = 0.0;
+                for (int l = 0; l < nf; ++l) {
+                    hom += sc->mean_pi[l] * sc->mean_pi[l];
+                    cross += 2.0 * sc->mean_pi[l] * sc->red[RC_S1 + l];
+                }
+                K2 = hom + (cross + sc->red[RC_KE]) / env.nvol;
+                for (int l = 0; l < nf; ++l) {
+                    sc->mean_phi[l] += cdr * sc->mean_pi[l];
+                    sc->sdphi[l] += cdr * sc->red[RC_S1 + l];
+                }
+            }
             p = p + h * K2 / (y * y * y);
             break;
         }
@@ -324,6 +387,18 @@ __global__ __launch_bounds__(TPB) void k_scalar(DevScalars *__restrict__ sc, con
                 sc->hA = h * (y * y) / (a * a);
                 sc->hB = h * (y * y);
             }
+            if (mixed) {
+                // homogeneous part of the kick in FP64; the kick sweep applies
+                // hA lap - hB dV' - mu to the fluctuations
+                apply_nu();
+                double dvm[2];
+                model_dv64(env.md, sc->mean_phi, dvm);
+                for (int l = 0; l < nf; ++l) {
+                    const double mu = (double)(float)(sc->red[RC_S1 + l] / env.nvol);
+                    sc->mu[l] = mu;
+                    sc->mean_pi[l] += -sc->hB * dvm[l] + mu;
+                }
+            }
             break;
         }
         case OP_CLEAR_DRIFT:
@@ -333,14 +408,18 @@ __global__ __launch_bounds__(TPB) void k_scalar(DevScalars *__restrict__ sc, con
             break;
         }
     }
+    apply_nu();
     sc->yscl = y;
     sc->ysclp = p;
 }
 
 int launch_scalar(Ctx &c, const ScalarOps &ops, int npart)
 {
-    ScalarEnv env{c.cfg.time_var, (double)c.nvol};
+    ScalarEnv env{c.cfg.time_var, (double)c.nvol,
+                  ModelDev{c.cfg.model, c.nfld, c.cfg.model_params[0], c.cfg.model_params[1]}};
+    const int pi = prof_begin(c, LAT_ST_SCALAR);
     hipLaunchKernelGGL(k_scalar, dim3(1), dim3(TPB), 0, c.stream, c.dsc, c.partials, npart, ops, env);
+    prof_end(c, pi);
     LAT_HIP(&c, hipGetLastError());
     return 0;
 }

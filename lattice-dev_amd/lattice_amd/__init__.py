@@ -57,8 +57,9 @@ EXPORTS = [
     "lat_stencil_laplacian", "lat_num_bins", "lat_spectrum_3d", "lat_crossspec_3d",
     "lat_output_spectra", "lat_set_state", "lat_get_state", "lat_get_scalars",
     "lat_set_scalars", "lat_split", "lat_symp_o2step", "lat_symp", "lat_calc_metric",
-    "lat_energies", "lat_device_state", "lat_last_symp_ms",
+    "lat_energies", "lat_sample_field", "lat_device_state", "lat_profile", "lat_profile_read",
 ]
+STAGES = ["x_r2c", "y_fwd", "z_lap", "y_inv", "x_c2r", "scalar", "drift", "stencil", "reduce"]
 
 
 def lib():
@@ -293,3 +294,20 @@ class Lattice:
         f, p = ctypes.c_void_p(), ctypes.c_void_p()
         self._check(lib().lat_device_state(self._h, ctypes.byref(f), ctypes.byref(p)), "device_state")
         return f.value, p.value
+
+    # ---------------------------------------------------------------- initial conditions
+    def sample_field(self, field, momentum, mean, gamma, m2eff, mpl, seed, stream, kcut_fac=1.0):
+        """sample() + init_fields assignment (evolve_spectral.f90:262-303,421-506) on the device."""
+        self._check(lib().lat_sample_field(self._h, field, momentum, ctypes.c_double(mean), ctypes.c_double(gamma),
+                                           ctypes.c_double(m2eff), ctypes.c_double(mpl), ctypes.c_double(kcut_fac),
+                                           ctypes.c_ulonglong(seed), ctypes.c_ulonglong(stream)), "sample_field")
+
+    # ---------------------------------------------------------------- measurement
+    def profile(self, enable=True):
+        self._check(lib().lat_profile(self._h, 1 if enable else 0), "profile")
+
+    def profile_read(self):
+        ms = (ctypes.c_double * len(STAGES))()
+        cnt = (ctypes.c_longlong * len(STAGES))()
+        self._check(lib().lat_profile_read(self._h, ms, cnt), "profile_read")
+        return {s: (ms[i], cnt[i]) for i, s in enumerate(STAGES)}

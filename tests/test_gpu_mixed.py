@@ -1,0 +1,84 @@
+"""Mixed form (FP32 fluctuation fields and FFTs about FP64 homogeneous parts,
+FP64 background and reductions) against the plain FP64 GPU form, from
+identical initial conditions.  Tolerances are the stated errors of the mixed
+form at these sizes/run lengths."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+PHI0, DPHI0 = 1.0093430384226378929425913902459, -0.7137133070120812430962278466136
+
+
+def run(gpu, prec, n, lap, steps, dt, chunks, seed=11):
+    L, g2 = 10.0, 2.5e5
+    lat = gpu.Lattice(n, nfld=2, length=L, precision=prec, laplacian=lap, stencil=gpu.STENCIL_7PT,
+                      time_var=gpu.COSMIC, model=gpu.MODEL_TWOFIELD, model_params=(1.0, g2))
+    lat.sample_field(0, 0, PHI0, -0.25, 1.0, 2e5, seed, 0)
+    lat.sample_field(0, 1, DPHI0, 0.25, 1.0, 2e5, seed, 1)
+    lat.sample_field(1, 0, 0.0, -0.25, g2 * PHI0**2, 2e5, seed, 2)
+    lat.sample_field(1, 1, 0.0, 0.25, g2 * PHI0**2, 2e5, seed, 3)
+    lat.set_scalars(1.0, 0.0)
+    lat.calc_metric()
+    e0 = lat.energies()
+    spectra = []
+    for _ in range(chunks):
+        lat.symp4(dt, steps // chunks)
+        spectra.append((lat.output_spectra(0), lat.output_spectra(1)))
+    e1 = lat.energies()
+    st = lat.get_state()
+    lat.close()
+    return e0, e1, spectra, st
+
+
+def spec_err(a, b, floor=1e-6):
+    """max relative error of S_phi and S_chi over k>0 bins that carry at least
+    `floor` of the peak bin power (FP32 transforms have an absolute roundoff
+    floor ~1e-7 of the rms mode amplitude; bins beyond the initial-condition
+    cutoff carry ~1e-9 of the peak and are resolved only in FP64)."""
+    err = 0.0
+    for (a0, a1), (b0, b1) in zip(a, b):
+        for x, y in ((a0[0], b0[0]), (a1[0], b1[0])):
+            x, y = x[1:], y[1:]
+            m = np.abs(y) >= floor * np.abs(y).max()
+            err = max(err, float(np.max(np.abs(x[m] - y[m]) / np.abs(y[m]))))
+    return err
+
+
+@pytest.mark.parametrize("lap", ["spectral", "fd7"])
+def test_mixed_tracks_fp64(gpu, lap):
+    print("case mixed", lap, flush=True)
+    n, steps, dt = 32, 200, 1e-3
+    lp = gpu.LAP_SPECTRAL if lap == "spectral" else gpu.LAP_STENCIL
+    e0d, e1d, sd, std_ = run(gpu, gpu.F64, n, lp, steps, dt, 4)
+    e0m, e1m, sm, stm = run(gpu, gpu.MIXED, n, lp, steps, dt, 4)
+    # initial state identical up to FP32 rounding of the fluctuations
+    assert abs(e0m["rho"] - e0d["rho"]) <= 1e-9 * abs(e0d["rho"])
+    assert abs(e1m["rho"] - e1d["rho"]) <= 1e-9 * abs(e1d["rho"])
+    assert abs(e1m["a"] - e1d["a"]) <= 1e-11
+    assert abs(e1m["resid"]) < 1e-7 and abs(e1d["resid"]) < 1e-7
+    err3, err6 = spec_err(sm, sd, 1e-3), spec_err(sm, sd, 1e-6)
+    print("mixed spectrum max rel err: floor 1e-3", err3, "floor 1e-6", err6, "all bins", spec_err(sm, sd, 0.0),
+          flush=True)
+    assert err3 < 1e-5
+    assert err6 < 1e-4
+    # fluctuation fields
+    for l in range(2):
+        df = stm[0][l] - std_[0][l]
+        ref = std_[0][l] - std_[0][l].mean()
+        assert np.linalg.norm(df) <= 1e-4 * np.linalg.norm(ref)
+
+
+def test_mixed_state_roundtrip(gpu):
+    n = 16
+    rng = np.random.default_rng(3)
+    fld = np.empty((2, n, n, n))
+    fld[0] = PHI0 + 1e-6 * rng.standard_normal((n, n, n))
+    fld[1] = 1e-6 * rng.standard_normal((n, n, n))
+    fldp = DPHI0 + 1e-6 * rng.standard_normal((2, n, n, n))
+    with gpu.Lattice(n, nfld=2, length=10.0, precision=gpu.MIXED) as lat:
+        lat.set_state(fld, fldp, 1.0, -0.5)
+        f, p, y, py = lat.get_state()
+    for l in range(2):  # fluctuations ~1e-6 stored in FP32: ~1e-13 absolute
+        assert np.abs(f[l] - fld[l]).max() <= 1e-12
+        assert np.abs(p[l] - fldp[l]).max() <= 1e-12
