@@ -130,6 +130,13 @@ int  lat_symp_o2step(lat_ctx *ctx, double dt, double w1, double w2);
 /* Replaces symp2/symp4/symp6/symp8(dt, nsteps): Pseudospec/mpi_dev/integrator.f90:11-110
  * and openmp_dev/evolve_spectral.f90:129-227.  State stays on the device. */
 int  lat_symp(lat_ctx *ctx, int order, double dt, int nsteps);
+/* Host-only (no device work): the linear sequence of split operators that
+ * lat_symp(order, dt, nsteps) executes, as (type, h) pairs with type 1 = drift
+ * (fields_kin), 2 = grav (grav_kinetic), 3 = kick (fields_grad_pot), mapped
+ * through Hamiltonian_Split's index order for time_var.  Returns the number of
+ * operators (writes at most max_ops), or < 0 for a bad order. */
+int  lat_schedule(int time_var, int order, double dt, int nsteps, int symp6_tree, int *types, double *h,
+                  int max_ops);
 /* Replaces calc_metric(): hamiltonian_conformal.f90:263-304 / hamiltonian_cosmic.f90:351-403. */
 int  lat_calc_metric(lat_ctx *ctx);
 /* Replaces dump_rho (openmp_dev/evolve_spectral.f90:305-415) / scalar_rho

@@ -35,36 +35,123 @@ template <typename T>
 __device__ __forceinline__ cpx<T> cconj(cpx<T> a) { return {a.x, -a.y}; }
 
 // ---------------------------------------------------------------- schedule
-constexpr int pick_radix(int rem)
+// Largest radix first (fewest LDS round trips), capped at MR: 16, 8, 4, 2,
+// then 3, 5.
+constexpr int pick_radix(int rem, int mr = 16)
 {
-    return (rem % 4 == 0) ? 4 : (rem % 2 == 0) ? 2 : (rem % 3 == 0) ? 3 : (rem % 5 == 0) ? 5 : rem;
+    return (mr >= 16 && rem % 16 == 0) ? 16
+           : (mr >= 8 && rem % 8 == 0) ? 8
+           : (rem % 4 == 0) ? 4
+           : (rem % 2 == 0) ? 2
+           : (rem % 3 == 0) ? 3
+           : (rem % 5 == 0) ? 5
+                            : rem;
 }
-constexpr int num_passes(int n)
+constexpr int num_passes(int n, int mr = 16)
 {
     int c = 0;
-    while (n > 1) { n /= pick_radix(n); ++c; }
+    while (n > 1) { n /= pick_radix(n, mr); ++c; }
     return c;
 }
-constexpr int radix_at(int n, int p)
+constexpr int radix_at(int n, int p, int mr = 16)
 {
     int r = 1;
-    for (int i = 0; i <= p; ++i) { r = pick_radix(n); n /= r; }
+    for (int i = 0; i <= p; ++i) { r = pick_radix(n, mr); n /= r; }
     return r;
 }
-constexpr int ns_at(int n, int p)
+constexpr int ns_at(int n, int p, int mr = 16)
 {
     int ns = 1;
-    for (int i = 0; i < p; ++i) { int r = pick_radix(n); ns *= r; n /= r; }
+    for (int i = 0; i < p; ++i) { int r = pick_radix(n, mr); ns *= r; n /= r; }
     return ns;
 }
 constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 // ---------------------------------------------------------------- radix kernels
+// multiply by e^{SIGN 2 pi i m / 16} for a compile-time m
+template <int M, int SIGN, typename T>
+__device__ __forceinline__ cpx<T> rot16(cpx<T> a)
+{
+    constexpr int m = ((M % 16) + 16) % 16;
+    if constexpr (m == 0) {
+        return a;
+    } else if constexpr (m == 4) { // SIGN*i
+        return SIGN < 0 ? cpx<T>{a.y, -a.x} : cpx<T>{-a.y, a.x};
+    } else if constexpr (m == 8) {
+        return {-a.x, -a.y};
+    } else if constexpr (m == 12) {
+        return SIGN < 0 ? cpx<T>{-a.y, a.x} : cpx<T>{a.y, -a.x};
+    } else {
+        // cos and sin of 2 pi m / 16 for m = 1..15 (octant-exact constants)
+        constexpr double C1 = 0.92387953251128675612818318939679, S1 = 0.38268343236508977172845998403040;
+        constexpr double C2 = 0.70710678118654752440084436210485;
+        constexpr double cs[16] = {1, C1, C2, S1, 0, -S1, -C2, -C1, -1, -C1, -C2, -S1, 0, S1, C2, C1};
+        constexpr double sn[16] = {0, S1, C2, C1, 1, C1, C2, S1, 0, -S1, -C2, -C1, -1, -C1, -C2, -S1};
+        const T c = T(cs[m]), s = T(SIGN < 0 ? -sn[m] : sn[m]);
+        return {a.x * c - a.y * s, a.x * s + a.y * c};
+    }
+}
+
+template <int R, int SIGN, typename T>
+__device__ __forceinline__ void dft_small(cpx<T> *v);
+
+// 16-point DFT as 4 x 4: n = 4 n1 + n2, k = k1 + 4 k2.
+template <int SIGN, typename T>
+__device__ __forceinline__ void dft16(cpx<T> *v)
+{
+    cpx<T> y[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+        cpx<T> t[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+        dft_small<4, SIGN>(t);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) y[n2][k1] = t[k1];
+    }
+    // internal twiddles W16^{n2 k1}
+    y[1][1] = rot16<1, SIGN>(y[1][1]);
+    y[1][2] = rot16<2, SIGN>(y[1][2]);
+    y[1][3] = rot16<3, SIGN>(y[1][3]);
+    y[2][1] = rot16<2, SIGN>(y[2][1]);
+    y[2][2] = rot16<4, SIGN>(y[2][2]);
+    y[2][3] = rot16<6, SIGN>(y[2][3]);
+    y[3][1] = rot16<3, SIGN>(y[3][1]);
+    y[3][2] = rot16<6, SIGN>(y[3][2]);
+    y[3][3] = rot16<9, SIGN>(y[3][3]);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        cpx<T> t[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+        dft_small<4, SIGN>(t);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+    }
+}
+
+// 8-point DFT as 2 x 4: n = 2 n1 + n2, k = k1 + 4 k2.
+template <int SIGN, typename T>
+__device__ __forceinline__ void dft8(cpx<T> *v)
+{
+    cpx<T> a[4] = {v[0], v[2], v[4], v[6]}, b[4] = {v[1], v[3], v[5], v[7]};
+    dft_small<4, SIGN>(a);
+    dft_small<4, SIGN>(b);
+    b[1] = rot16<2, SIGN>(b[1]);
+    b[2] = rot16<4, SIGN>(b[2]);
+    b[3] = rot16<6, SIGN>(b[3]);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        v[k1] = cadd(a[k1], b[k1]);
+        v[k1 + 4] = csub(a[k1], b[k1]);
+    }
+}
+
 // SIGN = -1: forward e^{-2 pi i}; +1: backward.
 template <int R, int SIGN, typename T>
 __device__ __forceinline__ void dft_small(cpx<T> *v)
 {
-    if constexpr (R == 2) {
+    if constexpr (R == 16) {
+        dft16<SIGN>(v);
+    } else if constexpr (R == 8) {
+        dft8<SIGN>(v);
+    } else if constexpr (R == 2) {
         cpx<T> a = v[0], b = v[1];
         v[0] = cadd(a, b);
         v[1] = csub(a, b);
@@ -164,6 +251,57 @@ __device__ __forceinline__ void lds_fft(cpx<T> *buf, int lstr, int estr, const c
     if constexpr (P < num_passes(N)) {
         stockham_pass<T, N, radix_at(N, P), ns_at(N, P), SIGN, L, TPB, COLMAJOR, TWS>(buf, lstr, estr, tw);
         lds_fft<T, N, SIGN, L, TPB, COLMAJOR, TWS, P + 1>(buf, lstr, estr, tw);
+    }
+}
+
+// Passes [P, PEND) in LDS (schedule with max radix MR).
+template <typename T, int N, int SIGN, int L, int TPB, bool COLMAJOR, int TWS, int P, int PEND, int MR = 16>
+__device__ __forceinline__ void lds_fft_range(cpx<T> *buf, int lstr, int estr, const cpx<T> *__restrict__ tw)
+{
+    if constexpr (P < PEND) {
+        stockham_pass<T, N, radix_at(N, P, MR), ns_at(N, P, MR), SIGN, L, TPB, COLMAJOR, TWS>(buf, lstr, estr, tw);
+        lds_fft_range<T, N, SIGN, L, TPB, COLMAJOR, TWS, P + 1, PEND, MR>(buf, lstr, estr, tw);
+    }
+}
+
+// Pass P read from LDS, result handed to st(line, position, value) instead of
+// being written back (used for the last pass: results go straight to HBM).
+// No trailing barrier: the caller synchronises before LDS is reused.
+template <typename T, int N, int P, int SIGN, int L, int TPB, bool COLMAJOR, int TWS, int MR, typename Store>
+__device__ __forceinline__ void stockham_pass_out(const cpx<T> *buf, int lstr, int estr,
+                                                  const cpx<T> *__restrict__ tw, Store st)
+{
+    constexpr int R = radix_at(N, P, MR), NS = ns_at(N, P, MR);
+    constexpr int NR = N / R;
+    constexpr int TOTAL = L * NR;
+    constexpr int BPT = cdiv(TOTAL, TPB);
+    constexpr int TSTEP = (N / (NS * R)) * TWS;
+    constexpr int TN = N * TWS;
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+        const int g = threadIdx.x + b * TPB;
+        if (TOTAL % TPB == 0 || g < TOTAL) {
+            int line, j;
+            if constexpr (COLMAJOR) { line = g % L; j = g / L; }
+            else { j = g % NR; line = g / NR; }
+            const int k = j % NS;
+            const cpx<T> *base = buf + line * lstr;
+            cpx<T> v[R];
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                cpx<T> a = base[(j + q * NR) * estr];
+                if (q > 0 && NS > 1) {
+                    cpx<T> w = tw[(k * q * TSTEP) % TN];
+                    if (SIGN > 0) w.y = -w.y;
+                    a = cmul(a, w);
+                }
+                v[q] = a;
+            }
+            dft_small<R, SIGN>(v);
+            const int o = (j / NS) * NS * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) st(line, o + q * NS, v[q]);
+        }
     }
 }
 

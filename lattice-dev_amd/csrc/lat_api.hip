@@ -912,6 +912,23 @@ int lat_symp_o2step(lat_ctx *h, double dt, double w1, double w2)
     return run_ops(c, ops);
 }
 
+int lat_schedule(int time_var, int order, double dt, int nsteps, int symp6_tree, int *types, double *hh,
+                 int max_ops)
+{
+    Ctx c{};
+    c.cfg.time_var = time_var;
+    c.cfg.symp6_tree = symp6_tree;
+    std::vector<Op> ops;
+    if (build_symp(c, ops, order, dt, nsteps)) return -1;
+    const bool conf = time_var == LAT_CONFORMAL;
+    for (size_t q = 0; q < ops.size() && (int)q < max_ops; ++q) {
+        if (types) types[q] = ops[q].type == OP_KICK ? 3 : (ops[q].type == OP_DRIFT ? 1 : 2);
+        if (hh) hh[q] = ops[q].h;
+    }
+    (void)conf;
+    return (int)ops.size();
+}
+
 int lat_symp(lat_ctx *h, int order, double dt, int nsteps)
 {
     CTX_OR_FAIL(h);

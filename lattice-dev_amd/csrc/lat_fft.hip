@@ -72,7 +72,7 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NCH], double *out
 // ---------------------------------------------------------------- x R2C
 // With DRIFT: phi += dc0*pi (+ dc1*pi) first, written back (mixed form: on
 // the fluctuations, then minus the re-centering shift nu[field]).
-template <typename T, int N, int L, bool DRIFT>
+template <typename T, int N, int L, bool DRIFT, int MR = 4>
 __global__ __launch_bounds__(TPB) void k_x_r2c(T *__restrict__ f, const T *__restrict__ pi,
                                                cpx<T> *__restrict__ Fk, const cpx<T> *__restrict__ tw,
                                                int rows, int P, const DevScalars *__restrict__ dsc, int field)
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(TPB) void k_x_r2c(T *__restrict__ f, const T *__res
             buf[line * LS + n] = v;
         }
         __syncthreads();
-        lds_fft<T, M, -1, L, TPB, false, 2>(buf, LS, 1, tw);
+        lds_fft_range<T, M, -1, L, TPB, false, 2, 0, num_passes(M, MR), MR>(buf, LS, 1, tw);
         // real-FFT split: X[k] = (Z[k] + Z*[M-k])/2 - i W^k (Z[k] - Z*[M-k])/2
         for (int idx = threadIdx.x; idx < L * (M / 2 + 1); idx += TPB) {
             const int line = idx / (M / 2 + 1), k = idx - line * (M / 2 + 1);
@@ -154,57 +154,98 @@ __global__ __launch_bounds__(TPB) void k_x_r2c(T *__restrict__ f, const T *__res
 
 // ---------------------------------------------------------------- y / z columns
 // MODE 0 forward, 1 inverse, 2 forward * (-k^2 kfac) inverse (z axis only).
-template <typename T, int N, int TC, int MODE>
-__global__ __launch_bounds__(TPB) void k_col(cpx<T> *__restrict__ Fk, const cpx<T> *__restrict__ tw, int nxh,
+template <typename T, int N, int TC, int MODE, int MR = 16, bool PF = true, int MINW = 1, int NT = TPB>
+__global__ __launch_bounds__(NT, MINW) void k_col(cpx<T> *__restrict__ Fk, const cpx<T> *__restrict__ tw, int nxh,
                                              int n_other, size_t s_axis, size_t s_other, size_t s_arr,
                                              int n_arr, int ny, double kfac)
 {
+    // Software pipeline per tile of TC adjacent columns:
+    //   pass 0 straight from HBM into registers (radix R0, no twiddles),
+    //   results to LDS, then the next tile's pass-0 loads are issued into the
+    //   same (now dead) registers so they are in flight during the remaining
+    //   LDS passes; the last pass writes straight to HBM.
     __shared__ cpx<T> buf[TC * N];
+    constexpr int NP = num_passes(N, MR);
+    constexpr int SIGN0 = (MODE == 1) ? +1 : -1;
+    constexpr int R0 = radix_at(N, 0, MR);
+    constexpr int NR0 = N / R0;
+    constexpr int TOT0 = TC * NR0;
+    constexpr int B0 = cdiv(TOT0, NT);
     const int nct = (nxh + TC - 1) / TC;
     int t0, t1, ts;
     tile_range(nct * n_other * n_arr, t0, t1, ts);
-    for (int t = t0; t < t1; t += ts) {
-        const int ct = t % nct;
+    cpx<T> v[B0][R0];
+    auto tile_base = [&](int t, int &ncol, int &ct, int &o) -> cpx<T> * {
+        ct = t % nct;
         const int rest = t / nct;
-        const int o = rest % n_other, arr = rest / n_other;
-        cpx<T> *base = Fk + (size_t)arr * s_arr + (size_t)o * s_other + (size_t)ct * TC;
-        const int ncol = min(TC, nxh - ct * TC);
-        for (int idx = threadIdx.x; idx < TC * N; idx += TPB) {
-            const int c = idx % TC, n = idx / TC;
-            cpx<T> v = {0, 0};
-            if (c < ncol) v = base[(size_t)n * s_axis + c];
-            buf[idx] = v;
+        o = rest % n_other;
+        const int arr = rest / n_other;
+        ncol = min(TC, nxh - ct * TC);
+        return Fk + (size_t)arr * s_arr + (size_t)o * s_other + (size_t)ct * TC;
+    };
+    auto load = [&](int t) {
+        int ncol, ct, o;
+        const cpx<T> *base = tile_base(t, ncol, ct, o);
+#pragma unroll
+        for (int b = 0; b < B0; ++b) {
+            const int g = threadIdx.x + b * NT;
+            const int line = g % TC, j = g / TC;
+#pragma unroll
+            for (int q = 0; q < R0; ++q) {
+                cpx<T> a = {0, 0};
+                if ((TOT0 % NT == 0 || g < TOT0) && line < ncol) a = base[(size_t)(j + q * NR0) * s_axis + line];
+                v[b][q] = a;
+            }
         }
+    };
+    if (PF && t0 < t1) load(t0);
+    for (int t = t0; t < t1; t += ts) {
+        int ncol, ct, o;
+        cpx<T> *base = tile_base(t, ncol, ct, o);
+        if (!PF) load(t);
+#pragma unroll
+        for (int b = 0; b < B0; ++b) {
+            const int g = threadIdx.x + b * NT;
+            if (TOT0 % NT == 0 || g < TOT0) {
+                const int line = g % TC, j = g / TC;
+                dft_small<R0, SIGN0>(v[b]);
+#pragma unroll
+                for (int q = 0; q < R0; ++q) buf[(j * R0 + q) * TC + line] = v[b][q];
+            }
+        }
+        if (PF && t + ts < t1) load(t + ts);
         __syncthreads();
-        if constexpr (MODE == 1) {
-            lds_fft<T, N, +1, TC, TPB, true, 1>(buf, 1, TC, tw);
-        } else {
-            lds_fft<T, N, -1, TC, TPB, true, 1>(buf, 1, TC, tw);
-        }
+        auto store = [&](int line, int pos, cpx<T> a) {
+            if (line < ncol) base[(size_t)pos * s_axis + line] = a;
+        };
         if constexpr (MODE == 2) {
+            lds_fft_range<T, N, -1, TC, NT, true, 1, 1, NP, MR>(buf, 1, TC, tw);
             const int jj = wmag(o, ny);
-            for (int idx = threadIdx.x; idx < TC * N; idx += TPB) {
+            for (int idx = threadIdx.x; idx < TC * N; idx += NT) {
                 const int c = idx % TC, n = idx / TC;
                 const int ii = ct * TC + c;
                 const int kk = wmag(n, N);
                 const double rad2 = double(ii * ii) + double(jj * jj) + double(kk * kk);
                 const T m = T(-rad2 * kfac);
-                cpx<T> v = buf[idx];
-                buf[idx] = {v.x * m, v.y * m};
+                cpx<T> a = buf[idx];
+                buf[idx] = {a.x * m, a.y * m};
             }
             __syncthreads();
-            lds_fft<T, N, +1, TC, TPB, true, 1>(buf, 1, TC, tw);
-        }
-        for (int idx = threadIdx.x; idx < TC * N; idx += TPB) {
-            const int c = idx % TC, n = idx / TC;
-            if (c < ncol) base[(size_t)n * s_axis + c] = buf[idx];
+            lds_fft_range<T, N, +1, TC, NT, true, 1, 0, NP - 1, MR>(buf, 1, TC, tw);
+            stockham_pass_out<T, N, NP - 1, +1, TC, NT, true, 1, MR>(buf, 1, TC, tw, store);
+        } else if constexpr (NP == 1) {
+            // single-pass transform: pass 0 was the whole FFT
+            for (int idx = threadIdx.x; idx < TC * N; idx += NT) store(idx % TC, idx / TC, buf[idx]);
+        } else {
+            lds_fft_range<T, N, SIGN0, TC, NT, true, 1, 1, NP - 1, MR>(buf, 1, TC, tw);
+            stockham_pass_out<T, N, NP - 1, SIGN0, TC, NT, true, 1, MR>(buf, 1, TC, tw, store);
         }
         __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------- x C2R + epilogue
-template <typename T, int N, int L, int EPI>
+template <typename T, int N, int L, int EPI, int MR = 4>
 __global__ __launch_bounds__(TPB) void k_x_c2r(const cpx<T> *__restrict__ Fk, const cpx<T> *__restrict__ tw,
                                                int rows, int P, size_t nk, size_t nvol, int nf, T *__restrict__ out,
                                                const T *__restrict__ fld, T *__restrict__ fldp,
@@ -271,8 +312,8 @@ __global__ __launch_bounds__(TPB) void k_x_c2r(const cpx<T> *__restrict__ Fk, co
             }
         }
         __syncthreads();
-        if (nf == 2) lds_fft<T, M, +1, 2 * L, TPB, false, 2>(buf, LS, 1, tw);
-        else lds_fft<T, M, +1, L, TPB, false, 2>(buf, LS, 1, tw);
+        if (nf == 2) lds_fft_range<T, M, +1, 2 * L, TPB, false, 2, 0, num_passes(M, MR), MR>(buf, LS, 1, tw);
+        else lds_fft_range<T, M, +1, L, TPB, false, 2, 0, num_passes(M, MR), MR>(buf, LS, 1, tw);
         for (int idx = threadIdx.x; idx < L * N; idx += TPB) {
             const int line = idx / N, i = idx - line * N;
             const int row = r0 + line;
@@ -315,22 +356,28 @@ __global__ __launch_bounds__(TPB) void k_x_c2r(const cpx<T> *__restrict__ Fk, co
 }
 
 // ---------------------------------------------------------------- launch helpers
+// Launch shapes, chosen from sweeps of tile width, maximum radix, prefetch
+// and block size at 1024^3 on MI355X (see the measurement notes in the
+// repository documentation) and scaled by the line length N for other sizes.
+constexpr int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// rows per workgroup: x R2C 2048/(N/2) complex elements, x C2R 1024/(N/2)
+template <int N>
+constexpr int r2c_lines() { return clampi(2048 / (N / 2), 1, 512); }
+template <int N>
+constexpr int c2r_lines() { return clampi(1024 / (N / 2), 1, 256); }
 template <typename T>
-constexpr int elems_budget() { return sizeof(T) == 8 ? 1024 : 2048; }
+constexpr int r2c_maxradix() { return sizeof(T) == 8 ? 4 : 8; }
 
-template <typename T, int N>
-constexpr int row_lines() // rows per workgroup for the x kernels
-{
-    return (elems_budget<T>() / (N / 2)) < 1 ? 1 : (elems_budget<T>() / (N / 2));
-}
-
-template <typename T, int N>
-constexpr int col_tile() // adjacent kx columns per workgroup for the y/z kernels
-{
-    constexpr int lim = (sizeof(T) == 8 ? 4096 : 8192) / N;
-    constexpr int mn = sizeof(T) == 8 ? 4 : 8;
-    return lim < mn ? mn : (lim > 64 ? 64 : lim);
-}
+// y/z column passes: TC adjacent kx columns per tile, 512 threads
+constexpr int COL_NT = 512;
+template <typename T, int N, int MODE>
+struct ColCfg {
+    // y passes: 8 columns at N = 1024 (128 B rows in FP64); fused z pass: 4
+    static constexpr int TC = MODE == 2 ? clampi(4096 / N, 4, 64) : clampi(8192 / N, 8, 64);
+    static constexpr int MR = MODE == 2 ? 8 : 16;
+    static constexpr bool PF = MODE == 2 && sizeof(T) == 8;
+};
 
 #define LAT_FOR_SIZES(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(48) X(96) X(192)
 
@@ -341,14 +388,15 @@ int launch_x_r2c(Ctx &c, const cpx<T> *tw, T *f, const T *pi, cpx<T> *Fk, int dr
     switch (c.nx) {
 #define CASE(NN)                                                                                         \
     case NN: {                                                                                           \
-        constexpr int L = row_lines<T, NN>();                                                            \
+        constexpr int L = r2c_lines<NN>();                                                               \
+        constexpr int MR = r2c_maxradix<T>();                                                            \
         const int g = grid_for((rows + L - 1) / L);                                                      \
         if (drift)                                                                                       \
-            hipLaunchKernelGGL((k_x_r2c<T, NN, L, true>), dim3(g), dim3(TPB), 0, c.stream, f, pi, Fk, tw, \
-                               rows, c.P, c.dsc, field);                                                 \
+            hipLaunchKernelGGL((k_x_r2c<T, NN, L, true, MR>), dim3(g), dim3(TPB), 0, c.stream, f, pi, Fk, \
+                               tw, rows, c.P, c.dsc, field);                                             \
         else                                                                                             \
-            hipLaunchKernelGGL((k_x_r2c<T, NN, L, false>), dim3(g), dim3(TPB), 0, c.stream, f, pi, Fk, tw, \
-                               rows, c.P, c.dsc, field);                                                 \
+            hipLaunchKernelGGL((k_x_r2c<T, NN, L, false, MR>), dim3(g), dim3(TPB), 0, c.stream, f, pi, Fk, \
+                               tw, rows, c.P, c.dsc, field);                                             \
         break;                                                                                           \
     }
         LAT_FOR_SIZES(CASE)
@@ -372,17 +420,19 @@ int launch_col(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int axis, int mode, int n_a
     switch (n) {
 #define CASE(NN)                                                                                          \
     case NN: {                                                                                            \
-        constexpr int TC = col_tile<T, NN>();                                                             \
-        const int g = grid_for(((c.nxh + TC - 1) / TC) * n_other * n_arrays);                              \
+        using CY = ColCfg<T, NN, 0>;                                                                      \
+        using CZ = ColCfg<T, NN, 2>;                                                                      \
+        const int tc = mode == 2 ? CZ::TC : CY::TC;                                                       \
+        const int g = grid_for(((c.nxh + tc - 1) / tc) * n_other * n_arrays);                              \
         if (mode == 0)                                                                                    \
-            hipLaunchKernelGGL((k_col<T, NN, TC, 0>), dim3(g), dim3(TPB), 0, c.stream, Fk, tw, c.nxh, n_other, \
-                               s_axis, s_other, c.nk, n_arrays, c.ny, kfac);                              \
+            hipLaunchKernelGGL((k_col<T, NN, CY::TC, 0, CY::MR, CY::PF, 1, COL_NT>), dim3(g), dim3(COL_NT), 0, \
+                               c.stream, Fk, tw, c.nxh, n_other, s_axis, s_other, c.nk, n_arrays, c.ny, kfac);   \
         else if (mode == 1)                                                                               \
-            hipLaunchKernelGGL((k_col<T, NN, TC, 1>), dim3(g), dim3(TPB), 0, c.stream, Fk, tw, c.nxh, n_other, \
-                               s_axis, s_other, c.nk, n_arrays, c.ny, kfac);                              \
+            hipLaunchKernelGGL((k_col<T, NN, CY::TC, 1, CY::MR, CY::PF, 1, COL_NT>), dim3(g), dim3(COL_NT), 0, \
+                               c.stream, Fk, tw, c.nxh, n_other, s_axis, s_other, c.nk, n_arrays, c.ny, kfac);   \
         else                                                                                              \
-            hipLaunchKernelGGL((k_col<T, NN, TC, 2>), dim3(g), dim3(TPB), 0, c.stream, Fk, tw, c.nxh, n_other, \
-                               s_axis, s_other, c.nk, n_arrays, c.ny, kfac);                              \
+            hipLaunchKernelGGL((k_col<T, NN, CZ::TC, 2, CZ::MR, CZ::PF, 1, COL_NT>), dim3(g), dim3(COL_NT), 0, \
+                               c.stream, Fk, tw, c.nxh, n_other, s_axis, s_other, c.nk, n_arrays, c.ny, kfac);   \
         break;                                                                                            \
     }
         LAT_FOR_SIZES(CASE)
@@ -407,7 +457,7 @@ int launch_x_c2r(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int nf, int epi, T *out, 
     switch (c.nx) {
 #define CASE(NN)                                                                                         \
     case NN: {                                                                                           \
-        constexpr int L = row_lines<T, NN>();                                                            \
+        constexpr int L = c2r_lines<NN>();                                                               \
         const int g = grid_for((rows + L - 1) / L);                                                      \
         if (epi == EPI_STORE)                                                                            \
             hipLaunchKernelGGL((k_x_c2r<T, NN, L, EPI_STORE>), dim3(g), dim3(TPB), 0, c.stream, Fk, tw, rows, \

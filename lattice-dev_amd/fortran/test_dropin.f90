@@ -1,0 +1,70 @@
+!
+! Exercises lattice_dropin the way a reference driver calls its modules
+! (Pseudospec/openmp_dev/evolve_spectral.f90: laplacian_3d, init_spectrum_3d,
+! spectrum_3d, calc_metric, symp4).  Prints DROPIN OK or stops with code 1.
+!
+program test_dropin
+  use, intrinsic :: iso_c_binding
+  use lattice_dropin
+  implicit none
+  integer, parameter :: dl = kind(1.d0)
+  integer, parameter :: n = 16, nfld = 2
+  real(dl), parameter :: len = 10._dl, twopi = 6.2831853071795864769252867665590_dl
+  real(dl) :: dk, dx, err, x, y, z, kk2, ysc, yscp, e(LAT_NE)
+  real(c_double), allocatable, target :: f(:,:,:), g(:,:,:)
+  complex(c_double_complex), allocatable, target :: Fk(:,:,:)
+  real(dl), allocatable :: S(:)
+  real(dl), allocatable :: fld(:,:,:,:), fldp(:,:,:,:)
+  type(c_ptr) :: planf, planb
+  integer :: i, j, k, b
+
+  dk = twopi / len
+  dx = len / n
+  call lattice_init(n, n, n, nfld, len, LAT_F64, LAT_LAP_SPECTRAL, LAT_COSMIC, LAT_MODEL_TWOFIELD, &
+                    (/ 1._dl, 2.5e5_dl /))
+
+  allocate(f(n,n,n), g(n,n,n), Fk(n/2+1,n,n))
+  kk2 = dk**2 * (3**2 + 2**2 + 1**2)
+  do k = 1, n; do j = 1, n; do i = 1, n
+     x = (i-1)*dx; y = (j-1)*dx; z = (k-1)*dx
+     f(i,j,k) = cos(dk*(3*x + 2*y - z))
+  end do; end do; end do
+  g = f
+  planf = c_null_ptr; planb = c_null_ptr
+  call laplacian_3d(n, n, n, g, Fk, dk, planf, planb)
+  err = maxval(abs(g + kk2*f))
+  print *, "laplacian_3d plane wave max error", err
+  if (err > 1e-10_dl) stop 1
+
+  call init_spectrum_3d(n, n, n)
+  allocate(S(ns))
+  g = f
+  call spectrum_3d(S, g, Fk)
+  ! |k| = sqrt(14) = 3.74: weight only in bins 3 and 4 (1-based 4, 5)
+  do b = 1, ns
+     if ((b == 4 .or. b == 5) .neqv. (abs(S(b)) > 1e-8_dl * maxval(abs(S)))) then
+        print *, "spectrum bin", b, S(b)
+        stop 1
+     end if
+  end do
+  print *, "spectrum_3d bins", ns, " peak", maxval(S)
+
+  allocate(fld(nfld,0:n+1,0:n+1,0:n+1), fldp(nfld,n,n,n))
+  fld = 0._dl; fldp = 0._dl
+  do k = 1, n; do j = 1, n; do i = 1, n
+     fld(1,i,j,k) = 1.0093430384226378929425913902459_dl + 1e-6_dl*f(i,j,k)
+     fld(2,i,j,k) = 1e-6_dl*f(j,k,i)
+     fldp(1,i,j,k) = -0.7137133070120812430962278466136_dl
+  end do; end do; end do
+  call lattice_put_state(fld, fldp, 1, 1._dl, 0._dl)
+  call calc_metric()
+  call symp4(1e-3_dl, 5)
+  call lattice_get_state(fld, fldp, 1, ysc, yscp)
+  call lattice_energies(e)
+  print *, "after symp4: a", get_scale_factor(), " H", get_hubble(), " resid", e(8)
+  if (abs(e(8)) > 1e-9_dl) stop 1
+  if (abs(fld(1,0,5,5) - fld(1,n,5,5)) > 0._dl) stop 1   ! periodic ghost
+  if (.not. (ysc > 1._dl)) stop 1
+  call lattice_finalize()
+  print *, "DROPIN OK"
+end program test_dropin
