@@ -162,7 +162,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 2 GiB device copy (3 times) to calibrate HBM byte counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        a = torch.ones(2 ** 28, dtype=torch.float64, device="cuda")  # 2 GiB
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design note, "
               "not a measured configuration (use --gpus 1)", file=sys.stderr)

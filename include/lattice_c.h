@@ -111,6 +111,15 @@ int  lat_crossspec_3d(lat_ctx *ctx, const double *Fk1, const double *Fk2, double
 /* Device-resident output spectra of make_output (evolve_spectral.f90:523-551):
  * out[0..ns) S_phi1, [ns..2ns) S_pi1, [2ns..3ns) Re S(pi1,phi1), [3ns..4ns) Im. */
 int  lat_output_spectra(lat_ctx *ctx, int field, double *out);
+/* Discrete-tree spectra on a cubic lattice (nx = ny = nz = n), with
+ * ns_d = lat_num_bins_discrete(n) = int(sqrt(3) * (n/2) + 2) bins
+ * (Discrete/Entropy/params.f90:45):
+ *   kind 0 replaces spectrum(f, S), Discrete/Entropy/analysis.f90:89-114:
+ *          two-bin weights, S / W / n^6;
+ *   kind 1 replaces spectrum_bin(f, S), analysis.f90:22-52:
+ *          integer count per floor(|k|/dk) bin, S / W. */
+int  lat_num_bins_discrete(int n);
+int  lat_spectrum_discrete(lat_ctx *ctx, const double *f, double *S, int kind);
 
 /* ---- state (module Hamiltonian globals fld, fldp, yscl, ysclp,
  *      hamiltonian_conformal.f90:22-24; hamiltonian_cosmic.f90:27,31) ---- */

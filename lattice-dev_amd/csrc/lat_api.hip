@@ -494,7 +494,7 @@ int lat_create(const lat_config *cfg, lat_ctx **out)
     lat_ctx *h = new lat_ctx();
     Ctx &c = h->c;
     c.cfg = *cfg;
-    hipGetDevice(&c.device);
+    (void)hipGetDevice(&c.device);
     c.nx = cfg->nx;
     c.ny = cfg->ny;
     c.nz = cfg->nz;
@@ -507,6 +507,12 @@ int lat_create(const lat_config *cfg, lat_ctx **out)
     c.prec = cfg->precision;
     c.nfld = cfg->nfld;
     c.ns = lat_num_bins(c.nx, c.ny, c.nz);
+    {
+        int nmax = c.nx > c.ny ? c.nx : c.ny;
+        nmax = nmax > c.nz ? nmax : c.nz;
+        const int nd = lat_num_bins_discrete(nmax);
+        c.spec_cap = c.ns > nd ? c.ns : nd;
+    }
     int rc = 0;
     hipError_t e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete h; return -3; }
@@ -529,9 +535,9 @@ int lat_create(const lat_config *cfg, lat_ctx **out)
     c.host_sc.yscl = 1.0;
     c.host_sc.mixed = c.prec == LAT_MIXED ? 1 : 0;
     if (push_scalars(c)) { lat_destroy(h); return -3; }
-    hipMemset(c.fld, 0, es * c.nfld * c.nvol);
-    hipMemset(c.fldp, 0, es * c.nfld * c.nvol);
-    hipMemset(c.partials, 0, sizeof(double) * MAX_GRID * NRED);
+    (void)hipMemset(c.fld, 0, es * c.nfld * c.nvol);
+    (void)hipMemset(c.fldp, 0, es * c.nfld * c.nvol);
+    (void)hipMemset(c.partials, 0, sizeof(double) * MAX_GRID * NRED);
     c.ke_valid = false;
     *out = h;
     return 0;
@@ -541,8 +547,8 @@ void lat_destroy(lat_ctx *h)
 {
     if (!h) return;
     Ctx &c = h->c;
-    hipSetDevice(c.device);
-    if (c.stream) hipStreamSynchronize(c.stream);
+    (void)hipSetDevice(c.device);
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
     void *ptrs[] = {c.fld, c.fldp, c.Fk, c.Fk2, c.io64, (c.Fk64 != c.Fk) ? c.Fk64 : nullptr, c.dsc, c.partials,
                     c.spec_part, c.spec_out, c.twx64, c.twy64, c.twz64, c.twx32, c.twy32, c.twz32};
     for (void *p : ptrs)
@@ -662,19 +668,20 @@ int lat_stencil_laplacian(lat_ctx *h, const double *f, double *lap)
 
 static int ensure_spec(Ctx &c)
 {
-    int rc = ensure(c, (void **)&c.spec_part, sizeof(double) * 1024 * 5 * c.ns, "hipMalloc spec_part");
-    rc |= ensure(c, (void **)&c.spec_out, sizeof(double) * 5 * c.ns, "hipMalloc spec_out");
+    const size_t cap = (size_t)c.spec_cap;
+    int rc = ensure(c, (void **)&c.spec_part, sizeof(double) * 1024 * 5 * cap, "hipMalloc spec_part");
+    rc |= ensure(c, (void **)&c.spec_out, sizeof(double) * 5 * cap, "hipMalloc spec_out");
     return rc;
 }
 
-static void finish_spectra(const Ctx &c, const std::vector<double> &raw, int nch, double *const *outs)
+// S(b) / W(b) / div where W(b) != 0 (analysis_spec.f90:101)
+static void finish_spectra(const std::vector<double> &raw, int ns, int nch, double *const *outs, double div = 1.0)
 {
-    const int ns = c.ns;
     const double *W = raw.data() + (size_t)nch * ns;
     for (int ch = 0; ch < nch; ++ch)
         for (int b = 0; b < ns; ++b) {
             double s = raw[(size_t)ch * ns + b];
-            if (W[b] != 0.0) s = s / W[b]; // analysis_spec.f90:101
+            if (W[b] != 0.0) s = s / W[b] / div;
             outs[ch][b] = s;
         }
 }
@@ -693,7 +700,37 @@ int lat_spectrum_3d(lat_ctx *h, double *S, const double *f, double *Fk)
     if (Fk && (rc = fk_to_host(c, F, Fk))) return rc;
     if ((rc = sync(c))) return rc;
     double *outs[1] = {S};
-    finish_spectra(c, raw, 1, outs);
+    finish_spectra(raw, c.ns, 1, outs);
+    return 0;
+}
+
+int lat_num_bins_discrete(int n)
+{
+    // Discrete/Entropy/params.f90:45, integer conversion truncates
+    return (int)(1.7320508075688772935274463415059 * (double)(n / 2) + 2.0);
+}
+
+int lat_spectrum_discrete(lat_ctx *h, const double *f, double *S, int kind)
+{
+    CTX_OR_FAIL(h);
+    if (!S || !f || (kind != 0 && kind != 1)) return -1;
+    if (c.nx != c.ny || c.nx != c.nz) {
+        c.last_error = "Discrete-tree spectra need a cubic lattice";
+        return -2;
+    }
+    int rc = ensure_spec(c);
+    if (rc) return rc;
+    if ((rc = host_r2c(c, f))) return rc;
+    const int ns = lat_num_bins_discrete(c.nx);
+    const cpx<double> *F = (const cpx<double> *)c.Fk64;
+    if ((rc = launch_bin<double>(c, F, F, 0, ns, kind)) < 0) return rc;
+    std::vector<double> raw(2 * (size_t)ns);
+    LAT_HIP(&c, hipMemcpyAsync(raw.data(), c.spec_out, sizeof(double) * 2 * ns, hipMemcpyDeviceToHost, c.stream));
+    if ((rc = sync(c))) return rc;
+    double *outs[1] = {S};
+    // kind 0: S/W/n^6 (analysis.f90:113); kind 1: S/W (analysis.f90:50)
+    const double n3 = (double)c.nvol;
+    finish_spectra(raw, ns, 1, outs, kind == 0 ? n3 * n3 : 1.0);
     return 0;
 }
 
@@ -713,7 +750,7 @@ int lat_crossspec_3d(lat_ctx *h, const double *Fk1, const double *Fk2, double *S
     LAT_HIP(&c, hipMemcpyAsync(raw.data(), c.spec_out, sizeof(double) * 3 * c.ns, hipMemcpyDeviceToHost, c.stream));
     if ((rc = sync(c))) return rc;
     double *outs[2] = {S_re, S_im};
-    finish_spectra(c, raw, 2, outs);
+    finish_spectra(raw, c.ns, 2, outs);
     return 0;
 }
 
@@ -758,7 +795,7 @@ static int output_spectra_t(Ctx &c, int field, double *out)
         raw[3 * (size_t)c.ns] = 0.0;
     }
     double *outs[4] = {out, out + c.ns, out + 2 * c.ns, out + 3 * c.ns};
-    finish_spectra(c, raw, 4, outs);
+    finish_spectra(raw, c.ns, 4, outs);
     return 0;
 }
 

@@ -100,6 +100,7 @@ struct Ctx {
     double *spec_part;      // spectrum block partials
     double *spec_out;       // 5 * ns reduced spectrum channels
     int ns;
+    int spec_cap;           // bins the histogram buffers hold
     int spec_blocks;
 
     // twiddle tables w[m] = exp(-2 pi i m / n) for n = nx, ny, nz
@@ -142,8 +143,9 @@ int launch_scalar(Ctx &c, const ScalarOps &ops, int npart);
 int grid_for(int tiles);
 
 // ---- spectra (lat_spec.hip) ----
+// mode 0: |F1|^2; 1: F1 conj(F2); 2: output channels.  nbins <= 0: c.ns.
 template <typename T>
-int launch_bin(Ctx &c, const cpx<T> *F1, const cpx<T> *F2, int nch);
+int launch_bin(Ctx &c, const cpx<T> *F1, const cpx<T> *F2, int mode, int nbins = 0, int counts = 0);
 
 // ---- initial conditions (lat_ic.hip) ----
 int sample_to_io(Ctx &c, double gamma, double m2eff, double mpl, double kcut_fac, uint64_t seed, uint64_t stream);

@@ -19,9 +19,12 @@ namespace lat {
 
 // channel sets: mode 0 -> |F1|^2 ; mode 1 -> Re, Im of F1 conj(F2) ;
 // mode 2 -> |F2|^2, |F1|^2, Re, Im of F1 conj(F2).  W is always the last channel.
+// counts = 1: integer-count bins floor(p) (spectrum_bin,
+// Discrete/Entropy/analysis.f90:22-52) instead of the two-bin weights.
 template <typename T, int NCH, int MODE>
 __global__ __launch_bounds__(64) void k_bin(const cpx<T> *__restrict__ F1, const cpx<T> *__restrict__ F2, int nx,
-                                            int ny, int nz, int nxh, int P, int ns, double *__restrict__ part)
+                                            int ny, int nz, int nxh, int P, int ns, double *__restrict__ part,
+                                            int counts)
 {
     extern __shared__ double hist[]; // (NCH+1) * ns
     const int lane = threadIdx.x;
@@ -41,8 +44,8 @@ __global__ __launch_bounds__(64) void k_bin(const cpx<T> *__restrict__ F1, const
                 const double p = sqrt(double(i * i + jj * jj + kk * kk));
                 const int l = (int)floor(p);
                 const double d1 = double(l) - p, d2 = double(l + 1) - p;
-                const double c1 = (1.0 - d1 * d1) * (1.0 - d1 * d1);
-                const double c2 = (1.0 - d2 * d2) * (1.0 - d2 * d2);
+                const double c1 = counts ? 1.0 : (1.0 - d1 * d1) * (1.0 - d1 * d1);
+                const double c2 = counts ? 0.0 : (1.0 - d2 * d2) * (1.0 - d2 * d2);
                 const double mult = (i == 0 || (2 * i == nx)) ? 1.0 : 2.0;
                 const size_t id = (size_t)row * P + i;
                 double val[NCH];
@@ -124,22 +127,26 @@ __global__ __launch_bounds__(256) void k_bin_reduce(const double *__restrict__ p
 constexpr int BIN_BLOCKS = 1024;
 
 template <typename T>
-int launch_bin(Ctx &c, const cpx<T> *F1, const cpx<T> *F2, int mode)
+int launch_bin(Ctx &c, const cpx<T> *F1, const cpx<T> *F2, int mode, int nbins, int counts)
 {
-    const int ns = c.ns;
+    const int ns = nbins > 0 ? nbins : c.ns;
+    if (ns > c.spec_cap) {
+        c.last_error = "spectrum bin count exceeds the allocated histogram";
+        return -1;
+    }
     const int nch = mode == 0 ? 1 : (mode == 1 ? 2 : 4);
     const int rows = c.ny * c.nz;
     const int g = rows < BIN_BLOCKS ? rows : BIN_BLOCKS;
     const size_t lds = sizeof(double) * (nch + 1) * ns;
     if (mode == 0)
         hipLaunchKernelGGL((k_bin<T, 1, 0>), dim3(g), dim3(64), lds, c.stream, F1, F2, c.nx, c.ny, c.nz, c.nxh, c.P,
-                           ns, c.spec_part);
+                           ns, c.spec_part, counts);
     else if (mode == 1)
         hipLaunchKernelGGL((k_bin<T, 2, 1>), dim3(g), dim3(64), lds, c.stream, F1, F2, c.nx, c.ny, c.nz, c.nxh, c.P,
-                           ns, c.spec_part);
+                           ns, c.spec_part, counts);
     else
         hipLaunchKernelGGL((k_bin<T, 4, 2>), dim3(g), dim3(64), lds, c.stream, F1, F2, c.nx, c.ny, c.nz, c.nxh, c.P,
-                           ns, c.spec_part);
+                           ns, c.spec_part, counts);
     LAT_HIP(&c, hipGetLastError());
     const int nq = (nch + 1) * ns;
     hipLaunchKernelGGL(k_bin_reduce, dim3((nq + 255) / 256), dim3(256), 0, c.stream, c.spec_part, g, nq, c.spec_out);
@@ -147,7 +154,7 @@ int launch_bin(Ctx &c, const cpx<T> *F1, const cpx<T> *F2, int mode)
     return nch;
 }
 
-template int launch_bin<double>(Ctx &, const cpx<double> *, const cpx<double> *, int);
-template int launch_bin<float>(Ctx &, const cpx<float> *, const cpx<float> *, int);
+template int launch_bin<double>(Ctx &, const cpx<double> *, const cpx<double> *, int, int, int);
+template int launch_bin<float>(Ctx &, const cpx<float> *, const cpx<float> *, int, int, int);
 
 } // namespace lat

@@ -58,6 +58,7 @@ EXPORTS = [
     "lat_output_spectra", "lat_set_state", "lat_get_state", "lat_get_scalars",
     "lat_set_scalars", "lat_split", "lat_symp_o2step", "lat_symp", "lat_calc_metric",
     "lat_energies", "lat_sample_field", "lat_device_state", "lat_profile", "lat_profile_read",
+    "lat_num_bins_discrete", "lat_spectrum_discrete",
 ]
 STAGES = ["x_r2c", "y_fwd", "z_lap", "y_inv", "x_c2r", "scalar", "drift", "stencil", "reduce"]
 
@@ -199,6 +200,15 @@ class Lattice:
         self._check(lib().lat_spectrum_3d(self._h, _p(S), _p(f), _p(F.view(np.float64)) if return_fk else None),
                     "spectrum_3d")
         return (S, F) if return_fk else S
+
+    def spectrum_discrete(self, f, kind=0):
+        """Discrete-tree spectra (cubic lattice): kind 0 = spectrum(f, S)
+        (Discrete/Entropy/analysis.f90:89-114, two-bin weights, /n^6);
+        kind 1 = spectrum_bin(f, S) (analysis.f90:22-52, integer counts)."""
+        f = self._field(f)
+        S = np.empty(lib().lat_num_bins_discrete(self.nx))
+        self._check(lib().lat_spectrum_discrete(self._h, _p(f), _p(S), int(kind)), "spectrum_discrete")
+        return S
 
     def crossspec_3d(self, Fk1, Fk2):
         """crossspec_3d (analysis_spec.f90:145-178) -> (S_real, S_imag)."""

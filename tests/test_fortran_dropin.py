@@ -1,0 +1,30 @@
+"""The reference-language side of the boundary: the iso_c_binding module
+lattice_dropin.f90 compiles and links against the C ABI (CPU), and the
+reference-style driver test_dropin.f90 runs on the GPU."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "lattice-dev_amd")
+BIN = os.path.join(PKG, "fortran", "bin", "test_dropin")
+
+
+@pytest.mark.skipif(not os.path.exists("/opt/rocm/bin/amdflang"), reason="amdflang not installed")
+def test_dropin_compiles_and_links():
+    subprocess.run(["make", "-s", "-C", PKG, "fortran"], check=True)
+    assert os.path.exists(BIN)
+    out = subprocess.run(["nm", "-D", "--undefined-only", BIN], capture_output=True, text=True).stdout
+    for sym in ("lat_create", "lat_laplacian_3d", "lat_spectrum_3d", "lat_symp", "lat_calc_metric"):
+        assert sym in out
+
+
+@pytest.mark.gpu
+def test_dropin_driver_runs_on_gpu(gpu):
+    if not os.path.exists(BIN):
+        pytest.fail("Fortran driver not built: run __graft_entry__.build()")
+    r = subprocess.run([BIN], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0 and "DROPIN OK" in r.stdout

@@ -96,7 +96,30 @@ def test_spectra_match_oracle(gpu, shape):
     assert np.array_equal(S1, S1b)
 
 
-@pytest.mark.parametrize("coef", [(-6.0, 1.0, 0.0, 0.0, 1.0), (-128.0, 14.0, 3.0, 1.0, 30.0)], ids=["7pt", "27pt"])
+@pytest.mark.parametrize("n", [8, 32, 48, 64])
+@pytest.mark.parametrize("kind", [0, 1], ids=["weighted", "counts"])
+def test_discrete_spectra_match_oracle(gpu, n, kind):
+    """Discrete-tree spectrum (two-bin weights, /n^6) and spectrum_bin
+    (integer-count bins) with ns = int(sqrt(3) n/2 + 2) bins."""
+    print("case discrete spectra", n, kind, flush=True)
+    rng = np.random.default_rng(11 + n + kind)
+    f = rng.standard_normal((n, n, n))
+    with _lat(gpu, (n, n, n), nfld=1, model=gpu.MODEL_PHI4, model_params=(1.0,)) as lat:
+        S = lat.spectrum_discrete(f, kind)
+        S2 = lat.spectrum_discrete(f, kind)
+    ref = O.spectrum_discrete(O.r2c_3d(f), n, kind)
+    assert S.shape == ref.shape == (int(np.sqrt(3.0) * (n // 2) + 2),)
+    assert np.allclose(S, ref, rtol=1e-12, atol=0)
+    assert np.array_equal(S, S2)
+
+
+def test_discrete_spectra_need_cubic_lattice(gpu):
+    with _lat(gpu, (16, 32, 8)) as lat:
+        with pytest.raises(gpu.LatticeError):
+            lat.spectrum_discrete(np.zeros((8, 32, 16)), 0)
+
+
+@pytest.mark.parametrize("coef",[(-6.0, 1.0, 0.0, 0.0, 1.0), (-128.0, 14.0, 3.0, 1.0, 30.0)], ids=["7pt", "27pt"])
 def test_stencil_matches_oracle(gpu, coef):
     n, L = 32, 10.0
     rng = np.random.default_rng(2)
