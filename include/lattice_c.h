@@ -64,7 +64,11 @@ typedef struct lat_config {
     int model;           /* LAT_MODEL_*                                           */
     double model_params[4]; /* phi4: lambda; twofield: m^2, g^2                  */
     int symp6_tree;      /* symp6 w1 constant: 0 openmp_dev, 1 mpi_dev            */
-    int reserved[7];
+    int gw;              /* 1: evolve the metric perturbations u_ij with the fields
+                            (grav-waves/data-gw.f90:14 hij, pi_hij)                  */
+    double gw_mpl;       /* reduced Planck mass in program units (0 means 1); the
+                            u_ij source coefficient is 2 / gw_mpl^2                */
+    int reserved[4];
 } lat_config;
 
 /* ---- lifetime ---- */
@@ -120,6 +124,38 @@ int  lat_output_spectra(lat_ctx *ctx, int field, double *out);
  *          integer count per floor(|k|/dk) bin, S / W. */
 int  lat_num_bins_discrete(int n);
 int  lat_spectrum_discrete(lat_ctx *ctx, const double *f, double *S, int kind);
+/* The `nofft` variants on a given transform Fk (FFTW layout (n/2+1, n, n)),
+ * summed over the stored half space without the Hermitian mirror:
+ *   kind 0 replaces spectrum_nofft(famp, S), analysis.f90:118-147: two-bin
+ *          weights, S / W / n^6;
+ *   kind 1 replaces spectrum_bin_nofft(famp, S), analysis.f90:54-84: counts,
+ *          x-Nyquist plane skipped (:69).  The reference bins the module
+ *          array Fk there, not its argument famp; pass the array it holds. */
+int  lat_spectrum_discrete_k(lat_ctx *ctx, const double *Fk, double *S, int kind);
+/* Replaces crossspec_nofft(famp1, famp2, S1, S2), analysis.f90:149-186:
+ * Re and Im of F1 conj(F2), two-bin weights, x-Nyquist plane skipped, S / W. */
+int  lat_crossspec_discrete_k(lat_ctx *ctx, const double *Fk1, const double *Fk2, double *S1, double *S2);
+
+/* ---- gravitational waves (module GW_data / GW_Evolution, grav-waves/) ----
+ * State: hij(2,3,nx,ny,nz) and pi_hij(2,3,nx,ny,nz), the allocation shape of
+ * grav-waves/data-gw.f90:28,33 (Fortran order, component pair fastest):
+ * hij(1,b) is the diagonal u_bb, hij(2,1..3) the off-diagonals u_12, u_13, u_23.
+ * Evolution (lat_symp and the split calls, when lat_config.gw = 1):
+ *   drift  u += h pi_u / yscl^2       (the field drift coefficient; cosmic form
+ *                                       of gw_kinetic_cosmic, evolve-gw.f90:12-19)
+ *   kick   pi_u += hA (lap u + (2/M_p^2) S_ij),  S_ij = sum_l d_i phi_l d_j phi_l
+ *          by central differences, hA the field kick's Laplacian coefficient
+ *          (the source the stub gw_source_cosmic, evolve-gw.f90:21-36, leaves empty).
+ * The metric perturbations do not act back on the background. */
+int  lat_set_gw(lat_ctx *ctx, const double *hij, const double *pi_hij);
+int  lat_get_gw(lat_ctx *ctx, double *hij, double *pi_hij);
+/* S_ij of the resident fields, 6 SoA arrays of nvol: xx, xy, xz, yy, yz, zz. */
+int  lat_gw_source(lat_ctx *ctx, double *Sij);
+/* GW spectra of the resident u_ij: TT projection Lambda_ij,lm(k) = P_il P_jm
+ * - P_ij P_lm / 2, P_ij = delta_ij - k_i k_j / k^2 (gw-projectors.f90:13-26,
+ * a stub), then shell binning with the spectrum_3d weights:
+ * out[0..ns) = sum_ij |Lambda pi_u^(k)|^2, out[ns..2ns) = sum_ij |Lambda u^(k)|^2. */
+int  lat_gw_spectrum(lat_ctx *ctx, double *out);
 
 /* ---- state (module Hamiltonian globals fld, fldp, yscl, ysclp,
  *      hamiltonian_conformal.f90:22-24; hamiltonian_cosmic.f90:27,31) ---- */
@@ -179,6 +215,7 @@ enum lat_stage {
     LAT_ST_DRIFT,     /* standalone drift sweep                    */
     LAT_ST_STENCIL,   /* FD stencil + kick                         */
     LAT_ST_REDUCE,    /* state reductions                          */
+    LAT_ST_GW_SRC,    /* u_ij source S_ij = sum d_i phi d_j phi     */
     LAT_NSTAGE
 };
 /* enable = 1 starts per-launch HIP-event timing on the context stream (and

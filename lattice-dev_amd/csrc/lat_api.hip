@@ -147,16 +147,69 @@ static int forward_state(Ctx &c, bool drift, int zmode)
 
 // Laplacian of the state into the C2R epilogue: EPI_KICK or EPI_REDUCE.
 // Returns the number of reduction partials (>0) or an error (<0).
+// Kick of the metric perturbations after the field kick of the same step (the
+// fields hold the drifted phi the source needs).  Spectral: three pairs of
+// components, each S_ij pair -> x R2C (pending drift of u fused) -> y -> z lap
+// -> y -> x C2R with pi_u += hA (lap + gw_coup S).  FD: one stencil sweep.
+template <typename T>
+static int gw_kick(Ctx &c, bool drift)
+{
+    if (c.cfg.laplacian == LAT_LAP_STENCIL) {
+        StageTimer st(c, LAT_ST_STENCIL);
+        return launch_gw_stencil_kick<T>(c); // drift of u applied by launch_drift
+    }
+    const cpx<T> *twx = (const cpx<T> *)(sizeof(T) == 8 ? (void *)c.twx64 : (void *)c.twx32);
+    const cpx<T> *twy = (const cpx<T> *)(sizeof(T) == 8 ? (void *)c.twy64 : (void *)c.twy32);
+    const cpx<T> *twz = (const cpx<T> *)(sizeof(T) == 8 ? (void *)c.twz64 : (void *)c.twz32);
+    T *u = (T *)c.gwu, *p = (T *)c.gwp, *src = (T *)c.gwsrc;
+    cpx<T> *Fk = (cpx<T> *)c.Fk;
+    int rc;
+    for (int c0 = 0; c0 < 6; c0 += 2) {
+        if ((rc = launch_gw_source<T>(c, c0, 2, src))) return rc;
+        for (int q = 0; q < 2; ++q) {
+            StageTimer st(c, LAT_ST_X_R2C);
+            const size_t off = (size_t)(c0 + q) * c.nvol;
+            if ((rc = launch_x_r2c<T>(c, twx, u + off, p + off, Fk + (size_t)q * c.nk, drift ? 1 : 0, -1))) return rc;
+        }
+        {
+            StageTimer st(c, LAT_ST_Y_FWD);
+            if ((rc = launch_col<T>(c, twy, Fk, 1, 0, 2))) return rc;
+        }
+        {
+            StageTimer st(c, LAT_ST_Z_LAP);
+            if ((rc = launch_col<T>(c, twz, Fk, 2, 2, 2))) return rc;
+        }
+        {
+            StageTimer st(c, LAT_ST_Y_INV);
+            if ((rc = launch_col<T>(c, twy, Fk, 1, 1, 2))) return rc;
+        }
+        StageTimer st(c, LAT_ST_X_C2R);
+        rc = launch_x_c2r<T>(c, twx, Fk, 2, EPI_GWKICK, nullptr, src, p + (size_t)c0 * c.nvol);
+        if (rc < 0) return rc;
+    }
+    return 0;
+}
+
 template <typename T>
 static int state_force(Ctx &c, bool drift, int epi)
 {
+    const bool gw = c.gw && epi == EPI_KICK;
     if (c.cfg.laplacian == LAT_LAP_STENCIL) {
         if (drift) {
             int rc = launch_drift<T>(c);
             if (rc) return rc;
         }
-        StageTimer st(c, LAT_ST_STENCIL);
-        return launch_stencil<T>(c, epi);
+        int g;
+        {
+            StageTimer st(c, LAT_ST_STENCIL);
+            g = launch_stencil<T>(c, epi);
+        }
+        if (g < 0) return g;
+        if (gw) {
+            int rc = gw_kick<T>(c, drift);
+            if (rc) return rc;
+        }
+        return g;
     }
     int rc = forward_state<T>(c, drift, 2);
     if (rc) return rc;
@@ -166,8 +219,14 @@ static int state_force(Ctx &c, bool drift, int epi)
         StageTimer st(c, LAT_ST_Y_INV);
         if ((rc = launch_col<T>(c, twy, (cpx<T> *)c.Fk, 1, 1, c.nfld))) return rc;
     }
-    StageTimer st(c, LAT_ST_X_C2R);
-    return launch_x_c2r<T>(c, twx, (cpx<T> *)c.Fk, c.nfld, epi, nullptr, (const T *)c.fld, (T *)c.fldp);
+    int g;
+    {
+        StageTimer st(c, LAT_ST_X_C2R);
+        g = launch_x_c2r<T>(c, twx, (cpx<T> *)c.Fk, c.nfld, epi, nullptr, (const T *)c.fld, (T *)c.fldp);
+    }
+    if (g < 0) return g;
+    if (gw && (rc = gw_kick<T>(c, drift))) return rc;
+    return g;
 }
 
 static int force(Ctx &c, bool drift, int epi)
@@ -444,6 +503,10 @@ __global__ void k_sq_acc(const double *__restrict__ d, double *__restrict__ acc,
 template <typename T>
 static int upload_fields(Ctx &c, void *dst, const double *soa, size_t n)
 {
+    // the copy runs on the null stream, which does not order against the
+    // context's non-blocking stream: drain it first
+    int rc = sync(c);
+    if (rc) return rc;
     if (sizeof(T) == 8) {
         LAT_HIP(&c, hipMemcpy(dst, soa, sizeof(double) * n, hipMemcpyHostToDevice));
         return 0;
@@ -457,6 +520,8 @@ static int upload_fields(Ctx &c, void *dst, const double *soa, size_t n)
 template <typename T>
 static int download_fields(Ctx &c, double *soa, const void *src, size_t n)
 {
+    int rc = sync(c);
+    if (rc) return rc;
     if (sizeof(T) == 8) {
         LAT_HIP(&c, hipMemcpy(soa, src, sizeof(double) * n, hipMemcpyDeviceToHost));
         return 0;
@@ -521,6 +586,13 @@ int lat_create(const lat_config *cfg, lat_ctx **out)
     rc |= ensure(c, &c.fldp, es * c.nfld * c.nvol, "hipMalloc fldp");
     if (cfg->laplacian == LAT_LAP_SPECTRAL || c.prec == LAT_F64)
         rc |= ensure(c, &c.Fk, 2 * es * (c.nfld > 2 ? c.nfld : 2) * c.nk, "hipMalloc Fk");
+    c.gw = cfg->gw != 0;
+    c.gw_coup = 2.0 / ((cfg->gw_mpl > 0.0 ? cfg->gw_mpl : 1.0) * (cfg->gw_mpl > 0.0 ? cfg->gw_mpl : 1.0));
+    if (c.gw) {
+        rc |= ensure(c, &c.gwu, es * 6 * c.nvol, "hipMalloc gwu");
+        rc |= ensure(c, &c.gwp, es * 6 * c.nvol, "hipMalloc gwp");
+        if (cfg->laplacian == LAT_LAP_SPECTRAL) rc |= ensure(c, &c.gwsrc, es * 2 * c.nvol, "hipMalloc gwsrc");
+    }
     rc |= ensure(c, (void **)&c.dsc, sizeof(DevScalars), "hipMalloc dsc");
     rc |= ensure(c, (void **)&c.partials, sizeof(double) * MAX_GRID * NRED, "hipMalloc partials");
     if (rc) { lat_destroy(h); return -3; }
@@ -538,6 +610,10 @@ int lat_create(const lat_config *cfg, lat_ctx **out)
     (void)hipMemset(c.fld, 0, es * c.nfld * c.nvol);
     (void)hipMemset(c.fldp, 0, es * c.nfld * c.nvol);
     (void)hipMemset(c.partials, 0, sizeof(double) * MAX_GRID * NRED);
+    if (c.gw) {
+        (void)hipMemset(c.gwu, 0, es * 6 * c.nvol);
+        (void)hipMemset(c.gwp, 0, es * 6 * c.nvol);
+    }
     c.ke_valid = false;
     *out = h;
     return 0;
@@ -550,7 +626,8 @@ void lat_destroy(lat_ctx *h)
     (void)hipSetDevice(c.device);
     if (c.stream) (void)hipStreamSynchronize(c.stream);
     void *ptrs[] = {c.fld, c.fldp, c.Fk, c.Fk2, c.io64, (c.Fk64 != c.Fk) ? c.Fk64 : nullptr, c.dsc, c.partials,
-                    c.spec_part, c.spec_out, c.twx64, c.twy64, c.twz64, c.twx32, c.twy32, c.twz32};
+                    c.spec_part, c.spec_out, c.twx64, c.twy64, c.twz64, c.twx32, c.twy32, c.twz32,
+                    c.gwu, c.gwp, c.gwsrc, c.gwk};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : c.prof.pool) (void)hipEventDestroy(e);
@@ -723,7 +800,7 @@ int lat_spectrum_discrete(lat_ctx *h, const double *f, double *S, int kind)
     if ((rc = host_r2c(c, f))) return rc;
     const int ns = lat_num_bins_discrete(c.nx);
     const cpx<double> *F = (const cpx<double> *)c.Fk64;
-    if ((rc = launch_bin<double>(c, F, F, 0, ns, kind)) < 0) return rc;
+    if ((rc = launch_bin<double>(c, F, F, 0, ns, kind == 1 ? BIN_COUNTS : 0)) < 0) return rc;
     std::vector<double> raw(2 * (size_t)ns);
     LAT_HIP(&c, hipMemcpyAsync(raw.data(), c.spec_out, sizeof(double) * 2 * ns, hipMemcpyDeviceToHost, c.stream));
     if ((rc = sync(c))) return rc;
@@ -731,6 +808,58 @@ int lat_spectrum_discrete(lat_ctx *h, const double *f, double *S, int kind)
     // kind 0: S/W/n^6 (analysis.f90:113); kind 1: S/W (analysis.f90:50)
     const double n3 = (double)c.nvol;
     finish_spectra(raw, ns, 1, outs, kind == 0 ? n3 * n3 : 1.0);
+    return 0;
+}
+
+int lat_spectrum_discrete_k(lat_ctx *h, const double *Fk, double *S, int kind)
+{
+    CTX_OR_FAIL(h);
+    if (!S || !Fk || (kind != 0 && kind != 1)) return -1;
+    if (c.nx != c.ny || c.nx != c.nz) {
+        c.last_error = "Discrete-tree spectra need a cubic lattice";
+        return -2;
+    }
+    int rc = ensure_spec(c);
+    if (rc) return rc;
+    if ((rc = ensure_io(c))) return rc;
+    cpx<double> *F = (cpx<double> *)c.Fk64;
+    if ((rc = fk_from_host(c, F, Fk))) return rc;
+    const int ns = lat_num_bins_discrete(c.nx);
+    // spectrum_nofft: half space, weights; spectrum_bin_nofft: half space
+    // without the x-Nyquist plane, counts
+    const int flags = kind == 0 ? BIN_HALF : (BIN_HALF | BIN_NO_NYQ | BIN_COUNTS);
+    if ((rc = launch_bin<double>(c, F, F, 0, ns, flags)) < 0) return rc;
+    std::vector<double> raw(2 * (size_t)ns);
+    LAT_HIP(&c, hipMemcpyAsync(raw.data(), c.spec_out, sizeof(double) * 2 * ns, hipMemcpyDeviceToHost, c.stream));
+    if ((rc = sync(c))) return rc;
+    double *outs[1] = {S};
+    const double n3 = (double)c.nvol;
+    finish_spectra(raw, ns, 1, outs, kind == 0 ? n3 * n3 : 1.0); // analysis.f90:146 / :82
+    return 0;
+}
+
+int lat_crossspec_discrete_k(lat_ctx *h, const double *Fk1, const double *Fk2, double *S1, double *S2)
+{
+    CTX_OR_FAIL(h);
+    if (!Fk1 || !Fk2 || !S1 || !S2) return -1;
+    if (c.nx != c.ny || c.nx != c.nz) {
+        c.last_error = "Discrete-tree spectra need a cubic lattice";
+        return -2;
+    }
+    int rc = ensure_spec(c);
+    if (rc) return rc;
+    if ((rc = ensure_io(c))) return rc;
+    cpx<double> *F1 = (cpx<double> *)c.Fk64;
+    cpx<double> *F2 = F1 + c.nk;
+    if ((rc = fk_from_host(c, F1, Fk1))) return rc;
+    if ((rc = fk_from_host(c, F2, Fk2))) return rc;
+    const int ns = lat_num_bins_discrete(c.nx);
+    if ((rc = launch_bin<double>(c, F1, F2, 1, ns, BIN_HALF | BIN_NO_NYQ)) < 0) return rc;
+    std::vector<double> raw(3 * (size_t)ns);
+    LAT_HIP(&c, hipMemcpyAsync(raw.data(), c.spec_out, sizeof(double) * 3 * ns, hipMemcpyDeviceToHost, c.stream));
+    if ((rc = sync(c))) return rc;
+    double *outs[2] = {S1, S2};
+    finish_spectra(raw, ns, 2, outs); // analysis.f90:183-184, no n^6
     return 0;
 }
 
@@ -808,6 +937,124 @@ int lat_output_spectra(lat_ctx *h, int field, double *out)
     int rc = ensure_spec(c);
     if (rc) return rc;
     return c.prec == LAT_F64 ? output_spectra_t<double>(c, field, out) : output_spectra_t<float>(c, field, out);
+}
+
+} // extern "C"
+
+// internal component (xx, xy, xz, yy, yz, zz) of the boundary pair hij(a+1, b+1):
+// a = 0 diagonal u_bb, a = 1 off-diagonal u_12, u_13, u_23
+static int gw_comp(int a, int b)
+{
+    static const int diag[3] = {0, 3, 5}, off[3] = {1, 2, 4};
+    return a == 0 ? diag[b] : off[b];
+}
+
+template <typename T>
+static int gw_spectrum_t(Ctx &c, double *out)
+{
+    const cpx<T> *twx = (const cpx<T> *)(sizeof(T) == 8 ? (void *)c.twx64 : (void *)c.twx32);
+    const cpx<T> *twy = (const cpx<T> *)(sizeof(T) == 8 ? (void *)c.twy64 : (void *)c.twy32);
+    const cpx<T> *twz = (const cpx<T> *)(sizeof(T) == 8 ? (void *)c.twz64 : (void *)c.twz32);
+    int rc = ensure(c, &c.gwk, 6 * sizeof(cpx<T>) * c.nk, "hipMalloc gwk");
+    if (rc) return rc;
+    cpx<T> *K = (cpx<T> *)c.gwk;
+    const int ns = c.ns;
+    std::vector<double> raw(2 * (size_t)ns);
+    for (int ch = 0; ch < 2; ++ch) {
+        T *arr = (T *)(ch == 0 ? c.gwp : c.gwu);
+        for (int q = 0; q < 6; ++q)
+            if ((rc = launch_x_r2c<T>(c, twx, arr + (size_t)q * c.nvol, nullptr, K + (size_t)q * c.nk, 0, -1)))
+                return rc;
+        if ((rc = launch_col<T>(c, twy, K, 1, 0, 6))) return rc;
+        if ((rc = launch_col<T>(c, twz, K, 2, 0, 6))) return rc;
+        if ((rc = launch_bin<T>(c, K, nullptr, 3)) < 0) return rc;
+        LAT_HIP(&c, hipMemcpyAsync(raw.data(), c.spec_out, sizeof(double) * 2 * ns, hipMemcpyDeviceToHost, c.stream));
+        if ((rc = sync(c))) return rc;
+        double *outs[1] = {out + (size_t)ch * ns};
+        finish_spectra(raw, ns, 1, outs);
+    }
+    return 0;
+}
+
+extern "C" {
+
+int lat_set_gw(lat_ctx *h, const double *hij, const double *pi_hij)
+{
+    CTX_OR_FAIL(h);
+    if (!hij || !pi_hij) return -1;
+    if (!c.gw) {
+        c.last_error = "context created without gw";
+        return -2;
+    }
+    std::vector<double> u(6 * c.nvol), p(6 * c.nvol);
+    for (size_t s = 0; s < c.nvol; ++s)
+        for (int b = 0; b < 3; ++b)
+            for (int a = 0; a < 2; ++a) {
+                const size_t q = (size_t)gw_comp(a, b) * c.nvol + s;
+                u[q] = hij[a + 2 * (b + 3 * s)];
+                p[q] = pi_hij[a + 2 * (b + 3 * s)];
+            }
+    int rc = c.prec == LAT_F64 ? upload_fields<double>(c, c.gwu, u.data(), u.size())
+                               : upload_fields<float>(c, c.gwu, u.data(), u.size());
+    if (rc) return rc;
+    return c.prec == LAT_F64 ? upload_fields<double>(c, c.gwp, p.data(), p.size())
+                             : upload_fields<float>(c, c.gwp, p.data(), p.size());
+}
+
+int lat_get_gw(lat_ctx *h, double *hij, double *pi_hij)
+{
+    CTX_OR_FAIL(h);
+    if (!hij || !pi_hij) return -1;
+    if (!c.gw) {
+        c.last_error = "context created without gw";
+        return -2;
+    }
+    std::vector<double> u(6 * c.nvol), p(6 * c.nvol);
+    int rc = c.prec == LAT_F64 ? download_fields<double>(c, u.data(), c.gwu, u.size())
+                               : download_fields<float>(c, u.data(), c.gwu, u.size());
+    if (rc) return rc;
+    rc = c.prec == LAT_F64 ? download_fields<double>(c, p.data(), c.gwp, p.size())
+                           : download_fields<float>(c, p.data(), c.gwp, p.size());
+    if (rc) return rc;
+    for (size_t s = 0; s < c.nvol; ++s)
+        for (int b = 0; b < 3; ++b)
+            for (int a = 0; a < 2; ++a) {
+                const size_t q = (size_t)gw_comp(a, b) * c.nvol + s;
+                hij[a + 2 * (b + 3 * s)] = u[q];
+                pi_hij[a + 2 * (b + 3 * s)] = p[q];
+            }
+    return 0;
+}
+
+int lat_gw_source(lat_ctx *h, double *Sij)
+{
+    CTX_OR_FAIL(h);
+    if (!Sij) return -1;
+    const size_t es = elem_size(c);
+    int rc = ensure(c, &c.gwsrc, es * 2 * c.nvol, "hipMalloc gwsrc");
+    if (rc) return rc;
+    for (int c0 = 0; c0 < 6; c0 += 2) {
+        rc = c.prec == LAT_F64 ? launch_gw_source<double>(c, c0, 2, (double *)c.gwsrc)
+                               : launch_gw_source<float>(c, c0, 2, (float *)c.gwsrc);
+        if (rc) return rc;
+        rc = c.prec == LAT_F64 ? download_fields<double>(c, Sij + (size_t)c0 * c.nvol, c.gwsrc, 2 * c.nvol)
+                               : download_fields<float>(c, Sij + (size_t)c0 * c.nvol, c.gwsrc, 2 * c.nvol);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int lat_gw_spectrum(lat_ctx *h, double *out)
+{
+    CTX_OR_FAIL(h);
+    if (!out) return -1;
+    if (!c.gw) {
+        c.last_error = "context created without gw";
+        return -2;
+    }
+    int rc = ensure_spec(c);
+    if (rc) return rc;
+    return c.prec == LAT_F64 ? gw_spectrum_t<double>(c, out) : gw_spectrum_t<float>(c, out);
 }
 
 int lat_set_state(lat_ctx *h, const double *fld, const double *fldp, int ghost, double yscl, double ysclp)

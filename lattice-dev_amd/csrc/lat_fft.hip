@@ -47,6 +47,12 @@ int grid_for(int tiles)
 
 __device__ __forceinline__ int wmag(int j0, int n) { return (j0 <= n / 2) ? j0 : n - j0; }
 
+// 16-byte vector of reals (4 floats or 2 doubles) for vector loads/stores.
+template <typename T>
+struct alignas(16) Vec16 {
+    T e[16 / sizeof(T)];
+};
+
 // ---------------------------------------------------------------- block reduction
 template <int NCH>
 __device__ __forceinline__ void block_reduce_store(double (&v)[NCH], double *out)
@@ -72,7 +78,19 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NCH], double *out
 // ---------------------------------------------------------------- x R2C
 // With DRIFT: phi += dc0*pi (+ dc1*pi) first, written back (mixed form: on
 // the fluctuations, then minus the re-centering shift nu[field]).
-template <typename T, int N, int L, bool DRIFT, int MR = 4>
+// Twiddle tables: TWL = true copies the N-entry table into LDS once per
+// workgroup, so the FFT passes read twiddles from LDS instead of through the
+// vector memory path.
+#define LAT_TWIDDLES_TO_LDS(NTW, NTHREADS)                                        \
+    __shared__ cpx<T> twl_[TWL ? (NTW) : 1];                                      \
+    const cpx<T> *twp = tw;                                                       \
+    if constexpr (TWL) {                                                          \
+        for (int i_ = threadIdx.x; i_ < (NTW); i_ += (NTHREADS)) twl_[i_] = tw[i_]; \
+        __syncthreads();                                                          \
+        twp = twl_;                                                               \
+    }
+
+template <typename T, int N, int L, bool DRIFT, int MR = 4, bool TWL = false>
 __global__ __launch_bounds__(TPB) void k_x_r2c(T *__restrict__ f, const T *__restrict__ pi,
                                                cpx<T> *__restrict__ Fk, const cpx<T> *__restrict__ tw,
                                                int rows, int P, const DevScalars *__restrict__ dsc, int field)
@@ -80,46 +98,47 @@ __global__ __launch_bounds__(TPB) void k_x_r2c(T *__restrict__ f, const T *__res
     constexpr int M = N / 2;
     constexpr int LS = M + 1;
     __shared__ cpx<T> buf[L * LS];
+    LAT_TWIDDLES_TO_LDS(N, TPB)
     int nd = 0;
     T dc0 = 0, dc1 = 0, nu = 0;
     if constexpr (DRIFT) {
         nd = dsc->nd;
         dc0 = T(dsc->dc[0]);
         dc1 = T(dsc->dc[1]);
-        if (dsc->mixed) nu = T(dsc->nu[field]);
+        if (dsc->mixed && field >= 0) nu = T(dsc->nu[field]);
     }
     int t0, t1, ts;
     tile_range((rows + L - 1) / L, t0, t1, ts);
+    constexpr int VW = 16 / sizeof(T); // reals per 16-byte lane access
+    using VT = Vec16<T>;
     for (int t = t0; t < t1; t += ts) {
         const int r0 = t * L;
-        for (int idx = threadIdx.x; idx < L * M; idx += TPB) {
-            const int line = idx / M, n = idx - line * M;
+        for (int idx = threadIdx.x; idx < L * N / VW; idx += TPB) {
+            const int line = idx / (N / VW), i0 = (idx - line * (N / VW)) * VW;
             const int row = r0 + line;
-            cpx<T> v = {0, 0};
+            VT v{};
             if (row < rows) {
-                cpx<T> *src = reinterpret_cast<cpx<T> *>(f + (size_t)row * N) + n;
+                VT *src = reinterpret_cast<VT *>(f + (size_t)row * N + i0);
                 v = *src;
                 if constexpr (DRIFT) {
                     if (nd > 0) {
-                        const cpx<T> p = reinterpret_cast<const cpx<T> *>(pi + (size_t)row * N)[n];
-                        v.x = v.x + dc0 * p.x;
-                        v.y = v.y + dc0 * p.y;
-                        if (nd > 1) {
-                            v.x = v.x + dc1 * p.x;
-                            v.y = v.y + dc1 * p.y;
-                        }
-                        if constexpr (sizeof(T) == 4) {
-                            v.x = v.x - nu;
-                            v.y = v.y - nu;
+                        const VT p = *reinterpret_cast<const VT *>(pi + (size_t)row * N + i0);
+#pragma unroll
+                        for (int e = 0; e < VW; ++e) {
+                            T x = v.e[e] + dc0 * p.e[e];
+                            if (nd > 1) x = x + dc1 * p.e[e];
+                            if constexpr (sizeof(T) == 4) x = x - nu;
+                            v.e[e] = x;
                         }
                         *src = v;
                     }
                 }
             }
-            buf[line * LS + n] = v;
+#pragma unroll
+            for (int e = 0; e < VW; e += 2) buf[line * LS + (i0 + e) / 2] = cpx<T>{v.e[e], v.e[e + 1]};
         }
         __syncthreads();
-        lds_fft_range<T, M, -1, L, TPB, false, 2, 0, num_passes(M, MR), MR>(buf, LS, 1, tw);
+        lds_fft_range<T, M, -1, L, TPB, false, 2, 0, num_passes(M, MR), MR>(buf, LS, 1, twp);
         // real-FFT split: X[k] = (Z[k] + Z*[M-k])/2 - i W^k (Z[k] - Z*[M-k])/2
         for (int idx = threadIdx.x; idx < L * (M / 2 + 1); idx += TPB) {
             const int line = idx / (M / 2 + 1), k = idx - line * (M / 2 + 1);
@@ -127,7 +146,7 @@ __global__ __launch_bounds__(TPB) void k_x_r2c(T *__restrict__ f, const T *__res
             const int km = (M - k) % M;
             const cpx<T> a = z[k], b = z[km];
             {
-                const cpx<T> w = tw[k];
+                const cpx<T> w = twp[k];
                 const cpx<T> s = {T(0.5) * (a.x + b.x), T(0.5) * (a.y - b.y)};
                 const cpx<T> d = {T(0.5) * (a.x - b.x), T(0.5) * (a.y + b.y)};
                 const cpx<T> wd = cmul(w, d);
@@ -135,7 +154,7 @@ __global__ __launch_bounds__(TPB) void k_x_r2c(T *__restrict__ f, const T *__res
             }
             if (k != M - k) {
                 const int k2 = M - k;
-                const cpx<T> w = tw[k2];
+                const cpx<T> w = twp[k2];
                 const cpx<T> s = {T(0.5) * (b.x + a.x), T(0.5) * (b.y - a.y)};
                 const cpx<T> d = {T(0.5) * (b.x - a.x), T(0.5) * (b.y + a.y)};
                 const cpx<T> wd = cmul(w, d);
@@ -143,10 +162,23 @@ __global__ __launch_bounds__(TPB) void k_x_r2c(T *__restrict__ f, const T *__res
             }
         }
         __syncthreads();
-        for (int idx = threadIdx.x; idx < L * (M + 1); idx += TPB) {
-            const int line = idx / (M + 1), k = idx - line * (M + 1);
-            const int row = r0 + line;
-            if (row < rows) Fk[(size_t)row * P + k] = buf[line * LS + k];
+        if constexpr (sizeof(T) == 4) {
+            // pairs of complex per lane (16 bytes); the odd last element k = M alone
+            for (int idx = threadIdx.x; idx < L * (M / 2 + 1); idx += TPB) {
+                const int line = idx / (M / 2 + 1), kp = idx - line * (M / 2 + 1);
+                const int row = r0 + line;
+                if (row >= rows) continue;
+                cpx<T> *dst = Fk + (size_t)row * P + 2 * kp;
+                const cpx<T> *s = buf + line * LS + 2 * kp;
+                if (kp < M / 2) *reinterpret_cast<VT *>(dst) = VT{{s[0].x, s[0].y, s[1].x, s[1].y}};
+                else *dst = s[0];
+            }
+        } else {
+            for (int idx = threadIdx.x; idx < L * (M + 1); idx += TPB) {
+                const int line = idx / (M + 1), k = idx - line * (M + 1);
+                const int row = r0 + line;
+                if (row < rows) Fk[(size_t)row * P + k] = buf[line * LS + k];
+            }
         }
         __syncthreads();
     }
@@ -154,11 +186,13 @@ __global__ __launch_bounds__(TPB) void k_x_r2c(T *__restrict__ f, const T *__res
 
 // ---------------------------------------------------------------- y / z columns
 // MODE 0 forward, 1 inverse, 2 forward * (-k^2 kfac) inverse (z axis only).
-template <typename T, int N, int TC, int MODE, int MR = 16, bool PF = true, int MINW = 1, int NT = TPB>
+template <typename T, int N, int TC, int MODE, int MR = 16, bool PF = true, int MINW = 1, int NT = TPB,
+          bool TWL = false>
 __global__ __launch_bounds__(NT, MINW) void k_col(cpx<T> *__restrict__ Fk, const cpx<T> *__restrict__ tw, int nxh,
                                              int n_other, size_t s_axis, size_t s_other, size_t s_arr,
                                              int n_arr, int ny, double kfac)
 {
+    LAT_TWIDDLES_TO_LDS(N, NT)
     // Software pipeline per tile of TC adjacent columns:
     //   pass 0 straight from HBM into registers (radix R0, no twiddles),
     //   results to LDS, then the next tile's pass-0 loads are issued into the
@@ -219,7 +253,7 @@ __global__ __launch_bounds__(NT, MINW) void k_col(cpx<T> *__restrict__ Fk, const
             if (line < ncol) base[(size_t)pos * s_axis + line] = a;
         };
         if constexpr (MODE == 2) {
-            lds_fft_range<T, N, -1, TC, NT, true, 1, 1, NP, MR>(buf, 1, TC, tw);
+            lds_fft_range<T, N, -1, TC, NT, true, 1, 1, NP, MR>(buf, 1, TC, twp);
             const int jj = wmag(o, ny);
             for (int idx = threadIdx.x; idx < TC * N; idx += NT) {
                 const int c = idx % TC, n = idx / TC;
@@ -231,21 +265,21 @@ __global__ __launch_bounds__(NT, MINW) void k_col(cpx<T> *__restrict__ Fk, const
                 buf[idx] = {a.x * m, a.y * m};
             }
             __syncthreads();
-            lds_fft_range<T, N, +1, TC, NT, true, 1, 0, NP - 1, MR>(buf, 1, TC, tw);
-            stockham_pass_out<T, N, NP - 1, +1, TC, NT, true, 1, MR>(buf, 1, TC, tw, store);
+            lds_fft_range<T, N, +1, TC, NT, true, 1, 0, NP - 1, MR>(buf, 1, TC, twp);
+            stockham_pass_out<T, N, NP - 1, +1, TC, NT, true, 1, MR>(buf, 1, TC, twp, store);
         } else if constexpr (NP == 1) {
             // single-pass transform: pass 0 was the whole FFT
             for (int idx = threadIdx.x; idx < TC * N; idx += NT) store(idx % TC, idx / TC, buf[idx]);
         } else {
-            lds_fft_range<T, N, SIGN0, TC, NT, true, 1, 1, NP - 1, MR>(buf, 1, TC, tw);
-            stockham_pass_out<T, N, NP - 1, SIGN0, TC, NT, true, 1, MR>(buf, 1, TC, tw, store);
+            lds_fft_range<T, N, SIGN0, TC, NT, true, 1, 1, NP - 1, MR>(buf, 1, TC, twp);
+            stockham_pass_out<T, N, NP - 1, SIGN0, TC, NT, true, 1, MR>(buf, 1, TC, twp, store);
         }
         __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------- x C2R + epilogue
-template <typename T, int N, int L, int EPI, int MR = 4>
+template <typename T, int N, int L, int EPI, int MR = 4, bool PRE = true, bool TWL = false>
 __global__ __launch_bounds__(TPB) void k_x_c2r(const cpx<T> *__restrict__ Fk, const cpx<T> *__restrict__ tw,
                                                int rows, int P, size_t nk, size_t nvol, int nf, T *__restrict__ out,
                                                const T *__restrict__ fld, T *__restrict__ fldp,
@@ -253,36 +287,95 @@ __global__ __launch_bounds__(TPB) void k_x_c2r(const cpx<T> *__restrict__ Fk, co
                                                double *__restrict__ partials)
 {
     constexpr int M = N / 2;
-    constexpr int LS = M + 1;
+    constexpr int LS = M + 2; // even pitch: every row starts 16-byte aligned
     constexpr int MAXF = 2;
     __shared__ cpx<T> buf[MAXF * L * LS];
+    LAT_TWIDDLES_TO_LDS(N, TPB)
     double acc[NRED] = {0, 0, 0, 0, 0, 0, 0, 0}; // RedChan order
     T hA = 0, hB = 0;
-    if constexpr (EPI == EPI_KICK) {
+    if constexpr (EPI == EPI_KICK || EPI == EPI_GWKICK) {
         hA = T(dsc->hA);
         hB = T(dsc->hB);
     }
     float pb32[MAXF] = {0, 0};
     double pb64[MAXF] = {0, 0};
     T mu[MAXF] = {0, 0};
-    if constexpr (sizeof(T) == 4 && EPI != EPI_STORE) {
+    if constexpr (sizeof(T) == 4 && (EPI == EPI_KICK || EPI == EPI_REDUCE)) {
         for (int fi = 0; fi < MAXF; ++fi) {
             pb64[fi] = dsc->mean_phi[fi];
             pb32[fi] = float(pb64[fi]);
             if constexpr (EPI == EPI_KICK) mu[fi] = T(dsc->mu[fi]);
         }
     }
+    constexpr int VW = 16 / sizeof(T); // reals per 16-byte lane access
+    using VT = Vec16<T>;
+    // k-space units of 16 bytes per row: complex pairs in FP32 (the odd last
+    // element k = M is a unit of its own), single complex in FP64
+    constexpr int U = sizeof(T) == 4 ? M / 2 + 1 : M + 1;
+    constexpr int FIT = (MAXF * L * U + TPB - 1) / TPB; // k-space units per thread
+    constexpr int NV = L * N / VW;                      // real 16-byte slots per field
+    constexpr int IT = (NV + TPB - 1) / TPB;            // slots per thread
     int t0, t1, ts;
     tile_range((rows + L - 1) / L, t0, t1, ts);
     for (int t = t0; t < t1; t += ts) {
         const int r0 = t * L;
-        for (int idx = threadIdx.x; idx < nf * L * (M + 1); idx += TPB) {
-            const int fl = idx / (M + 1), k = idx - fl * (M + 1);
+        // 1. issue the k-space loads of this tile
+        VT fr[FIT];
+#pragma unroll
+        for (int q = 0; q < FIT; ++q) {
+            const int idx = threadIdx.x + q * TPB;
+            const int fl = idx / U, u = idx - fl * U;
             const int fi = fl / L, line = fl - fi * L;
             const int row = r0 + line;
-            cpx<T> v = {0, 0};
-            if (row < rows) v = Fk[(size_t)fi * nk + (size_t)row * P + k];
-            buf[fl * LS + k] = v;
+            fr[q] = VT{};
+            if (idx < nf * L * U && row < rows) {
+                const cpx<T> *src = Fk + (size_t)fi * nk + (size_t)row * P;
+                if (sizeof(T) == 8 || u < M / 2) {
+                    fr[q] = *reinterpret_cast<const VT *>(src + (sizeof(T) == 4 ? 2 * u : u));
+                } else {
+                    const cpx<T> v = src[M];
+                    fr[q].e[0] = v.x;
+                    fr[q].e[1] = v.y;
+                }
+            }
+        }
+        // 2. issue the epilogue's state loads now (PRE), so that their latency
+        //    overlaps the transform below
+        VT phv[IT][MAXF], pv[IT][MAXF];
+        auto load_state = [&](int it) {
+            const int idx = threadIdx.x + it * TPB;
+            const int line = idx / (N / VW), i0 = (idx - line * (N / VW)) * VW;
+            const int row = r0 + line;
+            const bool ok = idx < NV && row < rows;
+            const size_t site = (size_t)row * N + i0;
+#pragma unroll
+            for (int fi = 0; fi < MAXF; ++fi) {
+                phv[it][fi] = VT{};
+                pv[it][fi] = VT{};
+                if (ok && fi < nf) {
+                    phv[it][fi] = *reinterpret_cast<const VT *>(fld + (size_t)fi * nvol + site);
+                    pv[it][fi] = *reinterpret_cast<const VT *>(fldp + (size_t)fi * nvol + site);
+                }
+            }
+        };
+        if constexpr (EPI != EPI_STORE && PRE) {
+#pragma unroll
+            for (int it = 0; it < IT; ++it) load_state(it);
+        }
+        // 3. k-space units into LDS
+#pragma unroll
+        for (int q = 0; q < FIT; ++q) {
+            const int idx = threadIdx.x + q * TPB;
+            const int fl = idx / U, u = idx - fl * U;
+            if (idx < nf * L * U) {
+                cpx<T> *d = buf + fl * LS;
+                if constexpr (sizeof(T) == 8) {
+                    d[u] = cpx<T>{fr[q].e[0], fr[q].e[1]};
+                } else {
+                    d[2 * u] = cpx<T>{fr[q].e[0], fr[q].e[1]};
+                    if (u < M / 2) d[2 * u + 1] = cpx<T>{fr[q].e[2], fr[q].e[3]};
+                }
+            }
         }
         __syncthreads();
         // Hermitian pre-split: Z[k] = (X[k] + X*[M-k]) + i e^{+2 pi i k/N} (X[k] - X*[M-k]),
@@ -294,7 +387,7 @@ __global__ __launch_bounds__(TPB) void k_x_c2r(const cpx<T> *__restrict__ Fk, co
             cpx<T> a = z[k], b = z[M - k];
             if (k == 0) { a.y = 0; b.y = 0; }
             {
-                cpx<T> w = tw[k];
+                cpx<T> w = twp[k];
                 w.y = -w.y; // e^{+2 pi i k/N}
                 const cpx<T> s = {a.x + b.x, a.y - b.y};
                 const cpx<T> d = {a.x - b.x, a.y + b.y};
@@ -303,7 +396,7 @@ __global__ __launch_bounds__(TPB) void k_x_c2r(const cpx<T> *__restrict__ Fk, co
             }
             if (k != 0 && k != M - k) {
                 const int k2 = M - k;
-                cpx<T> w = tw[k2];
+                cpx<T> w = twp[k2];
                 w.y = -w.y;
                 const cpx<T> s = {b.x + a.x, b.y - a.y};
                 const cpx<T> d = {b.x - a.x, b.y + a.y};
@@ -312,47 +405,84 @@ __global__ __launch_bounds__(TPB) void k_x_c2r(const cpx<T> *__restrict__ Fk, co
             }
         }
         __syncthreads();
-        if (nf == 2) lds_fft_range<T, M, +1, 2 * L, TPB, false, 2, 0, num_passes(M, MR), MR>(buf, LS, 1, tw);
-        else lds_fft_range<T, M, +1, L, TPB, false, 2, 0, num_passes(M, MR), MR>(buf, LS, 1, tw);
-        for (int idx = threadIdx.x; idx < L * N; idx += TPB) {
-            const int line = idx / N, i = idx - line * N;
+        if (nf == 2) lds_fft_range<T, M, +1, 2 * L, TPB, false, 2, 0, num_passes(M, MR), MR>(buf, LS, 1, twp);
+        else lds_fft_range<T, M, +1, L, TPB, false, 2, 0, num_passes(M, MR), MR>(buf, LS, 1, twp);
+        // 4. epilogue: VW consecutive sites per thread, every global access a
+        //    16-byte vector per lane (4 sites in FP32, 2 in FP64).  Field loops
+        //    are unrolled over MAXF with an nf guard so that every register
+        //    array is indexed by compile-time constants.
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int idx = threadIdx.x + it * TPB;
+            const int line = idx / (N / VW), i0 = (idx - line * (N / VW)) * VW;
             const int row = r0 + line;
-            if (row >= rows) continue;
-            const size_t site = (size_t)row * N + i;
-            T lap[MAXF];
-            for (int fi = 0; fi < nf; ++fi)
-                lap[fi] = reinterpret_cast<const T *>(buf + (fi * L + line) * LS)[i];
+            if (idx >= NV || row >= rows) continue;
+            const size_t site = (size_t)row * N + i0;
+            VT lap[MAXF];
+#pragma unroll
+            for (int fi = 0; fi < MAXF; ++fi)
+                if (fi < nf)
+                    lap[fi] =
+                        *reinterpret_cast<const VT *>(reinterpret_cast<const T *>(buf + (fi * L + line) * LS) + i0);
             if constexpr (EPI == EPI_STORE) {
-                for (int fi = 0; fi < nf; ++fi) out[(size_t)fi * nvol + site] = lap[fi];
-            } else {
-                T phi[MAXF] = {0, 0}, dv[MAXF] = {0, 0};
-                for (int fi = 0; fi < nf; ++fi) phi[fi] = fld[(size_t)fi * nvol + site];
-                if constexpr (sizeof(T) == 4) {
-                    // mixed form: phi, p hold fluctuations; V' and V as differences
-                    if constexpr (EPI == EPI_KICK) model_ddv(md, pb32, phi, dv);
-                    acc[RC_PE] += model_dvv(md, pb64, phi);
-                } else {
-                    if constexpr (EPI == EPI_KICK) model_dv(md, phi, dv);
-                    acc[RC_PE] += model_v(md, phi);
+#pragma unroll
+                for (int fi = 0; fi < MAXF; ++fi)
+                    if (fi < nf) *reinterpret_cast<VT *>(out + (size_t)fi * nvol + site) = lap[fi];
+            } else if constexpr (EPI == EPI_GWKICK) {
+                // tensor components: phv holds the source S, pv the momenta pi_u
+                if constexpr (!PRE) load_state(it);
+                const T gwc = T(md.gwc);
+#pragma unroll
+                for (int fi = 0; fi < MAXF; ++fi) {
+                    if (fi >= nf) break;
+#pragma unroll
+                    for (int v = 0; v < VW; ++v)
+                        pv[it][fi].e[v] = pv[it][fi].e[v] + hA * (lap[fi].e[v] + gwc * phv[it][fi].e[v]);
+                    *reinterpret_cast<VT *>(fldp + (size_t)fi * nvol + site) = pv[it][fi];
                 }
-                acc[RC_M1] += double(phi[0]);
-                acc[RC_M2] += double(phi[1]);
-                for (int fi = 0; fi < nf; ++fi) {
-                    T p = fldp[(size_t)fi * nvol + site];
-                    if constexpr (EPI == EPI_KICK) {
-                        p = p + (hA * lap[fi] - hB * dv[fi]);
-                        if constexpr (sizeof(T) == 4) p = p - mu[fi];
-                        fldp[(size_t)fi * nvol + site] = p;
+            } else {
+                if constexpr (!PRE) load_state(it);
+#pragma unroll
+                for (int v = 0; v < VW; ++v) {
+                    T phi[MAXF] = {0, 0}, dv[MAXF] = {0, 0};
+#pragma unroll
+                    for (int fi = 0; fi < MAXF; ++fi)
+                        if (fi < nf) phi[fi] = phv[it][fi].e[v];
+                    if constexpr (sizeof(T) == 4) {
+                        // mixed form: phi, p hold fluctuations; V' and V as differences
+                        if constexpr (EPI == EPI_KICK) model_ddv(md, pb32, phi, dv);
+                        acc[RC_PE] += model_dvv(md, pb64, phi);
+                    } else {
+                        if constexpr (EPI == EPI_KICK) model_dv(md, phi, dv);
+                        acc[RC_PE] += model_v(md, phi);
                     }
-                    acc[RC_GE] -= double(phi[fi]) * double(lap[fi]);
-                    acc[RC_KE] += double(p) * double(p);
-                    acc[RC_S1 + fi] += double(p);
+                    acc[RC_M1] += double(phi[0]);
+                    acc[RC_M2] += double(phi[1]);
+#pragma unroll
+                    for (int fi = 0; fi < MAXF; ++fi) {
+                        if (fi >= nf) break;
+                        T p = pv[it][fi].e[v];
+                        if constexpr (EPI == EPI_KICK) {
+                            p = p + (hA * lap[fi].e[v] - hB * dv[fi]);
+                            if constexpr (sizeof(T) == 4) p = p - mu[fi];
+                            pv[it][fi].e[v] = p;
+                        }
+                        acc[RC_GE] -= double(phi[fi]) * double(lap[fi].e[v]);
+                        acc[RC_KE] += double(p) * double(p);
+                        acc[RC_S1 + fi] += double(p);
+                    }
+                }
+                if constexpr (EPI == EPI_KICK) {
+#pragma unroll
+                    for (int fi = 0; fi < MAXF; ++fi)
+                        if (fi < nf) *reinterpret_cast<VT *>(fldp + (size_t)fi * nvol + site) = pv[it][fi];
                 }
             }
         }
         __syncthreads();
     }
-    if constexpr (EPI != EPI_STORE) block_reduce_store<NRED>(acc, partials + (size_t)blockIdx.x * NRED);
+    if constexpr (EPI == EPI_KICK || EPI == EPI_REDUCE)
+        block_reduce_store<NRED>(acc, partials + (size_t)blockIdx.x * NRED);
 }
 
 // ---------------------------------------------------------------- launch helpers
@@ -449,7 +579,7 @@ template <typename T>
 int launch_x_c2r(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int nf, int epi, T *out, const T *fld, T *fldp)
 {
     const int rows = c.ny * c.nz;
-    ModelDev md{c.cfg.model, c.nfld, c.cfg.model_params[0], c.cfg.model_params[1]};
+    ModelDev md{c.cfg.model, c.nfld, c.cfg.model_params[0], c.cfg.model_params[1], c.gw_coup};
     if (nf < 1 || nf > 2) {
         c.last_error = "nf must be 1 or 2";
         return -1;
@@ -465,6 +595,9 @@ int launch_x_c2r(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int nf, int epi, T *out, 
         else if (epi == EPI_KICK)                                                                        \
             hipLaunchKernelGGL((k_x_c2r<T, NN, L, EPI_KICK>), dim3(g), dim3(TPB), 0, c.stream, Fk, tw, rows, \
                                c.P, c.nk, c.nvol, nf, out, fld, fldp, c.dsc, md, c.partials);           \
+        else if (epi == EPI_GWKICK)                                                                      \
+            hipLaunchKernelGGL((k_x_c2r<T, NN, L, EPI_GWKICK>), dim3(g), dim3(TPB), 0, c.stream, Fk, tw,    \
+                               rows, c.P, c.nk, c.nvol, nf, out, fld, fldp, c.dsc, md, c.partials);     \
         else                                                                                             \
             hipLaunchKernelGGL((k_x_c2r<T, NN, L, EPI_REDUCE>), dim3(g), dim3(TPB), 0, c.stream, Fk, tw, rows, \
                                c.P, c.nk, c.nvol, nf, out, fld, fldp, c.dsc, md, c.partials);           \

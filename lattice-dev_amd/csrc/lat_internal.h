@@ -68,6 +68,7 @@ struct ModelDev {
     int model;  // LAT_MODEL_*
     int nfld;
     double p0, p1; // phi4: lambda; twofield: m2, g2
+    double gwc;    // u_ij source coefficient 2 / M_p^2 (EPI_GWKICK only)
 };
 
 // Per-stage HIP-event accounting (lat_profile / lat_profile_read).
@@ -95,6 +96,12 @@ struct Ctx {
     void *Fk2;              // nk of cpx<T>, spectra second operand
     double *io64;           // nvol doubles: host-interface staging (FP64)
     void *Fk64;             // nk of cpx<double>: FP64 drop-in transforms
+    // metric perturbations (cfg.gw): 6 components xx,xy,xz,yy,yz,zz of T each
+    bool gw;
+    double gw_coup;         // 2 / M_p^2
+    void *gwu, *gwp;        // 6 * nvol of T
+    void *gwsrc;            // 2 * nvol of T: S_ij of the component pair being kicked
+    void *gwk;              // 6 * nk of cpx<T>: transforms for the GW spectra
     DevScalars *dsc;
     double *partials;       // MAX_GRID * NRED
     double *spec_part;      // spectrum block partials
@@ -117,7 +124,9 @@ size_t elem_size(const Ctx &c);
 
 // ---- FFT stage launchers (lat_fft.hip). T = double or float. ----
 // x-direction R2C of each real row of `f` (nvol), into half-complex rows of Fk.
-// If drift != 0, first applies the pending drifts of dsc to f in place using pi.
+// If drift != 0, first applies the pending drifts of dsc to f in place using pi
+// (field >= 0: a field, re-centred in the mixed form; field < 0: a tensor
+// component, never re-centred).
 template <typename T>
 int launch_x_r2c(Ctx &c, const cpx<T> *tw, T *f, const T *pi, cpx<T> *Fk, int drift, int field = 0);
 // Column FFT along y (axis 1) or z (axis 2) of a half-complex array.
@@ -128,13 +137,19 @@ int launch_col(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int axis, int mode, int n_a
 //   EPI_STORE:  write the real rows into out (one array per field)
 //   EPI_KICK:   kick pi with the device coefficients; reduce GE, PE, KE'
 //   EPI_REDUCE: reduce GE, PE, KE (no state change)
-enum { EPI_STORE = 0, EPI_KICK = 1, EPI_REDUCE = 2 };
+//   EPI_GWKICK: pi_u += hA (lap + gw_coup S) for nf tensor components, with
+//               fld = S and fldp = pi_u of those components; no reductions
+enum { EPI_STORE = 0, EPI_KICK = 1, EPI_REDUCE = 2, EPI_GWKICK = 3 };
 template <typename T>
 int launch_x_c2r(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int nf, int epi, T *out, const T *fld, T *fldp);
 
 // ---- step kernels (lat_step.hip) ----
 template <typename T>
-int launch_drift(Ctx &c);              // apply pending drifts of dsc to all fields
+int launch_drift(Ctx &c);              // apply pending drifts of dsc to all fields (and u_ij)
+template <typename T>
+int launch_gw_source(Ctx &c, int c0, int nc, T *dst); // S_ij components c0..c0+nc-1
+template <typename T>
+int launch_gw_stencil_kick(Ctx &c);    // FD: pi_u += hA (stencil lap u + gw_coup S)
 template <typename T>
 int launch_reduce_state(Ctx &c);        // partials: PE, KE, means (no Laplacian)
 template <typename T>
@@ -143,9 +158,11 @@ int launch_scalar(Ctx &c, const ScalarOps &ops, int npart);
 int grid_for(int tiles);
 
 // ---- spectra (lat_spec.hip) ----
-// mode 0: |F1|^2; 1: F1 conj(F2); 2: output channels.  nbins <= 0: c.ns.
+// mode 0: |F1|^2; 1: F1 conj(F2); 2: output channels; 3: sum_ij |Lambda U|^2 of
+// the six tensor components F1 + c*nk (TT projection).  nbins <= 0: c.ns.
+enum { BIN_COUNTS = 1, BIN_HALF = 2, BIN_NO_NYQ = 4 }; // launch_bin flags
 template <typename T>
-int launch_bin(Ctx &c, const cpx<T> *F1, const cpx<T> *F2, int mode, int nbins = 0, int counts = 0);
+int launch_bin(Ctx &c, const cpx<T> *F1, const cpx<T> *F2, int mode, int nbins = 0, int flags = 0);
 
 // ---- initial conditions (lat_ic.hip) ----
 int sample_to_io(Ctx &c, double gamma, double m2eff, double mpl, double kcut_fac, uint64_t seed, uint64_t stream);

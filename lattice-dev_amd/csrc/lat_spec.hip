@@ -17,14 +17,70 @@
 
 namespace lat {
 
+// TT projection of a symmetric tensor at wavevector kv (signed integers):
+// sum_ab |(Lambda U)_ab|^2 with Lambda_ab,lm = P_al P_bm - P_ab P_lm / 2,
+// P = 1 - k k^T / k^2 (gw-projectors.f90:13-26 is a stub); 0 at k = 0.
+// U holds the components xx, xy, xz, yy, yz, zz.
+__device__ __forceinline__ double tt_norm2(const double kv[3], const double ur[6], const double ui[6])
+{
+    const double k2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+    if (k2 == 0.0) return 0.0;
+    double P[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) P[a][b] = (a == b ? 1.0 : 0.0) - kv[a] * kv[b] / k2;
+    const int ix[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    double mr[3][3], mi[3][3]; // M = P U
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            double sr = 0.0, si = 0.0;
+#pragma unroll
+            for (int l = 0; l < 3; ++l) {
+                sr += P[a][l] * ur[ix[l][b]];
+                si += P[a][l] * ui[ix[l][b]];
+            }
+            mr[a][b] = sr;
+            mi[a][b] = si;
+        }
+    double tr = 0.0, ti = 0.0; // P_lm U_lm = trace(P U)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        tr += mr[a][a];
+        ti += mi[a][a];
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            double rr = -0.5 * P[a][b] * tr, ri = -0.5 * P[a][b] * ti; // (M P)_ab - P_ab tr / 2
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                rr += mr[a][m] * P[m][b];
+                ri += mi[a][m] * P[m][b];
+            }
+            acc += rr * rr + ri * ri;
+        }
+    return acc;
+}
+
 // channel sets: mode 0 -> |F1|^2 ; mode 1 -> Re, Im of F1 conj(F2) ;
-// mode 2 -> |F2|^2, |F1|^2, Re, Im of F1 conj(F2).  W is always the last channel.
-// counts = 1: integer-count bins floor(p) (spectrum_bin,
-// Discrete/Entropy/analysis.f90:22-52) instead of the two-bin weights.
+// mode 2 -> |F2|^2, |F1|^2, Re, Im of F1 conj(F2) ; mode 3 -> sum_ab |Lambda U|^2
+// of the six arrays F1 + c*nk.  W is always the last channel.
+// flags (Discrete/Entropy/analysis.f90 variants):
+//   BIN_COUNTS  integer-count bins floor(p) (spectrum_bin, :22-52) instead of
+//               the two-bin weights;
+//   BIN_HALF    weight 1 per stored half-space mode, no Hermitian mirror (the
+//               `nofft` routines, :54-186);
+//   BIN_NO_NYQ  skip the x-Nyquist plane (spectrum_bin_nofft :69,
+//               crossspec_nofft :166).
 template <typename T, int NCH, int MODE>
 __global__ __launch_bounds__(64) void k_bin(const cpx<T> *__restrict__ F1, const cpx<T> *__restrict__ F2, int nx,
                                             int ny, int nz, int nxh, int P, int ns, double *__restrict__ part,
-                                            int counts)
+                                            int flags, size_t nk)
 {
     extern __shared__ double hist[]; // (NCH+1) * ns
     const int lane = threadIdx.x;
@@ -37,7 +93,8 @@ __global__ __launch_bounds__(64) void k_bin(const cpx<T> *__restrict__ F1, const
         const int kk = (k <= nz / 2) ? k : nz - k;
         for (int i0 = 0; i0 < nxh; i0 += 64) {
             const int i = i0 + lane;
-            const bool valid = i < nxh;
+            const bool counts = (flags & BIN_COUNTS) != 0;
+            const bool valid = i < nxh && !((flags & BIN_NO_NYQ) && 2 * i == nx);
             double v1[NCH + 1], v2[NCH + 1];
             int key = 0x7fffffff;
             if (valid) {
@@ -46,13 +103,24 @@ __global__ __launch_bounds__(64) void k_bin(const cpx<T> *__restrict__ F1, const
                 const double d1 = double(l) - p, d2 = double(l + 1) - p;
                 const double c1 = counts ? 1.0 : (1.0 - d1 * d1) * (1.0 - d1 * d1);
                 const double c2 = counts ? 0.0 : (1.0 - d2 * d2) * (1.0 - d2 * d2);
-                const double mult = (i == 0 || (2 * i == nx)) ? 1.0 : 2.0;
+                const double mult = ((flags & BIN_HALF) || i == 0 || (2 * i == nx)) ? 1.0 : 2.0;
                 const size_t id = (size_t)row * P + i;
                 double val[NCH];
-                const cpx<T> a = F1[id];
-                if constexpr (MODE == 0) {
+                if constexpr (MODE == 3) {
+                    const double kv[3] = {double(i), double(j <= ny / 2 ? j : j - ny), double(k <= nz / 2 ? k : k - nz)};
+                    double ur[6], ui[6];
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) {
+                        const cpx<T> u = F1[(size_t)q * nk + id];
+                        ur[q] = double(u.x);
+                        ui[q] = double(u.y);
+                    }
+                    val[0] = tt_norm2(kv, ur, ui);
+                } else if constexpr (MODE == 0) {
+                    const cpx<T> a = F1[id];
                     val[0] = double(a.x) * a.x + double(a.y) * a.y;
                 } else {
+                    const cpx<T> a = F1[id];
                     const cpx<T> b = F2[id];
                     const double re = double(a.x) * b.x + double(a.y) * b.y;
                     const double im = double(a.y) * b.x - double(a.x) * b.y;
@@ -127,26 +195,29 @@ __global__ __launch_bounds__(256) void k_bin_reduce(const double *__restrict__ p
 constexpr int BIN_BLOCKS = 1024;
 
 template <typename T>
-int launch_bin(Ctx &c, const cpx<T> *F1, const cpx<T> *F2, int mode, int nbins, int counts)
+int launch_bin(Ctx &c, const cpx<T> *F1, const cpx<T> *F2, int mode, int nbins, int flags)
 {
     const int ns = nbins > 0 ? nbins : c.ns;
     if (ns > c.spec_cap) {
         c.last_error = "spectrum bin count exceeds the allocated histogram";
         return -1;
     }
-    const int nch = mode == 0 ? 1 : (mode == 1 ? 2 : 4);
+    const int nch = (mode == 0 || mode == 3) ? 1 : (mode == 1 ? 2 : 4);
     const int rows = c.ny * c.nz;
     const int g = rows < BIN_BLOCKS ? rows : BIN_BLOCKS;
     const size_t lds = sizeof(double) * (nch + 1) * ns;
     if (mode == 0)
         hipLaunchKernelGGL((k_bin<T, 1, 0>), dim3(g), dim3(64), lds, c.stream, F1, F2, c.nx, c.ny, c.nz, c.nxh, c.P,
-                           ns, c.spec_part, counts);
+                           ns, c.spec_part, flags, c.nk);
     else if (mode == 1)
         hipLaunchKernelGGL((k_bin<T, 2, 1>), dim3(g), dim3(64), lds, c.stream, F1, F2, c.nx, c.ny, c.nz, c.nxh, c.P,
-                           ns, c.spec_part, counts);
-    else
+                           ns, c.spec_part, flags, c.nk);
+    else if (mode == 2)
         hipLaunchKernelGGL((k_bin<T, 4, 2>), dim3(g), dim3(64), lds, c.stream, F1, F2, c.nx, c.ny, c.nz, c.nxh, c.P,
-                           ns, c.spec_part, counts);
+                           ns, c.spec_part, flags, c.nk);
+    else
+        hipLaunchKernelGGL((k_bin<T, 1, 3>), dim3(g), dim3(64), lds, c.stream, F1, F2, c.nx, c.ny, c.nz, c.nxh, c.P,
+                           ns, c.spec_part, flags, c.nk);
     LAT_HIP(&c, hipGetLastError());
     const int nq = (nch + 1) * ns;
     hipLaunchKernelGGL(k_bin_reduce, dim3((nq + 255) / 256), dim3(256), 0, c.stream, c.spec_part, g, nq, c.spec_out);

@@ -53,20 +53,22 @@ __device__ __forceinline__ double site_v(const ModelDev &m, const double *pb64, 
 // phi_l += dc[q] * pi_l for the pending drifts q < nd (hamiltonian_conformal.f90:98,
 // hamiltonian_cosmic.f90:107); the p-update was done by k_scalar.  Mixed
 // form: on the fluctuations, minus the re-centering shift nu[l].
+// recenter = 0: tensor components u_ij, never re-centred.
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_drift(T *__restrict__ f, const T *__restrict__ p, size_t nvol, int nf,
-                                               const DevScalars *__restrict__ dsc)
+                                               const DevScalars *__restrict__ dsc, int recenter)
 {
     const int nd = dsc->nd;
     T dc[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) dc[q] = T(dsc->dc[q]);
+    const bool rc = recenter && dsc->mixed;
     const size_t n = nvol * nf;
     for (size_t i = blockIdx.x * (size_t)TPB + threadIdx.x; i < n; i += (size_t)gridDim.x * TPB) {
         T x = f[i];
         const T pi = p[i];
         for (int q = 0; q < nd; ++q) x = x + dc[q] * pi;
-        if constexpr (sizeof(T) == 4) x = x - T(dsc->nu[i / nvol]);
+        if (rc) x = x - T(dsc->nu[i / nvol]);
         f[i] = x;
     }
 }
@@ -74,12 +76,79 @@ __global__ __launch_bounds__(TPB) void k_drift(T *__restrict__ f, const T *__res
 template <typename T>
 int launch_drift(Ctx &c)
 {
-    const size_t n = (size_t)c.nfld * c.nvol;
-    int g = (int)((n + TPB - 1) / TPB);
-    if (g > 8192) g = 8192;
     const int pi = prof_begin(c, LAT_ST_DRIFT);
-    hipLaunchKernelGGL((k_drift<T>), dim3(g), dim3(TPB), 0, c.stream, (T *)c.fld, (const T *)c.fldp, c.nvol, c.nfld,
-                       c.dsc);
+    for (int part = 0; part < (c.gw ? 2 : 1); ++part) {
+        const int nf = part == 0 ? c.nfld : 6;
+        const size_t n = (size_t)nf * c.nvol;
+        int g = (int)((n + TPB - 1) / TPB);
+        if (g > 8192) g = 8192;
+        T *f = (T *)(part == 0 ? c.fld : c.gwu);
+        const T *p = (const T *)(part == 0 ? c.fldp : c.gwp);
+        hipLaunchKernelGGL((k_drift<T>), dim3(g), dim3(TPB), 0, c.stream, f, p, c.nvol, nf, c.dsc, part == 0 ? 1 : 0);
+    }
+    prof_end(c, pi);
+    LAT_HIP(&c, hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------------------- u_ij source
+// S_ij = sum_l d_i phi_l d_j phi_l with second-order central differences
+// d_a f = (f(x + e_a) - f(x - e_a)) / (2 dx) on the periodic lattice; no
+// reference arithmetic exists (grav-waves/evolve-gw.f90:21-36 is a stub).
+// Component order xx, xy, xz, yy, yz, zz.  In the mixed form the fields hold
+// fluctuations about homogeneous means, whose gradients vanish.
+__device__ __forceinline__ void gw_pair(int q, int &a, int &b)
+{
+    a = q < 3 ? 0 : (q < 5 ? 1 : 2);
+    b = q < 3 ? q : (q < 5 ? q - 2 : 2);
+}
+
+template <typename T>
+__device__ __forceinline__ void site_grad(const T *__restrict__ f, int nx, int ny, int nz, int i, int j, int k,
+                                          T r, T g[3])
+{
+    const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
+    const int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
+    const int km = k == 0 ? nz - 1 : k - 1, kp = k == nz - 1 ? 0 : k + 1;
+    const size_t row = (size_t)nx * ((size_t)j + (size_t)ny * k);
+    g[0] = (f[row + ip] - f[row + im]) * r;
+    g[1] = (f[(size_t)nx * ((size_t)jp + (size_t)ny * k) + i] - f[(size_t)nx * ((size_t)jm + (size_t)ny * k) + i]) * r;
+    g[2] = (f[(size_t)nx * ((size_t)j + (size_t)ny * kp) + i] - f[(size_t)nx * ((size_t)j + (size_t)ny * km) + i]) * r;
+}
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_gw_source(const T *__restrict__ fld, int nf, int nx, int ny, int nz, T r,
+                                                   int c0, int nc, T *__restrict__ out)
+{
+    const size_t nvol = (size_t)nx * ny * nz;
+    for (size_t s = blockIdx.x * (size_t)TPB + threadIdx.x; s < nvol; s += (size_t)gridDim.x * TPB) {
+        const int i = (int)(s % nx);
+        const int j = (int)((s / nx) % ny);
+        const int k = (int)(s / ((size_t)nx * ny));
+        T g[2][3] = {{0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+        for (int l = 0; l < 2; ++l)
+            if (l < nf) site_grad(fld + (size_t)l * nvol, nx, ny, nz, i, j, k, r, g[l]);
+        for (int q = 0; q < nc; ++q) {
+            int a, b;
+            gw_pair(c0 + q, a, b);
+            out[(size_t)q * nvol + s] = g[0][a] * g[0][b] + g[1][a] * g[1][b];
+        }
+    }
+}
+
+template <typename T>
+int launch_gw_source(Ctx &c, int c0, int nc, T *dst)
+{
+    if (c0 < 0 || nc < 1 || c0 + nc > 6) {
+        c.last_error = "bad tensor component range";
+        return -1;
+    }
+    size_t blocks = (c.nvol + TPB - 1) / TPB;
+    int g = grid_for(blocks > (size_t)MAX_GRID ? MAX_GRID : (int)blocks);
+    const int pi = prof_begin(c, LAT_ST_GW_SRC);
+    hipLaunchKernelGGL((k_gw_source<T>), dim3(g), dim3(TPB), 0, c.stream, (const T *)c.fld, c.nfld, c.nx, c.ny, c.nz,
+                       T(0.5 / c.dx), c0, nc, dst);
     prof_end(c, pi);
     LAT_HIP(&c, hipGetLastError());
     return 0;
@@ -138,6 +207,32 @@ struct StencilC {
     double c0, c1, c2, c3, elap;
 };
 
+// STENCIL(c, LAPLACIAN) of f at site (xs[1], ys[1], zs[1]); xs/ys/zs hold the
+// periodic neighbour indices (-1, 0, +1) along each axis.  Not yet scaled by
+// 1/(cc dx^2).
+template <typename T>
+__device__ __forceinline__ T stencil_sum(const T *__restrict__ f, const int xs[3], const int ys[3], const int zs[3],
+                                         int nx, int ny, T c0, T c1, T c2, T c3, bool has2, bool has3, T *center)
+{
+#define AT(a, b, cc_) f[(size_t)xs[a] + (size_t)nx * ((size_t)ys[b] + (size_t)ny * zs[cc_])]
+    const T f0 = AT(1, 1, 1);
+    const T r1 = AT(0, 1, 1) + AT(2, 1, 1) + AT(1, 0, 1) + AT(1, 2, 1) + AT(1, 1, 0) + AT(1, 1, 2);
+    T v = c0 * f0 + c1 * r1;
+    if (has2) {
+        const T r2 = AT(0, 0, 1) + AT(2, 0, 1) + AT(0, 2, 1) + AT(2, 2, 1) + AT(0, 1, 0) + AT(2, 1, 0) +
+                     AT(0, 1, 2) + AT(2, 1, 2) + AT(1, 0, 0) + AT(1, 2, 0) + AT(1, 0, 2) + AT(1, 2, 2);
+        v = v + c2 * r2;
+    }
+    if (has3) {
+        const T r3 = AT(0, 0, 0) + AT(2, 0, 0) + AT(0, 2, 0) + AT(2, 2, 0) + AT(0, 0, 2) + AT(2, 0, 2) +
+                     AT(0, 2, 2) + AT(2, 2, 2);
+        v = v + c3 * r3;
+    }
+#undef AT
+    if (center) *center = f0;
+    return v;
+}
+
 template <typename T, int EPI>
 __global__ __launch_bounds__(TPB) void k_stencil(const T *__restrict__ fld, T *__restrict__ fldp, T *__restrict__ out,
                                                  int nx, int ny, int nz, int nf, StencilC sc, ModelDev md,
@@ -172,22 +267,8 @@ __global__ __launch_bounds__(TPB) void k_stencil(const T *__restrict__ fld, T *_
         const int xs[3] = {im, i, ip}, ys[3] = {jm, j, jp}, zs[3] = {km, k, kp};
         T lap[2] = {0, 0}, phi[2] = {0, 0};
         for (int l = 0; l < nf; ++l) {
-            const T *f = fld + (size_t)l * nvol;
-#define AT(a, b, cc_) f[(size_t)xs[a] + (size_t)nx * ((size_t)ys[b] + (size_t)ny * zs[cc_])]
-            const T f0 = AT(1, 1, 1);
-            const T r1 = AT(0, 1, 1) + AT(2, 1, 1) + AT(1, 0, 1) + AT(1, 2, 1) + AT(1, 1, 0) + AT(1, 1, 2);
-            T v = c0 * f0 + c1 * r1;
-            if (has2) {
-                const T r2 = AT(0, 0, 1) + AT(2, 0, 1) + AT(0, 2, 1) + AT(2, 2, 1) + AT(0, 1, 0) + AT(2, 1, 0) +
-                             AT(0, 1, 2) + AT(2, 1, 2) + AT(1, 0, 0) + AT(1, 2, 0) + AT(1, 0, 2) + AT(1, 2, 2);
-                v = v + c2 * r2;
-            }
-            if (has3) {
-                const T r3 = AT(0, 0, 0) + AT(2, 0, 0) + AT(0, 2, 0) + AT(2, 2, 0) + AT(0, 0, 2) + AT(2, 0, 2) +
-                             AT(0, 2, 2) + AT(2, 2, 2);
-                v = v + c3 * r3;
-            }
-#undef AT
+            T f0;
+            const T v = stencil_sum(fld + (size_t)l * nvol, xs, ys, zs, nx, ny, c0, c1, c2, c3, has2, has3, &f0);
             lap[l] = v * el;
             phi[l] = f0;
         }
@@ -235,6 +316,55 @@ int launch_stencil(Ctx &c, int epi)
                            c.ny, c.nz, 1, sc, md, c.dsc, c.partials);
     LAT_HIP(&c, hipGetLastError());
     return g;
+}
+
+// FD path of the u_ij kick: pi_u += hA (STENCIL(c) u / (cc dx^2) + gwc S_ij),
+// S_ij from central differences of the fields, all six components per site.
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_gw_stencil_kick(const T *__restrict__ u, T *__restrict__ pu,
+                                                         const T *__restrict__ fld, int nf, int nx, int ny, int nz,
+                                                         StencilC sc, T r, T gwc,
+                                                         const DevScalars *__restrict__ dsc)
+{
+    const size_t nvol = (size_t)nx * ny * nz;
+    const T c0 = T(sc.c0), c1 = T(sc.c1), c2 = T(sc.c2), c3 = T(sc.c3), el = T(sc.elap);
+    const bool has2 = sc.c2 != 0.0, has3 = sc.c3 != 0.0;
+    const T hA = T(dsc->hA);
+    for (size_t s = blockIdx.x * (size_t)TPB + threadIdx.x; s < nvol; s += (size_t)gridDim.x * TPB) {
+        const int i = (int)(s % nx);
+        const int j = (int)((s / nx) % ny);
+        const int k = (int)(s / ((size_t)nx * ny));
+        const int xs[3] = {i == 0 ? nx - 1 : i - 1, i, i == nx - 1 ? 0 : i + 1};
+        const int ys[3] = {j == 0 ? ny - 1 : j - 1, j, j == ny - 1 ? 0 : j + 1};
+        const int zs[3] = {k == 0 ? nz - 1 : k - 1, k, k == nz - 1 ? 0 : k + 1};
+        T g[2][3] = {{0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+        for (int l = 0; l < 2; ++l)
+            if (l < nf) site_grad(fld + (size_t)l * nvol, nx, ny, nz, i, j, k, r, g[l]);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            int a, b;
+            gw_pair(q, a, b);
+            const T src = g[0][a] * g[0][b] + g[1][a] * g[1][b];
+            const T lap = stencil_sum(u + (size_t)q * nvol, xs, ys, zs, nx, ny, c0, c1, c2, c3, has2, has3,
+                                      (T *)nullptr) * el;
+            T *p = pu + (size_t)q * nvol + s;
+            *p = *p + hA * (lap + gwc * src);
+        }
+    }
+}
+
+template <typename T>
+int launch_gw_stencil_kick(Ctx &c)
+{
+    const double *cs = c.cfg.stencil;
+    StencilC sc{cs[0], cs[1], cs[2], cs[3], 1.0 / (c.dx * c.dx) / cs[4]};
+    size_t blocks = (c.nvol + TPB - 1) / TPB;
+    int g = grid_for(blocks > (size_t)MAX_GRID ? MAX_GRID : (int)blocks);
+    hipLaunchKernelGGL((k_gw_stencil_kick<T>), dim3(g), dim3(TPB), 0, c.stream, (const T *)c.gwu, (T *)c.gwp,
+                       (const T *)c.fld, c.nfld, c.nx, c.ny, c.nz, sc, T(0.5 / c.dx), T(c.gw_coup), c.dsc);
+    LAT_HIP(&c, hipGetLastError());
+    return 0;
 }
 
 // FP64 stencil of one host-interface field (io64 -> io64 + nvol).
@@ -430,5 +560,9 @@ template int launch_reduce_state<double>(Ctx &);
 template int launch_reduce_state<float>(Ctx &);
 template int launch_stencil<double>(Ctx &, int);
 template int launch_stencil<float>(Ctx &, int);
+template int launch_gw_source<double>(Ctx &, int, int, double *);
+template int launch_gw_source<float>(Ctx &, int, int, float *);
+template int launch_gw_stencil_kick<double>(Ctx &);
+template int launch_gw_stencil_kick<float>(Ctx &);
 
 } // namespace lat

@@ -36,7 +36,9 @@ module lattice_dropin
      integer(c_int) :: time_var, model
      real(c_double) :: model_params(4)
      integer(c_int) :: symp6_tree
-     integer(c_int) :: reserved(7)
+     integer(c_int) :: gw
+     real(c_double) :: gw_mpl
+     integer(c_int) :: reserved(4)
   end type lat_config
 
   integer, parameter, public :: LAT_F64 = 0, LAT_MIXED = 1
@@ -53,6 +55,8 @@ module lattice_dropin
   public :: laplacian_3d, init_spectrum_3d, spectrum_3d, crossspec_3d
   public :: Hamiltonian_Split, symp_o2step, symp2, symp4, symp6, symp8, calc_metric
   public :: get_scale_factor, get_hubble, kinetic_norm, grav_energy, lattice_energies
+  public :: spectrum, spectrum_bin, spectrum_nofft, spectrum_bin_nofft, crossspec_nofft
+  public :: lattice_put_gw, lattice_get_gw, gw_spectrum
 
   interface
      integer(c_int) function c_lat_create(cfg, out) bind(C, name="lat_create")
@@ -136,6 +140,44 @@ module lattice_dropin
        type(c_ptr), value :: h
        real(c_double) :: out(*)
      end function c_lat_energies
+     integer(c_int) function c_lat_num_bins_discrete(n) bind(C, name="lat_num_bins_discrete")
+       import :: c_int
+       integer(c_int), value :: n
+     end function c_lat_num_bins_discrete
+     integer(c_int) function c_lat_spectrum_discrete(h, f, s, kind) bind(C, name="lat_spectrum_discrete")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double) :: f(*), s(*)
+       integer(c_int), value :: kind
+     end function c_lat_spectrum_discrete
+     integer(c_int) function c_lat_spectrum_discrete_k(h, fk, s, kind) bind(C, name="lat_spectrum_discrete_k")
+       import :: c_int, c_ptr, c_double, c_double_complex
+       type(c_ptr), value :: h
+       complex(c_double_complex) :: fk(*)
+       real(c_double) :: s(*)
+       integer(c_int), value :: kind
+     end function c_lat_spectrum_discrete_k
+     integer(c_int) function c_lat_crossspec_discrete_k(h, fk1, fk2, s1, s2) bind(C, name="lat_crossspec_discrete_k")
+       import :: c_int, c_ptr, c_double, c_double_complex
+       type(c_ptr), value :: h
+       complex(c_double_complex) :: fk1(*), fk2(*)
+       real(c_double) :: s1(*), s2(*)
+     end function c_lat_crossspec_discrete_k
+     integer(c_int) function c_lat_set_gw(h, hij, pij) bind(C, name="lat_set_gw")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double) :: hij(*), pij(*)
+     end function c_lat_set_gw
+     integer(c_int) function c_lat_get_gw(h, hij, pij) bind(C, name="lat_get_gw")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double) :: hij(*), pij(*)
+     end function c_lat_get_gw
+     integer(c_int) function c_lat_gw_spectrum(h, out) bind(C, name="lat_gw_spectrum")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double) :: out(*)
+     end function c_lat_gw_spectrum
      integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
        import :: c_size_t, c_ptr
        type(c_ptr), value :: s
@@ -160,10 +202,13 @@ contains
 
   ! Context creation: replaces the compile-time params.f90 / macros.h /
   ! HAM= model choice with runtime arguments.
-  subroutine lattice_init(nx, ny, nz, nfld, len, precision, laplacian, time_var, model, model_params, stencil)
+  subroutine lattice_init(nx, ny, nz, nfld, len, precision, laplacian, time_var, model, model_params, stencil, &
+                          gw, gw_mpl)
     integer, intent(in) :: nx, ny, nz, nfld, precision, laplacian, time_var, model
     real(dl), intent(in) :: len, model_params(:)
     real(dl), intent(in), optional :: stencil(5)
+    logical, intent(in), optional :: gw        ! evolve hij, pi_hij (grav-waves/data-gw.f90)
+    real(dl), intent(in), optional :: gw_mpl   ! M_p in program units (default 1)
     type(lat_config) :: cfg
     cfg%nx = nx; cfg%ny = ny; cfg%nz = nz; cfg%nfld = nfld
     cfg%len = len
@@ -174,6 +219,11 @@ contains
     cfg%model_params = 0._dl
     cfg%model_params(1:size(model_params)) = model_params
     cfg%symp6_tree = 1
+    cfg%gw = 0; cfg%gw_mpl = 0._dl
+    if (present(gw)) then
+       if (gw) cfg%gw = 1
+    end if
+    if (present(gw_mpl)) cfg%gw_mpl = gw_mpl
     cfg%reserved = 0
     if (c_associated(ctx)) call c_lat_destroy(ctx)
     call check(c_lat_create(cfg, ctx), "lat_create")
@@ -237,6 +287,65 @@ contains
     real(dl) :: S_real(:), S_imag(:)
     call check(c_lat_crossspec_3d(ctx, Fk1, Fk2, S_real, S_imag), "crossspec_3d")
   end subroutine crossspec_3d
+
+  ! Discrete/Entropy/analysis.f90:89-114 (two-bin weights, /n^6); S(ns_d)
+  subroutine spectrum(f, S)
+    real(c_double) :: f(:,:,:)
+    real(dl) :: S(:)
+    call check(c_lat_spectrum_discrete(ctx, f, S, 0_c_int), "spectrum")
+  end subroutine spectrum
+
+  ! Discrete/Entropy/analysis.f90:22-52 (integer-count bins)
+  subroutine spectrum_bin(f, S)
+    real(c_double) :: f(:,:,:)
+    real(dl) :: S(:)
+    call check(c_lat_spectrum_discrete(ctx, f, S, 1_c_int), "spectrum_bin")
+  end subroutine spectrum_bin
+
+  ! Discrete/Entropy/analysis.f90:118-147
+  subroutine spectrum_nofft(famp, S)
+    complex(c_double_complex) :: famp(:,:,:)
+    real(dl) :: S(:)
+    call check(c_lat_spectrum_discrete_k(ctx, famp, S, 0_c_int), "spectrum_nofft")
+  end subroutine spectrum_nofft
+
+  ! Discrete/Entropy/analysis.f90:54-84 (the reference bins its module Fk here)
+  subroutine spectrum_bin_nofft(famp, S)
+    complex(c_double_complex) :: famp(:,:,:)
+    real(dl) :: S(:)
+    call check(c_lat_spectrum_discrete_k(ctx, famp, S, 1_c_int), "spectrum_bin_nofft")
+  end subroutine spectrum_bin_nofft
+
+  ! Discrete/Entropy/analysis.f90:149-186
+  subroutine crossspec_nofft(famp1, famp2, S1, S2)
+    complex(c_double_complex) :: famp1(:,:,:), famp2(:,:,:)
+    real(dl) :: S1(:), S2(:)
+    call check(c_lat_crossspec_discrete_k(ctx, famp1, famp2, S1, S2), "crossspec_nofft")
+  end subroutine crossspec_nofft
+
+  ! hij(2,3,n,n,n), pi_hij (grav-waves/data-gw.f90:14,28,33)
+  subroutine lattice_put_gw(hij, pi_hij)
+    real(dl) :: hij(:,:,:,:,:), pi_hij(:,:,:,:,:)
+    call check(c_lat_set_gw(ctx, hij, pi_hij), "lattice_put_gw")
+  end subroutine lattice_put_gw
+
+  subroutine lattice_get_gw(hij, pi_hij)
+    real(dl) :: hij(:,:,:,:,:), pi_hij(:,:,:,:,:)
+    call check(c_lat_get_gw(ctx, hij, pi_hij), "lattice_get_gw")
+  end subroutine lattice_get_gw
+
+  ! TT-projected GW spectra: S_pi(ns) from pi_hij, S_h(ns) from hij
+  subroutine gw_spectrum(S_pi, S_h)
+    real(dl) :: S_pi(:), S_h(:)
+    real(dl), allocatable :: out(:)
+    integer :: nb
+    nb = int(c_lat_num_bins(int(gnx, c_int), int(gny, c_int), int(gnz, c_int)))
+    allocate(out(2 * nb))
+    call check(c_lat_gw_spectrum(ctx, out), "gw_spectrum")
+    S_pi(1:nb) = out(1:nb)
+    S_h(1:nb) = out(nb + 1:2 * nb)
+    deallocate(out)
+  end subroutine gw_spectrum
 
   ! hamiltonian_conformal.f90:66-81 / hamiltonian_cosmic.f90:74-89
   subroutine Hamiltonian_Split(dt, hamind)

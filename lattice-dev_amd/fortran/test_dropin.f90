@@ -49,6 +49,24 @@ program test_dropin
   end do
   print *, "spectrum_3d bins", ns, " peak", maxval(S)
 
+  ! Discrete-tree spectra of the same plane wave (Discrete/Entropy/analysis.f90)
+  deallocate(S)
+  allocate(S(int(sqrt(3._dl) * (n / 2) + 2)))
+  g = f
+  call spectrum_bin(g, S)
+  do b = 1, size(S)
+     if ((b == 4) .neqv. (abs(S(b)) > 1e-8_dl * maxval(abs(S)))) then
+        print *, "spectrum_bin bin", b, S(b)
+        stop 1
+     end if
+  end do
+  g = f
+  call spectrum(g, S)
+  if (abs(S(4)) + abs(S(5)) < 1e-8_dl .or. abs(S(3)) > 1e-8_dl * maxval(abs(S))) stop 1
+  print *, "Discrete spectrum / spectrum_bin ok"
+  deallocate(S)
+  allocate(S(ns))
+
   allocate(fld(nfld,0:n+1,0:n+1,0:n+1), fldp(nfld,n,n,n))
   fld = 0._dl; fldp = 0._dl
   do k = 1, n; do j = 1, n; do i = 1, n
@@ -65,6 +83,29 @@ program test_dropin
   if (abs(e(8)) > 1e-9_dl) stop 1
   if (abs(fld(1,0,5,5) - fld(1,n,5,5)) > 0._dl) stop 1   ! periodic ghost
   if (.not. (ysc > 1._dl)) stop 1
+
+  ! metric perturbations sourced by a two-field configuration with gradients
+  call lattice_init(n, n, n, nfld, len, LAT_F64, LAT_LAP_SPECTRAL, LAT_COSMIC, LAT_MODEL_TWOFIELD, &
+                    (/ 1._dl, 100._dl /), gw=.true., gw_mpl=1._dl)
+  do k = 1, n; do j = 1, n; do i = 1, n
+     x = (i-1)*dx; y = (j-1)*dx
+     fld(1,i,j,k) = 1._dl + 0.01_dl*f(i,j,k)
+     fld(2,i,j,k) = 0.01_dl*cos(dk*x)*cos(dk*y)   ! source with a transverse-traceless part
+  end do; end do; end do
+  fldp = 0._dl
+  call lattice_put_state(fld, fldp, 1, 1._dl, 0._dl)
+  block
+    real(dl), allocatable :: hij(:,:,:,:,:), pij(:,:,:,:,:), Sp(:), Sh(:)
+    allocate(hij(2,3,n,n,n), pij(2,3,n,n,n), Sp(ns), Sh(ns))
+    hij = 0._dl; pij = 0._dl
+    call lattice_put_gw(hij, pij)
+    call calc_metric()
+    call symp4(1e-2_dl, 5)
+    call lattice_get_gw(hij, pij)
+    call gw_spectrum(Sp, Sh)
+    print *, "gw: max |pi_h|", maxval(abs(pij)), " peak S_pi", maxval(Sp)
+    if (.not. (maxval(abs(pij)) > 1e-8_dl) .or. .not. (maxval(Sp) > 1e-12_dl)) stop 1
+  end block
   call lattice_finalize()
   print *, "DROPIN OK"
 end program test_dropin

@@ -47,7 +47,7 @@ class Config(ctypes.Structure):
         ("len", ctypes.c_double), ("precision", ctypes.c_int), ("laplacian", ctypes.c_int),
         ("stencil", ctypes.c_double * 5), ("time_var", ctypes.c_int), ("model", ctypes.c_int),
         ("model_params", ctypes.c_double * 4), ("symp6_tree", ctypes.c_int),
-        ("reserved", ctypes.c_int * 7),
+        ("gw", ctypes.c_int), ("gw_mpl", ctypes.c_double), ("reserved", ctypes.c_int * 4),
     ]
 
 
@@ -59,8 +59,10 @@ EXPORTS = [
     "lat_set_scalars", "lat_split", "lat_symp_o2step", "lat_symp", "lat_calc_metric",
     "lat_energies", "lat_sample_field", "lat_device_state", "lat_profile", "lat_profile_read",
     "lat_num_bins_discrete", "lat_spectrum_discrete",
+    "lat_set_gw", "lat_get_gw", "lat_gw_source", "lat_gw_spectrum",
+    "lat_spectrum_discrete_k", "lat_crossspec_discrete_k",
 ]
-STAGES = ["x_r2c", "y_fwd", "z_lap", "y_inv", "x_c2r", "scalar", "drift", "stencil", "reduce"]
+STAGES = ["x_r2c", "y_fwd", "z_lap", "y_inv", "x_c2r", "scalar", "drift", "stencil", "reduce", "gw_src"]
 
 
 def lib():
@@ -99,7 +101,7 @@ class Lattice:
 
     def __init__(self, n, nfld=2, length=10.0, precision=F64, laplacian=LAP_SPECTRAL,
                  stencil=STENCIL_7PT, time_var=COSMIC, model=MODEL_TWOFIELD, model_params=(1.0, 2.5e5),
-                 symp6_tree=1):
+                 symp6_tree=1, gw=False, gw_mpl=1.0):
         if isinstance(n, int):
             n = (n, n, n)
         self.nx, self.ny, self.nz = n
@@ -116,6 +118,7 @@ class Lattice:
         for i, v in enumerate(model_params):
             cfg.model_params[i] = v
         cfg.symp6_tree = symp6_tree
+        cfg.gw, cfg.gw_mpl = int(bool(gw)), gw_mpl
         self.cfg = cfg
         h = ctypes.c_void_p()
         rc = lib().lat_create(ctypes.byref(cfg), ctypes.byref(h))
@@ -210,6 +213,25 @@ class Lattice:
         self._check(lib().lat_spectrum_discrete(self._h, _p(f), _p(S), int(kind)), "spectrum_discrete")
         return S
 
+    def spectrum_discrete_k(self, Fk, kind=0):
+        """kind 0 = spectrum_nofft (Discrete/Entropy/analysis.f90:118-147),
+        kind 1 = spectrum_bin_nofft (:54-84), on a (n, n, n/2+1) transform."""
+        Fk = np.ascontiguousarray(Fk, dtype=np.complex128)
+        S = np.empty(lib().lat_num_bins_discrete(self.nx))
+        self._check(lib().lat_spectrum_discrete_k(self._h, _p(Fk.view(np.float64)), _p(S), int(kind)),
+                    "spectrum_discrete_k")
+        return S
+
+    def crossspec_discrete_k(self, Fk1, Fk2):
+        """crossspec_nofft (Discrete/Entropy/analysis.f90:149-186) -> (S1, S2)."""
+        Fk1 = np.ascontiguousarray(Fk1, dtype=np.complex128)
+        Fk2 = np.ascontiguousarray(Fk2, dtype=np.complex128)
+        ns = lib().lat_num_bins_discrete(self.nx)
+        s1, s2 = np.empty(ns), np.empty(ns)
+        self._check(lib().lat_crossspec_discrete_k(self._h, _p(Fk1.view(np.float64)), _p(Fk2.view(np.float64)),
+                                                   _p(s1), _p(s2)), "crossspec_discrete_k")
+        return s1, s2
+
     def crossspec_3d(self, Fk1, Fk2):
         """crossspec_3d (analysis_spec.f90:145-178) -> (S_real, S_imag)."""
         Fk1 = np.ascontiguousarray(Fk1, dtype=np.complex128)
@@ -225,6 +247,49 @@ class Lattice:
         out = np.empty(4 * self.ns)
         self._check(lib().lat_output_spectra(self._h, field, _p(out)), "output_spectra")
         return out.reshape(4, self.ns)
+
+    # ---------------------------------------------------------------- gravitational waves
+    # u_ij as (6, nz, ny, nx) arrays, components xx, xy, xz, yy, yz, zz; the C ABI
+    # takes the reference's hij(2,3,nx,ny,nz) (grav-waves/data-gw.f90:28).
+    def _gw_to_aos(self, u):
+        u = np.asarray(u, np.float64)
+        aos = np.empty((self.nz, self.ny, self.nx, 3, 2))
+        aos[..., 0, 0], aos[..., 1, 0], aos[..., 2, 0] = u[0], u[3], u[5]
+        aos[..., 0, 1], aos[..., 1, 1], aos[..., 2, 1] = u[1], u[2], u[4]
+        return aos
+
+    def _gw_from_aos(self, aos):
+        return np.stack([aos[..., 0, 0], aos[..., 0, 1], aos[..., 1, 1], aos[..., 1, 0], aos[..., 2, 1],
+                         aos[..., 2, 0]])
+
+    def set_gw(self, u, pu):
+        """Upload u_ij and their momenta, (6, nz, ny, nx) each."""
+        a, b = self._gw_to_aos(u), self._gw_to_aos(pu)
+        self._check(lib().lat_set_gw(self._h, _p(a), _p(b)), "set_gw")
+
+    def set_gw_aos(self, hij, pi_hij):
+        """Reference layout hij(2,3,nx,ny,nz) as a C-order (nz, ny, nx, 3, 2) array."""
+        a = np.ascontiguousarray(hij, np.float64)
+        b = np.ascontiguousarray(pi_hij, np.float64)
+        self._check(lib().lat_set_gw(self._h, _p(a), _p(b)), "set_gw")
+
+    def get_gw(self):
+        a = np.empty((self.nz, self.ny, self.nx, 3, 2))
+        b = np.empty_like(a)
+        self._check(lib().lat_get_gw(self._h, _p(a), _p(b)), "get_gw")
+        return self._gw_from_aos(a), self._gw_from_aos(b)
+
+    def gw_source(self):
+        """S_ij = sum_l d_i phi_l d_j phi_l (central differences) of the resident fields."""
+        out = np.empty((6, self.nz, self.ny, self.nx))
+        self._check(lib().lat_gw_source(self._h, _p(out)), "gw_source")
+        return out
+
+    def gw_spectrum(self):
+        """(sum_ij |Lambda pi_u^|^2, sum_ij |Lambda u^|^2) binned per shell."""
+        out = np.empty(2 * self.ns)
+        self._check(lib().lat_gw_spectrum(self._h, _p(out)), "gw_spectrum")
+        return out.reshape(2, self.ns)
 
     # ---------------------------------------------------------------- state
     def set_state(self, fld, fldp, yscl=1.0, ysclp=0.0):

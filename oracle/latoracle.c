@@ -22,7 +22,9 @@
  *       Pseudospec/openmp_dev/analysis_spec.f90:48-57,59-102,145-178
  *       -> full-spectrum (numpy fftn) brute force; sum rule on W; plane wave.
  *   orc_spectrum_discrete   Discrete/Entropy/analysis.f90:22-52,89-114
- *       -> same brute force with integer counts / n^6 normalisation.
+ *   orc_spectrum_discrete_k / orc_crossspec_discrete_k  analysis.f90:54-84,118-186
+ *       -> same brute force with integer counts / n^6 normalisation / half
+ *          space without mirror.
  *   orc_split / orc_symp_o2step / orc_symp
  *       Pseudospec/openmp_dev/hamiltonian_conformal.f90:50-207,
  *       Pseudospec/mpi_dev/hamiltonian_cosmic.f90:58-213,
@@ -36,11 +38,15 @@
  *   orc_energies       Pseudospec/openmp_dev/evolve_spectral.f90:305-415,
  *       Pseudospec/mpi_dev/hamiltonian_cosmic.f90:268-349, Discrete/Entropy/homlat.f90:90-213
  *       -> Parseval GE == sum k^2|F|^2/N^6; KE/PE by direct numpy sums.
- *   orc_gw_source / orc_tt_project / orc_gw_spectrum
+ *   orc_gw_source / orc_gw_source_fd / orc_tt_project / orc_gw_spectrum and
+ *   the u_ij pieces of the drift and kick (gw = 1)
  *       no reference arithmetic (grav-waves/evolve-gw.f90:21-36 and
- *       gw-projectors.f90:13-26 are empty stubs): PARITY UNPINNED against the
- *       reference; pinned by projector invariants (transverse, traceless,
- *       idempotent) and plane-wave source closed forms.
+ *       gw-projectors.f90:13-26 are empty stubs; the drift follows the
+ *       evolve-gw.f90:12-19 cosmic form h += pi_h dt / a^3): PARITY UNPINNED
+ *       against the reference; pinned by projector invariants (transverse,
+ *       traceless, idempotent), plane-wave source closed forms (spectral and
+ *       central-difference), the free-wave cos(k t) solution in a static
+ *       background, and the vanishing TT part of a one-dimensional source.
  *   orc_sample         Pseudospec/openmp_dev/evolve_spectral.f90:421-506
  *       with a Philox-4x32-10 stream in place of the compiler's random_number
  *       -> DST-II vs direct sum, Hermitian-consistent statistics.
@@ -598,6 +604,85 @@ int orc_spectrum_discrete(int n, const double *Fk, double *S, int kind)
     return ns;
 }
 
+/* Discrete/Entropy/analysis.f90 `nofft` variants on a given half-complex
+ * array: the loop runs over the stored half space only (i = 1..nn, no mirror,
+ * weight 1 per stored mode).
+ *   kind 0: spectrum_nofft (:118-147): all stored x-indices, two-bin weights,
+ *           S/W/n^6;
+ *   kind 1: spectrum_bin_nofft (:54-84): x-index < nn (the x-Nyquist plane
+ *           skipped, :69), integer count per floor(p) bin, S/count. */
+int orc_spectrum_discrete_k(int n, const double *Fk, double *S, int kind)
+{
+    const double sqrt3 = 1.7320508075688772935274463415059;
+    int ns = (int)(sqrt3 * (double)(n / 2) + 2.0);
+    const int nxh = n / 2 + 1;
+    const cplx *F = (const cplx *)Fk;
+    double *W = (double *)calloc((size_t)ns, sizeof(double));
+    memset(S, 0, sizeof(double) * ns);
+    for (int k = 0; k < n; ++k) {
+        int kk = wmag(k, n);
+        for (int j = 0; j < n; ++j) {
+            int jj = wmag(j, n);
+            for (int ii = 0; ii < nxh; ++ii) {
+                if (kind == 1 && ii == n / 2) continue;
+                size_t id = ii + (size_t)nxh * (j + (size_t)n * k);
+                double a2 = creal(F[id] * conj(F[id]));
+                double p = sqrt((double)(ii * ii + jj * jj + kk * kk));
+                int l = (int)floor(p);
+                if (kind == 1) {
+                    S[l] += a2;
+                    W[l] += 1.0;
+                } else {
+                    double d1 = (double)l - p, d2 = (double)(l + 1) - p;
+                    double c1 = (1.0 - d1 * d1) * (1.0 - d1 * d1);
+                    double c2 = (1.0 - d2 * d2) * (1.0 - d2 * d2);
+                    S[l] += c1 * a2; W[l] += c1;
+                    if (l + 1 < ns) { S[l + 1] += c2 * a2; W[l + 1] += c2; }
+                }
+            }
+        }
+    }
+    double n6 = pow((double)n, 6);
+    for (int b = 0; b < ns; ++b)
+        if (W[b] != 0.0) S[b] = (kind == 1) ? S[b] / W[b] : S[b] / W[b] / n6;
+    free(W);
+    return ns;
+}
+
+/* crossspec_nofft (Discrete/Entropy/analysis.f90:149-186): stored half space
+ * with the x-Nyquist plane skipped, two-bin weights on F1 conj(F2), S/W. */
+int orc_crossspec_discrete_k(int n, const double *Fk1, const double *Fk2, double *S1, double *S2)
+{
+    const double sqrt3 = 1.7320508075688772935274463415059;
+    int ns = (int)(sqrt3 * (double)(n / 2) + 2.0);
+    const int nxh = n / 2 + 1;
+    const cplx *F1 = (const cplx *)Fk1, *F2 = (const cplx *)Fk2;
+    double *W = (double *)calloc((size_t)ns, sizeof(double));
+    memset(S1, 0, sizeof(double) * ns);
+    memset(S2, 0, sizeof(double) * ns);
+    for (int k = 0; k < n; ++k) {
+        int kk = wmag(k, n);
+        for (int j = 0; j < n; ++j) {
+            int jj = wmag(j, n);
+            for (int ii = 0; ii < n / 2; ++ii) {
+                size_t id = ii + (size_t)nxh * (j + (size_t)n * k);
+                cplx a = F1[id] * conj(F2[id]);
+                double p = sqrt((double)(ii * ii + jj * jj + kk * kk));
+                int l = (int)floor(p);
+                double d1 = (double)l - p, d2 = (double)(l + 1) - p;
+                double c1 = (1.0 - d1 * d1) * (1.0 - d1 * d1);
+                double c2 = (1.0 - d2 * d2) * (1.0 - d2 * d2);
+                S1[l] += c1 * creal(a); S2[l] += c1 * cimag(a); W[l] += c1;
+                if (l + 1 < ns) { S1[l + 1] += c2 * creal(a); S2[l + 1] += c2 * cimag(a); W[l + 1] += c2; }
+            }
+        }
+    }
+    for (int b = 0; b < ns; ++b)
+        if (W[b] != 0.0) { S1[b] /= W[b]; S2[b] /= W[b]; }
+    free(W);
+    return ns;
+}
+
 /* ------------------------------------------------------------------ */
 /* Model plug-in                                                       */
 /* ------------------------------------------------------------------ */
@@ -679,6 +764,11 @@ static void fields_kin(orc_sim *s, double h)
     double KE2 = plane_sum(part, s->nz) / (double)nv;
     free(part);
     s->ysclp = s->ysclp + h * KE2 / (s->yscl * s->yscl * s->yscl);
+    if (s->gw) {
+        /* u_ij drift with the field drift coefficient; no back-reaction */
+#pragma omp parallel for schedule(static)
+        for (size_t q = 0; q < 6 * nv; ++q) s->gwu[q] = s->gwu[q] + h * s->gwp[q] / y2;
+    }
 }
 
 /* Hamiltonian_grav_kinetic: conformal a -= h p_a/6 (hamiltonian_conformal.f90:86-90);
@@ -687,6 +777,33 @@ static void grav_kinetic(orc_sim *s, double h)
 {
     if (s->time_var == ORC_CONFORMAL) s->yscl = s->yscl - h * s->ysclp / 6.0;
     else s->yscl = s->yscl - h * s->ysclp * (3.0 / 8.0);
+}
+
+/* GW kick pi_u += hA (lap u_c + gw_coup S_c), hA = h a^2 (conformal) or
+ * h y^2 / a^2 (cosmic): the coefficient of the Laplacian in the field kick
+ * (hamiltonian_conformal.f90:183-186, hamiltonian_cosmic.f90:153-157).
+ * No reference arithmetic exists (grav-waves/evolve-gw.f90:21-36 is a stub). */
+static void gw_kick(orc_sim *s, double h)
+{
+    size_t nv = nvol_of(s);
+    const double a = scale_factor(s), y = s->yscl;
+    const double hA = s->time_var == ORC_CONFORMAL ? h * (y * y) : h * (y * y) / (a * a);
+    double *S = (double *)malloc(sizeof(double) * 6 * nv);
+    orc_gw_source_fd(s->nx, s->ny, s->nz, s->nfld, s->fld, dx_of(s), S);
+    for (int c = 0; c < 6; ++c) {
+        const double *u = s->gwu + (size_t)c * nv;
+        if (s->lap_kind == ORC_LAP_SPECTRAL) {
+            memcpy(s->scratch, u, sizeof(double) * nv);
+            orc_laplacian_3d(s->nx, s->ny, s->nz, s->scratch, dk_of(s));
+        } else {
+            orc_stencil_laplacian(s->nx, s->ny, s->nz, u, s->scratch, s->c, dx_of(s));
+        }
+        double *p = s->gwp + (size_t)c * nv;
+        const double *src = S + (size_t)c * nv;
+#pragma omp parallel for schedule(static)
+        for (size_t q = 0; q < nv; ++q) p[q] = p[q] + hA * (s->scratch[q] + s->gw_coup * src[q]);
+    }
+    free(S);
 }
 
 /* Hamiltonian_fields_grad_pot: hamiltonian_conformal.f90:120-207,
@@ -740,6 +857,7 @@ static void fields_grad_pot(orc_sim *s, double h)
     }
     double PE = plane_sum(pe_part, nz);
     free(ge_part); free(pe_part);
+    if (s->gw) gw_kick(s, h);
     const double y = s->yscl;
     if (s->time_var == ORC_CONFORMAL) {
         GE2 = GE2 / (y * y);
@@ -974,6 +1092,38 @@ int orc_gw_source(int nx, int ny, int nz, int nfld, const double *fld, double dk
             for (size_t q = 0; q < nv; ++q) Sij[c * nv + q] += d[ci[c] * nv + q] * d[cj[c] * nv + q];
     }
     free(d); free(F);
+    return 0;
+}
+
+int orc_gw_source_fd(int nx, int ny, int nz, int nfld, const double *fld, double dx, double *Sij)
+{
+    size_t nv = (size_t)nx * ny * nz;
+    const double r = 0.5 / dx;
+    memset(Sij, 0, sizeof(double) * nv * 6);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nz; ++k) {
+        const int km = (k + nz - 1) % nz, kp = (k + 1) % nz;
+        for (int j = 0; j < ny; ++j) {
+            const int jm = (j + ny - 1) % ny, jp = (j + 1) % ny;
+            for (int i = 0; i < nx; ++i) {
+                const int im = (i + nx - 1) % nx, ip = (i + 1) % nx;
+                const size_t q = i + (size_t)nx * (j + (size_t)ny * k);
+                for (int l = 0; l < nfld; ++l) {
+                    const double *f = fld + (size_t)l * nv;
+                    double g[3];
+                    g[0] = (f[ip + (size_t)nx * (j + (size_t)ny * k)] - f[im + (size_t)nx * (j + (size_t)ny * k)]) * r;
+                    g[1] = (f[i + (size_t)nx * (jp + (size_t)ny * k)] - f[i + (size_t)nx * (jm + (size_t)ny * k)]) * r;
+                    g[2] = (f[i + (size_t)nx * (j + (size_t)ny * kp)] - f[i + (size_t)nx * (j + (size_t)ny * km)]) * r;
+                    Sij[0 * nv + q] += g[0] * g[0];
+                    Sij[1 * nv + q] += g[0] * g[1];
+                    Sij[2 * nv + q] += g[0] * g[2];
+                    Sij[3 * nv + q] += g[1] * g[1];
+                    Sij[4 * nv + q] += g[1] * g[2];
+                    Sij[5 * nv + q] += g[2] * g[2];
+                }
+            }
+        }
+    }
     return 0;
 }
 

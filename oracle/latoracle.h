@@ -49,6 +49,10 @@ int  orc_crossspec_3d(int nx, int ny, int nz, const double *Fk1, const double *F
 /* Discrete-tree variants: kind 0 = `spectrum` (weighted, /n^6),
  * kind 1 = `spectrum_bin` (integer counts, bin floor(p)+1). */
 int  orc_spectrum_discrete(int n, const double *Fk, double *S, int kind);
+/* `nofft` variants on a half-complex array: kind 0 = spectrum_nofft,
+ * kind 1 = spectrum_bin_nofft; and crossspec_nofft. */
+int  orc_spectrum_discrete_k(int n, const double *Fk, double *S, int kind);
+int  orc_crossspec_discrete_k(int n, const double *Fk1, const double *Fk2, double *S1, double *S2);
 
 /* ---- Evolution state ---- */
 enum { ORC_CONFORMAL = 0, ORC_COSMIC = 1 };
@@ -67,6 +71,14 @@ typedef struct orc_sim {
     double *fldp;          /* nfld*nvol */
     double *scratch;       /* nvol, Laplacian work array */
     double yscl, ysclp;
+    /* Metric perturbations u_ij (6 components xx,xy,xz,yy,yz,zz, SoA) and
+     * their momenta.  gw = 0: absent.  Drift u += h pi_u / yscl^2 (the field
+     * drift coefficient); kick pi_u += hA (lap u + gw_coup S_ij) with hA the
+     * field kick's Laplacian coefficient and S_ij = sum_l d_i phi_l d_j phi_l
+     * by second-order central differences.  No back-reaction on yscl. */
+    int gw;
+    double gw_coup;        /* 2 / M_p^2 */
+    double *gwu, *gwp;     /* 6*nvol each */
 } orc_sim;
 
 int  orc_split(orc_sim *s, double h, int hamind);
@@ -92,8 +104,11 @@ int  orc_gw_source(int nx, int ny, int nz, int nfld, const double *fld, double d
 int  orc_tt_project(int nx, int ny, int nz, double *Uk /* 6 half-complex arrays */);
 int  orc_gw_spectrum(int nx, int ny, int nz, const double *Uk, const double *dUk,
                      double dk, double *S);
+/* S_ij = sum_l d_i phi_l d_j phi_l with d_a f = (f(x+e_a) - f(x-e_a)) / (2 dx), periodic */
+int  orc_gw_source_fd(int nx, int ny, int nz, int nfld, const double *fld, double dx,
+                      double *Sij /* 6*nvol: xx,xy,xz,yy,yz,zz */);
 
-/* ---- Initial conditions (A16 sampler, deterministic project RNG) ---- */
+/* ---- Initial conditions (evolve_spectral.f90:421-506 sampler, Philox RNG) ---- */
 void orc_philox_normal_pairs(unsigned long long seed, unsigned long long stream,
                              unsigned long long count, double *u1, double *u2);
 int  orc_sample(int nx, int ny, int nz, double len, double gamma, double m2eff,

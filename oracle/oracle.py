@@ -65,6 +65,7 @@ class Sim(ctypes.Structure):
         ("c", ctypes.c_double * 5), ("model", ctypes.c_int), ("mp", ctypes.c_double * 4),
         ("fld", _dp), ("fldp", _dp), ("scratch", _dp),
         ("yscl", ctypes.c_double), ("ysclp", ctypes.c_double),
+        ("gw", ctypes.c_int), ("gw_coup", ctypes.c_double), ("gwu", _dp), ("gwp", _dp),
     ]
 
 
@@ -173,13 +174,29 @@ def spectrum_discrete(Fk, n, kind):
     return S
 
 
+def spectrum_discrete_k(Fk, n, kind):
+    Fk = _c128(Fk)
+    ns = int(1.7320508075688772935274463415059 * (n // 2) + 2.0)
+    S = np.zeros(ns)
+    lib().orc_spectrum_discrete_k(n, _p(Fk.view(np.float64)), _p(S), kind)
+    return S
+
+
+def crossspec_discrete_k(Fk1, Fk2, n):
+    Fk1, Fk2 = _c128(Fk1), _c128(Fk2)
+    ns = int(1.7320508075688772935274463415059 * (n // 2) + 2.0)
+    S1, S2 = np.zeros(ns), np.zeros(ns)
+    lib().orc_crossspec_discrete_k(n, _p(Fk1.view(np.float64)), _p(Fk2.view(np.float64)), _p(S1), _p(S2))
+    return S1, S2
+
+
 # ---------------------------------------------------------------- evolution
 
 class Simulation:
     """FP64 CPU evolution with the reference's split/composition semantics."""
 
     def __init__(self, n, nfld, length, time_var, lap_kind=LAP_SPECTRAL,
-                 stencil=(-6.0, 1.0, 0.0, 0.0, 1.0), model=MODEL_TWOFIELD, mp=(1.0, 0.0)):
+                 stencil=(-6.0, 1.0, 0.0, 0.0, 1.0), model=MODEL_TWOFIELD, mp=(1.0, 0.0), gw=False, gw_mpl=1.0):
         if isinstance(n, int):
             n = (n, n, n)
         self.nx, self.ny, self.nz = n
@@ -198,6 +215,11 @@ class Simulation:
             s.mp[i] = v
         s.fld, s.fldp, s.scratch = _p(self.fld), _p(self.fldp), _p(self.scratch)
         s.yscl, s.ysclp = 1.0, 0.0
+        # metric perturbations u_ij, components xx, xy, xz, yy, yz, zz
+        self.gwu = np.zeros((6, self.nz, self.ny, self.nx))
+        self.gwp = np.zeros_like(self.gwu)
+        s.gw, s.gw_coup = int(bool(gw)), 2.0 / gw_mpl ** 2
+        s.gwu, s.gwp = _p(self.gwu), _p(self.gwp)
 
     @property
     def yscl(self):
@@ -242,6 +264,14 @@ class Simulation:
 
 
 # ---------------------------------------------------------------- GW
+
+def gw_source_fd(fld, dx):
+    fld = _f64(fld)
+    nfld, nz, ny, nx = fld.shape
+    out = np.zeros((6, nz, ny, nx))
+    lib().orc_gw_source_fd(nx, ny, nz, nfld, _p(fld), ctypes.c_double(dx), _p(out))
+    return out
+
 
 def gw_source(fld, dk):
     fld = _f64(fld)

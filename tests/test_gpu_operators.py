@@ -113,13 +113,31 @@ def test_discrete_spectra_match_oracle(gpu, n, kind):
     assert np.array_equal(S, S2)
 
 
+@pytest.mark.parametrize("n", [8, 32, 48])
+def test_discrete_nofft_spectra_match_oracle(gpu, n):
+    print("case discrete nofft spectra", n, flush=True)
+    rng = np.random.default_rng(31 + n)
+    F1 = np.fft.rfftn(rng.standard_normal((n, n, n)))
+    F2 = np.fft.rfftn(rng.standard_normal((n, n, n)))
+    with _lat(gpu, (n, n, n), nfld=1, model=gpu.MODEL_PHI4, model_params=(1.0,)) as lat:
+        S0 = lat.spectrum_discrete_k(F1, 0)
+        S1 = lat.spectrum_discrete_k(F1, 1)
+        c1, c2 = lat.crossspec_discrete_k(F1, F2)
+    assert np.allclose(S0, O.spectrum_discrete_k(F1, n, 0), rtol=1e-12, atol=0)
+    assert np.allclose(S1, O.spectrum_discrete_k(F1, n, 1), rtol=1e-12, atol=0)
+    r1, r2 = O.crossspec_discrete_k(F1, F2, n)
+    scale = np.abs(r1).max()
+    assert np.allclose(c1, r1, rtol=1e-11, atol=1e-13 * scale)
+    assert np.allclose(c2, r2, rtol=1e-11, atol=1e-13 * scale)
+
+
 def test_discrete_spectra_need_cubic_lattice(gpu):
     with _lat(gpu, (16, 32, 8)) as lat:
         with pytest.raises(gpu.LatticeError):
             lat.spectrum_discrete(np.zeros((8, 32, 16)), 0)
 
 
-@pytest.mark.parametrize("coef",[(-6.0, 1.0, 0.0, 0.0, 1.0), (-128.0, 14.0, 3.0, 1.0, 30.0)], ids=["7pt", "27pt"])
+@pytest.mark.parametrize("coef", [(-6.0, 1.0, 0.0, 0.0, 1.0), (-128.0, 14.0, 3.0, 1.0, 30.0)], ids=["7pt", "27pt"])
 def test_stencil_matches_oracle(gpu, coef):
     n, L = 32, 10.0
     rng = np.random.default_rng(2)

@@ -244,6 +244,57 @@ def test_discrete_spectra_brute_force():
     assert np.allclose(S1, refc.real, rtol=1e-12)
 
 
+def brute_half(F1, F2, n, counts=False, skip_nyq=False, n6=False):
+    """Binning over the stored half space of numpy rfftn arrays (no mirror)."""
+    a = (F1 * np.conj(F2)).copy()
+    ii = np.arange(n // 2 + 1)
+    jj = np.minimum(np.arange(n), n - np.arange(n))
+    p = np.sqrt(jj[:, None, None] ** 2 + jj[None, :, None] ** 2 + ii[None, None, :] ** 2)
+    keep = np.ones(p.shape, bool)
+    if skip_nyq:
+        keep[..., n // 2] = False
+    p, a = p[keep], a[keep]
+    ns = int(1.7320508075688772935274463415059 * (n // 2) + 2.0)
+    l = np.floor(p).astype(int)
+    S = np.zeros(ns, complex)
+    W = np.zeros(ns)
+    if counts:
+        np.add.at(S, l, a)
+        np.add.at(W, l, 1.0)
+    else:
+        c1, c2 = (1 - (l - p) ** 2) ** 2, (1 - (l + 1 - p) ** 2) ** 2
+        np.add.at(S, l, c1 * a)
+        np.add.at(W, l, c1)
+        ok = l + 1 < ns
+        np.add.at(S, l[ok] + 1, c2[ok] * a[ok])
+        np.add.at(W, l[ok] + 1, c2[ok])
+    m = W != 0
+    S[m] /= W[m]
+    return S / float(n) ** 6 if n6 else S
+
+
+def test_discrete_nofft_spectra_brute_force():
+    """spectrum_nofft (weights, /n^6, all stored x), spectrum_bin_nofft (counts,
+    x-Nyquist skipped), crossspec_nofft (weights, x-Nyquist skipped, no /n^6)."""
+    n = 8
+    rng = np.random.default_rng(19)
+    F1 = np.fft.rfftn(rng.standard_normal((n, n, n)))
+    F2 = np.fft.rfftn(rng.standard_normal((n, n, n)))
+    S0 = O.spectrum_discrete_k(F1, n, 0)
+    assert np.allclose(S0, brute_half(F1, F1, n, n6=True).real, rtol=1e-12, atol=0)
+    S1 = O.spectrum_discrete_k(F1, n, 1)
+    assert np.allclose(S1, brute_half(F1, F1, n, counts=True, skip_nyq=True).real, rtol=1e-12, atol=0)
+    c1, c2 = O.crossspec_discrete_k(F1, F2, n)
+    ref = brute_half(F1, F2, n, skip_nyq=True)
+    assert np.allclose(c1, ref.real, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+    assert np.allclose(c2, ref.imag, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+    # the x-Nyquist plane enters spectrum_nofft only
+    G = F1.copy()
+    G[..., n // 2] *= 3.0
+    assert not np.allclose(O.spectrum_discrete_k(G, n, 0), S0)
+    assert np.array_equal(O.spectrum_discrete_k(G, n, 1), S1)
+
+
 # ---------------------------------------------------------------- evolution
 
 def hom_sim(time_var, model, mp, nfld, phi0, pi0, n=4, L=5.0):
@@ -485,3 +536,96 @@ def test_gw_source_plane_wave():
     pairs = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
     for c, (i, j) in enumerate(pairs):
         assert np.abs(S[c] - dk**2 * kv[i] * kv[j] * s2).max() < 1e-11
+
+
+def test_gw_source_fd_plane_wave():
+    """Central differences of cos(k.x + 0.3): d_a f = -sin(k.x + 0.3) sin(k_a dx) / dx exactly."""
+    n, L = 16, 6.0
+    dk, dx = 2 * np.pi / L, L / n
+    kv = (1, 2, -1)
+    f = plane(n, L, kv)
+    S = O.gw_source_fd(np.stack([f, 2.0 * f]), dx)
+    x = np.arange(n) * L / n
+    z3, y3, x3 = np.meshgrid(x, x, x, indexing="ij")
+    s2 = np.sin(dk * (kv[0] * x3 + kv[1] * y3 + kv[2] * z3) + 0.3) ** 2
+    g = [np.sin(kv[a] * dk * dx) / dx for a in range(3)]
+    pairs = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    for c, (i, j) in enumerate(pairs):
+        assert np.abs(S[c] - 5.0 * g[i] * g[j] * s2).max() < 1e-11  # 1 + 2^2 from the two fields
+
+
+def _gw_sim(n, L, lap, time_var=O.CONFORMAL, nfld=1, model=O.MODEL_PHI4, mp=(0.0,), **kw):
+    return O.Simulation(n, nfld, L, time_var, lap_kind=lap, model=model, mp=mp, gw=True, **kw)
+
+
+@pytest.mark.parametrize("lap", [O.LAP_SPECTRAL, O.LAP_STENCIL], ids=["spectral", "fd7"])
+def test_gw_free_wave_static_background(lap):
+    """No fields, a = 1, p_a = 0: the background stays static and each u_ij
+    mode is a free wave, u(t) = A cos(w t) cos(k.x) with w^2 = k^2 (spectral)
+    or the 7-point eigenvalue (FD).  Yoshida-4 error shrinks as dt^4."""
+    n, L = 16, 8.0
+    dk, dx = 2 * np.pi / L, L / n
+    kv = np.array([2, 1, -1])
+    x = np.arange(n) * dx
+    z3, y3, x3 = np.meshgrid(x, x, x, indexing="ij")
+    mode = np.cos(dk * (kv[0] * x3 + kv[1] * y3 + kv[2] * z3))
+    if lap == O.LAP_SPECTRAL:
+        w = dk * np.sqrt((kv ** 2).sum())
+    else:
+        w = np.sqrt((4.0 / dx ** 2) * (np.sin(kv * dk * dx / 2) ** 2).sum())
+    T = 1.0
+    errs = []
+    for nsteps in (20, 40):
+        s = _gw_sim(n, L, lap)
+        s.gwu[1] = 0.5 * mode
+        s.gwu[4] = -0.25 * mode
+        s.symp(4, T / nsteps, nsteps)
+        assert s.yscl == 1.0 and s.ysclp == 0.0
+        exact = np.cos(w * T) * mode
+        errs.append(max(np.abs(s.gwu[1] - 0.5 * exact).max(), np.abs(s.gwu[4] + 0.25 * exact).max()))
+        assert np.abs(s.gwu[[0, 2, 3, 5]]).max() == 0.0
+        assert np.abs(s.gwp[1] + 0.5 * w * np.sin(w * T) * mode).max() < 1e-3
+    assert errs[0] < 1e-4
+    assert 12.0 < errs[0] / errs[1] < 20.0
+
+
+@pytest.mark.parametrize("time_var", [O.CONFORMAL, O.COSMIC], ids=["conformal", "cosmic"])
+def test_gw_single_kick_map(time_var):
+    """One kick from u = pi_u = 0: pi_u = hA * (2 / M_p^2) * S_ij exactly, with
+    hA the field kick's Laplacian coefficient (a^2 h conformal, y^2 h / a^2
+    cosmic) and S from numpy central differences."""
+    n, L, h, mpl = 8, 5.0, 0.01, 0.7
+    rng = np.random.default_rng(4)
+    s = _gw_sim(n, L, O.LAP_SPECTRAL, time_var=time_var, nfld=2, model=O.MODEL_TWOFIELD, mp=(1.0, 10.0),
+                gw_mpl=mpl)
+    s.fld[...] = 0.1 * rng.standard_normal(s.fld.shape)
+    s.fldp[...] = 0.1 * rng.standard_normal(s.fld.shape)
+    s.yscl = 1.3
+    s.split(h, 3)
+    dx = L / n
+    g = [[(np.roll(f, -1, axis=2 - a) - np.roll(f, 1, axis=2 - a)) / (2 * dx) for a in range(3)] for f in s.fld]
+    pairs = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    a = 1.3 if time_var == O.CONFORMAL else 1.3 ** (2.0 / 3.0)
+    hA = h * 1.3 ** 2 if time_var == O.CONFORMAL else h * 1.3 ** 2 / a ** 2
+    for c, (i, j) in enumerate(pairs):
+        Sij = sum(gl[i] * gl[j] for gl in g)
+        assert np.abs(s.gwp[c] - hA * (2.0 / mpl ** 2) * Sij).max() < 1e-14 * max(1.0, np.abs(Sij).max())
+    assert np.abs(s.gwu).max() == 0.0
+
+
+def test_gw_one_dimensional_source_has_no_tt_part():
+    """A field varying along x only sources only u_xx, whose modes all have k
+    along x: the TT projection (and the GW spectrum) vanish identically."""
+    n, L = 16, 6.0
+    x = np.arange(n) * L / n
+    s = _gw_sim(n, L, O.LAP_SPECTRAL, time_var=O.COSMIC, nfld=2, model=O.MODEL_TWOFIELD, mp=(1.0, 10.0))
+    prof = 0.2 * np.cos(2 * np.pi * x / L) + 0.05 * np.sin(6 * np.pi * x / L)
+    s.fld[0] = 1.0 + prof[None, None, :]
+    s.fld[1] = 0.3 * prof[None, None, :]
+    s.calc_metric()
+    s.symp(4, 0.01, 10)
+    assert np.abs(s.gwu[0]).max() > 1e-6
+    assert np.abs(s.gwu[1:]).max() == 0.0
+    Uk = np.stack([O.r2c_3d(s.gwp[c]) for c in range(6)])
+    TT = O.tt_project(Uk, n)
+    assert np.abs(TT).max() < 1e-12 * np.abs(Uk).max()
