@@ -163,17 +163,18 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="only run a plain 2 GiB device copy (3 times) to calibrate HBM byte counters")
+                    help="only run the library's plain 2 GiB device copy (4 launches) to calibrate HBM byte "
+                         "counters")
     args = ap.parse_args()
     if args.copy_calibration:
-        import torch
+        # the library's plain copy kernel (k_copy16) over 2 GiB, 1 + 3 launches:
+        # per launch it reads and writes exactly 2 GiB
+        import lattice_amd as la
 
-        a = torch.ones(2 ** 28, dtype=torch.float64, device="cuda")  # 2 GiB
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8}))
+        with la.Lattice(8, nfld=2, length=1.0) as lat:
+            gbs = lat.copy_bandwidth(2 << 30, 3)
+        print(json.dumps({"copy_bytes_read_per_launch": 2 << 30, "copy_bytes_written_per_launch": 2 << 30,
+                          "copy_GBs": gbs}))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design note, "
@@ -185,6 +186,9 @@ def main():
     n = args.n
     dt = (10.0 / n) / 40.0
     forms = args.forms.split(",")
+    with la.Lattice(8, nfld=2, length=1.0) as probe:
+        copy_gbs = probe.copy_bandwidth(2 << 30, 5)
+    print(f"[bench] measured copy bandwidth {copy_gbs:.1f} GB/s", file=sys.stderr, flush=True)
     res = {}
     for form in forms:
         prec = la.F64 if form == "f64" else la.MIXED
@@ -214,7 +218,10 @@ def main():
                                f"Yoshida-4, L=10/m, dt=dx/40, spectra every {args.output_every} steps",
                    "form": head},
         "roofline": roofline(r["prof"], n, 2, w),
+        "hbm_copy_GBs": round(copy_gbs, 1),
     }
+    if line["roofline"]:
+        line["roofline"]["frac_of_copy"] = round(line["roofline"]["achieved"] / copy_gbs, 4)
     if "f64" in res:
         f = res["f64"]
         v64 = n ** 3 * args.steps / f["elapsed"]

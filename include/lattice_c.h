@@ -74,6 +74,54 @@ typedef struct lat_config {
 /* ---- lifetime ---- */
 int  lat_create(const lat_config *cfg, lat_ctx **out);
 void lat_destroy(lat_ctx *ctx);
+
+/* ---- batches of realisations (ensembles, coupling scans) ----
+ * No reference counterpart: the reference runs one realisation per process.
+ * A batch holds nmembers independent realisations of one lattice size, each
+ * with its own fields, FRW background and model parameters (model_params:
+ * nmembers x 4, nullable = cfg->model_params for all).  lat_symp / lat_split /
+ * lat_symp_o2step on a batch advance every member with one launch per sweep
+ * (grid y = member).  Per-member calls (state, IC, energies, spectra) go
+ * through lat_member views, which borrow the batch's memory and must not
+ * outlive it; lat_destroy on a view frees only the view.  Members of a
+ * batch share dt; the metric perturbations are not available in batches. */
+int  lat_create_batch(const lat_config *cfg, int nmembers, const double *model_params, lat_ctx **out);
+int  lat_num_members(const lat_ctx *ctx);
+int  lat_member(lat_ctx *batch, int member, lat_ctx **view);
+
+/* Measured HBM bandwidth (no reference counterpart): a plain 16-byte-per-lane
+ * device copy of `bytes` (read + write counted) repeated `reps` times on the
+ * context's stream; *gbps = 2 * bytes * reps / time. */
+int  lat_copy_bandwidth(lat_ctx *ctx, size_t bytes, int reps, double *gbps);
+
+/* Autotuner (no reference counterpart): for lattices of up to 256 per axis,
+ * times two launch shapes of each sweep of the spectral Laplacian (x R2C and
+ * x C2R rows per workgroup, y and z columns per tile) on the context's arrays
+ * and keeps the faster; fields are not modified.  Ensembles tune every size
+ * class at creation.  lat_tuning reports the choices (0 = default shape):
+ * out[4] = {x R2C, y, z, x C2R}. */
+int  lat_autotune(lat_ctx *ctx);
+int  lat_tuning(lat_ctx *ctx, int *out);
+int  lat_set_tuning(lat_ctx *ctx, const int *in);   /* in[4] of 0/1, as lat_tuning reports */
+
+/* Ensembles of uneven lattice sizes (no reference counterpart): nclass cubic
+ * size classes, class k with n[k]^3 sites, count[k] members and box side
+ * len[k] (nullable: base->len); model_params holds 4 values per member in
+ * class-major order (nullable).  All classes share one HIP stream.
+ * lat_ensemble_symp advances every member of class k by nsteps steps of
+ * dt[k]: the first call with given (order, nsteps, dt) is captured into a
+ * hipGraph (one launch per sweep per class), later calls replay it.  Members
+ * are addressed class-major through lat_ensemble_member views. */
+typedef struct lat_ensemble lat_ensemble;
+int  lat_ensemble_create(const lat_config *base, int nclass, const int *n, const int *count,
+                         const double *len, const double *model_params, lat_ensemble **out);
+void lat_ensemble_destroy(lat_ensemble *ens);
+const char *lat_ensemble_last_error(const lat_ensemble *ens);
+int  lat_ensemble_size(const lat_ensemble *ens);
+int  lat_ensemble_member(lat_ensemble *ens, int index, lat_ctx **view);
+int  lat_ensemble_use_graph(lat_ensemble *ens, int enable);   /* default 1 */
+int  lat_ensemble_symp(lat_ensemble *ens, int order, const double *dt, int nsteps);
+int  lat_ensemble_synchronize(lat_ensemble *ens);
 const char *lat_last_error(const lat_ctx *ctx);
 int  lat_synchronize(lat_ctx *ctx);
 
@@ -156,6 +204,23 @@ int  lat_gw_source(lat_ctx *ctx, double *Sij);
  * a stub), then shell binning with the spectrum_3d weights:
  * out[0..ns) = sum_ij |Lambda pi_u^(k)|^2, out[ns..2ns) = sum_ij |Lambda u^(k)|^2. */
 int  lat_gw_spectrum(lat_ctx *ctx, double *out);
+
+/* Replaces make_output(time) with dump_rho (openmp_dev/evolve_spectral.f90:
+ * 305-415,523-551; mpi_dev/evolve_spectral.f90:311-313,455-479): appends one
+ * '(30(ES22.15,2X))' line to <dir>/energy_spec.out (time, a, rho, KE, PE, GE,
+ * -3H^2, (rho-3H^2)/rho, -ysclp^2/12/a^4, <phi_1>, then <phi_2> for tree 0 =
+ * openmp_dev or <pi_1> for tree 1 = mpi_dev) and the ns-line spectrum block
+ * ((b-1) dk, S_phi1, S_pi1, Re and Im of the cross spectrum, 2*nfld-2 unset
+ * columns written as 0) plus an empty record to <dir>/spectrum.out. */
+int  lat_make_output(lat_ctx *ctx, double time, int tree, const char *dir);
+
+/* Replace the checkpoint write/read of Discrete/Entropy/driver.f90:68-118
+ * (make_check_pt / read_check_pt, scafld.f90:224-284) for one process:
+ * <dir>/fld_<i>.raw and fldp_<i>.raw (field i = 1..nfld, native FP64, x
+ * fastest, no ghosts) and <dir>/checkpt.dat ("yscl ysclp", "time",
+ * "outframenum", list-directed).  lat_restore sets the state from them. */
+int  lat_checkpoint(lat_ctx *ctx, const char *dir, double time, int outframenum);
+int  lat_restore(lat_ctx *ctx, const char *dir, double *time, int *outframenum);
 
 /* ---- state (module Hamiltonian globals fld, fldp, yscl, ysclp,
  *      hamiltonian_conformal.f90:22-24; hamiltonian_cosmic.f90:27,31) ---- */

@@ -1,5 +1,6 @@
 // lat_api.hip -- C ABI (include/lattice_c.h): context, state transfer, the
 // drop-in operators, and the device-resident symplectic executor.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -72,6 +73,26 @@ static int push_scalars(Ctx &c)
 {
     LAT_HIP(&c, hipMemcpyAsync(c.dsc, &c.host_sc, sizeof(DevScalars), hipMemcpyHostToDevice, c.stream));
     return sync(c);
+}
+
+// Validity of red[RC_KE] (sum pi^2 of the current state), per member: a view
+// reads and writes its batch's flag for that member.
+static bool get_ke_valid(const Ctx &c)
+{
+    if (c.view && c.parent) return c.parent->mvalid[c.member] != 0;
+    if (c.B > 1) {
+        for (char v : c.mvalid)
+            if (!v) return false;
+        return true;
+    }
+    return c.ke_valid;
+}
+
+static void set_ke_valid(Ctx &c, bool v)
+{
+    c.ke_valid = v;
+    if (c.view && c.parent) c.parent->mvalid[c.member] = v;
+    if (c.B > 1) std::fill(c.mvalid.begin(), c.mvalid.end(), (char)v);
 }
 
 // ---------------------------------------------------------------- profiling
@@ -315,7 +336,7 @@ static int build_symp(const Ctx &c, std::vector<Op> &ops, int order, double dt, 
 static int run_ops(Ctx &c, const std::vector<Op> &ops)
 {
     int npart = 0;
-    if (!c.ke_valid) {
+    if (!get_ke_valid(c)) {
         npart = c.prec == LAT_F64 ? launch_reduce_state<double>(c) : launch_reduce_state<float>(c);
         if (npart < 0) return npart;
     }
@@ -376,7 +397,7 @@ static int run_ops(Ctx &c, const std::vector<Op> &ops)
         int rc = c.prec == LAT_F64 ? launch_drift<double>(c) : launch_drift<float>(c);
         if (rc) return rc;
     }
-    c.ke_valid = true;
+    set_ke_valid(c, true);
     return 0;
 }
 
@@ -388,7 +409,7 @@ static int state_sums(Ctx &c)
     ScalarOps none{};
     int rc = launch_scalar(c, none, g);
     if (rc) return rc;
-    c.ke_valid = true;
+    set_ke_valid(c, true);
     return pull_scalars(c);
 }
 
@@ -541,12 +562,24 @@ using namespace lat;
     Ctx &c = (ctx)->c;              \
     LAT_HIP(&c, hipSetDevice(c.device))
 
+// per-member operations take a single context or a member view, not a batch
+#define ONE_MEMBER(c)                                                          \
+    if ((c).B != 1) {                                                          \
+        (c).last_error = "batch context: use lat_member for per-member calls"; \
+        return -2;                                                             \
+    }
+
 extern "C" {
 
-int lat_create(const lat_config *cfg, lat_ctx **out)
+} // extern "C"
+
+// B realisations of one configuration; mparams (B x 4, nullable) overrides
+// lat_config.model_params per member.
+static int create_ctx(const lat_config *cfg, int B, const double *mparams, lat_ctx **out)
 {
-    if (!cfg || !out) return -1;
+    if (!cfg || !out || B < 1) return -1;
     *out = nullptr;
+    if (B > 1 && cfg->gw) return -2; // u_ij are evolved in single contexts only
     if (!size_ok(cfg->nx) || !size_ok(cfg->ny) || !size_ok(cfg->nz)) return -2;
     if (cfg->nfld < 1 || cfg->nfld > 2) return -2;
     if (cfg->model == LAT_MODEL_TWOFIELD && cfg->nfld != 2) return -2;
@@ -578,14 +611,18 @@ int lat_create(const lat_config *cfg, lat_ctx **out)
         const int nd = lat_num_bins_discrete(nmax);
         c.spec_cap = c.ns > nd ? c.ns : nd;
     }
+    c.B = B;
+    c.mvalid.assign(B, 0);
+    c.ms.f = (size_t)c.nfld * c.nvol;
+    c.ms.k = (size_t)(c.nfld > 2 ? c.nfld : 2) * c.nk;
     int rc = 0;
     hipError_t e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete h; return -3; }
     const size_t es = elem_size(c);
-    rc |= ensure(c, &c.fld, es * c.nfld * c.nvol, "hipMalloc fld");
-    rc |= ensure(c, &c.fldp, es * c.nfld * c.nvol, "hipMalloc fldp");
+    rc |= ensure(c, &c.fld, es * B * c.ms.f, "hipMalloc fld");
+    rc |= ensure(c, &c.fldp, es * B * c.ms.f, "hipMalloc fldp");
     if (cfg->laplacian == LAT_LAP_SPECTRAL || c.prec == LAT_F64)
-        rc |= ensure(c, &c.Fk, 2 * es * (c.nfld > 2 ? c.nfld : 2) * c.nk, "hipMalloc Fk");
+        rc |= ensure(c, &c.Fk, 2 * es * B * c.ms.k, "hipMalloc Fk");
     c.gw = cfg->gw != 0;
     c.gw_coup = 2.0 / ((cfg->gw_mpl > 0.0 ? cfg->gw_mpl : 1.0) * (cfg->gw_mpl > 0.0 ? cfg->gw_mpl : 1.0));
     if (c.gw) {
@@ -593,8 +630,8 @@ int lat_create(const lat_config *cfg, lat_ctx **out)
         rc |= ensure(c, &c.gwp, es * 6 * c.nvol, "hipMalloc gwp");
         if (cfg->laplacian == LAT_LAP_SPECTRAL) rc |= ensure(c, &c.gwsrc, es * 2 * c.nvol, "hipMalloc gwsrc");
     }
-    rc |= ensure(c, (void **)&c.dsc, sizeof(DevScalars), "hipMalloc dsc");
-    rc |= ensure(c, (void **)&c.partials, sizeof(double) * MAX_GRID * NRED, "hipMalloc partials");
+    rc |= ensure(c, (void **)&c.dsc, sizeof(DevScalars) * B, "hipMalloc dsc");
+    rc |= ensure(c, (void **)&c.partials, sizeof(double) * MAX_GRID * NRED * B, "hipMalloc partials");
     if (rc) { lat_destroy(h); return -3; }
     c.twx64 = make_twiddles<double>(c, c.nx, &rc);
     c.twy64 = make_twiddles<double>(c, c.ny, &rc);
@@ -606,16 +643,89 @@ int lat_create(const lat_config *cfg, lat_ctx **out)
     memset(&c.host_sc, 0, sizeof(DevScalars));
     c.host_sc.yscl = 1.0;
     c.host_sc.mixed = c.prec == LAT_MIXED ? 1 : 0;
-    if (push_scalars(c)) { lat_destroy(h); return -3; }
-    (void)hipMemset(c.fld, 0, es * c.nfld * c.nvol);
-    (void)hipMemset(c.fldp, 0, es * c.nfld * c.nvol);
-    (void)hipMemset(c.partials, 0, sizeof(double) * MAX_GRID * NRED);
+    c.host_sc.mp[0] = cfg->model_params[0];
+    c.host_sc.mp[1] = cfg->model_params[1];
+    {
+        std::vector<DevScalars> all(B, c.host_sc);
+        for (int m = 0; m < B && mparams; ++m) {
+            all[m].mp[0] = mparams[4 * m];
+            all[m].mp[1] = mparams[4 * m + 1];
+        }
+        c.host_sc = all[0];
+        if (hipMemcpy(c.dsc, all.data(), sizeof(DevScalars) * B, hipMemcpyHostToDevice) != hipSuccess) {
+            lat_destroy(h);
+            return -3;
+        }
+    }
+    (void)hipMemset(c.fld, 0, es * B * c.ms.f);
+    (void)hipMemset(c.fldp, 0, es * B * c.ms.f);
+    (void)hipMemset(c.partials, 0, sizeof(double) * MAX_GRID * NRED * B);
     if (c.gw) {
         (void)hipMemset(c.gwu, 0, es * 6 * c.nvol);
         (void)hipMemset(c.gwp, 0, es * 6 * c.nvol);
     }
-    c.ke_valid = false;
+    set_ke_valid(c, false);
     *out = h;
+    return 0;
+}
+
+static int ensure_spec(Ctx &c);
+
+extern "C" {
+
+int lat_create(const lat_config *cfg, lat_ctx **out) { return create_ctx(cfg, 1, nullptr, out); }
+
+int lat_create_batch(const lat_config *cfg, int nmembers, const double *model_params, lat_ctx **out)
+{
+    return create_ctx(cfg, nmembers, model_params, out);
+}
+
+int lat_num_members(const lat_ctx *h) { return h ? h->c.B : -1; }
+
+int lat_member(lat_ctx *h, int m, lat_ctx **out)
+{
+    CTX_OR_FAIL(h);
+    if (!out || m < 0 || m >= c.B || c.view) return -1;
+    // the scratch buffers that single-member operations allocate on demand
+    // are allocated here, once, and borrowed by every view
+    int rc = ensure_io(c);
+    if (!rc) rc = ensure_spec(c);
+    if (rc) return rc;
+    const size_t es = elem_size(c);
+    lat_ctx *v = new lat_ctx();
+    Ctx &w = v->c;
+    w.cfg = c.cfg;
+    w.cfg.model_params[0] = 0.0; // filled from the member's scalars below
+    w.B = 1;
+    w.ms = c.ms;
+    w.view = true;
+    w.parent = &c;
+    w.nx = c.nx; w.ny = c.ny; w.nz = c.nz; w.nxh = c.nxh; w.P = c.P;
+    w.nvol = c.nvol; w.nk = c.nk; w.dx = c.dx; w.dk = c.dk; w.prec = c.prec; w.nfld = c.nfld;
+    w.stream = c.stream;
+    w.device = c.device;
+    w.fld = (char *)c.fld + es * c.ms.f * m;
+    w.fldp = (char *)c.fldp + es * c.ms.f * m;
+    w.Fk = c.Fk ? (void *)((char *)c.Fk + 2 * es * c.ms.k * m) : nullptr;
+    w.Fk2 = c.Fk2;
+    w.io64 = c.io64;
+    w.Fk64 = c.prec == LAT_F64 ? w.Fk : c.Fk64;
+    w.gw = false;
+    w.gw_coup = c.gw_coup;
+    w.dsc = c.dsc + m;
+    w.partials = c.partials + (size_t)MAX_GRID * NRED * m;
+    w.spec_part = c.spec_part;
+    w.spec_out = c.spec_out;
+    w.ns = c.ns;
+    w.spec_cap = c.spec_cap;
+    w.twx64 = c.twx64; w.twy64 = c.twy64; w.twz64 = c.twz64;
+    w.twx32 = c.twx32; w.twy32 = c.twy32; w.twz32 = c.twz32;
+    w.tune = c.tune;
+    if ((rc = pull_scalars(w))) { delete v; return rc; }
+    w.cfg.model_params[0] = w.host_sc.mp[0];
+    w.cfg.model_params[1] = w.host_sc.mp[1];
+    w.member = m;
+    *out = v;
     return 0;
 }
 
@@ -623,6 +733,10 @@ void lat_destroy(lat_ctx *h)
 {
     if (!h) return;
     Ctx &c = h->c;
+    if (c.view) { // borrows everything from its batch
+        delete h;
+        return;
+    }
     (void)hipSetDevice(c.device);
     if (c.stream) (void)hipStreamSynchronize(c.stream);
     void *ptrs[] = {c.fld, c.fldp, c.Fk, c.Fk2, c.io64, (c.Fk64 != c.Fk) ? c.Fk64 : nullptr, c.dsc, c.partials,
@@ -631,7 +745,7 @@ void lat_destroy(lat_ctx *h)
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : c.prof.pool) (void)hipEventDestroy(e);
-    if (c.stream) (void)hipStreamDestroy(c.stream);
+    if (c.stream && c.owns_stream) (void)hipStreamDestroy(c.stream);
     delete h;
 }
 
@@ -653,6 +767,7 @@ int lat_num_bins(int nx, int ny, int nz)
 int lat_fft_r2c_3d(lat_ctx *h, const double *f, double *Fk)
 {
     CTX_OR_FAIL(h);
+    ONE_MEMBER(c);
     if (!f || !Fk) return -1;
     int rc = host_r2c(c, f);
     if (rc) return rc;
@@ -663,6 +778,7 @@ int lat_fft_r2c_3d(lat_ctx *h, const double *f, double *Fk)
 int lat_fft_c2r_3d(lat_ctx *h, const double *Fk, double *f)
 {
     CTX_OR_FAIL(h);
+    ONE_MEMBER(c);
     if (!f || !Fk) return -1;
     int rc = ensure_io(c);
     if (rc) return rc;
@@ -679,6 +795,7 @@ int lat_fft_c2r_3d(lat_ctx *h, const double *Fk, double *f)
 int lat_laplacian_3d(lat_ctx *h, int n1, int n2, int n3, double *f, double dk)
 {
     CTX_OR_FAIL(h);
+    ONE_MEMBER(c);
     if (!f) return -1;
     if (n1 != c.nx || n2 != c.ny || n3 != c.nz) {
         c.last_error = "lattice size differs from the context";
@@ -705,6 +822,7 @@ int lat_laplacian_3d(lat_ctx *h, int n1, int n2, int n3, double *f, double dk)
 int lat_gradient_squared_3d(lat_ctx *h, const double *f, double *grad2, double dk)
 {
     CTX_OR_FAIL(h);
+    ONE_MEMBER(c);
     if (!f || !grad2) return -1;
     int rc = host_r2c(c, f);
     if (rc) return rc;
@@ -730,6 +848,7 @@ int lat_gradient_squared_3d(lat_ctx *h, const double *f, double *grad2, double d
 int lat_stencil_laplacian(lat_ctx *h, const double *f, double *lap)
 {
     CTX_OR_FAIL(h);
+    ONE_MEMBER(c);
     if (!f || !lap) return -1;
     if (c.cfg.stencil[4] == 0.0) {
         c.last_error = "stencil coefficients not set (cc = 0)";
@@ -766,6 +885,7 @@ static void finish_spectra(const std::vector<double> &raw, int ns, int nch, doub
 int lat_spectrum_3d(lat_ctx *h, double *S, const double *f, double *Fk)
 {
     CTX_OR_FAIL(h);
+    ONE_MEMBER(c);
     if (!S || !f) return -1;
     int rc = ensure_spec(c);
     if (rc) return rc;
@@ -790,6 +910,7 @@ int lat_num_bins_discrete(int n)
 int lat_spectrum_discrete(lat_ctx *h, const double *f, double *S, int kind)
 {
     CTX_OR_FAIL(h);
+    ONE_MEMBER(c);
     if (!S || !f || (kind != 0 && kind != 1)) return -1;
     if (c.nx != c.ny || c.nx != c.nz) {
         c.last_error = "Discrete-tree spectra need a cubic lattice";
@@ -814,6 +935,7 @@ int lat_spectrum_discrete(lat_ctx *h, const double *f, double *S, int kind)
 int lat_spectrum_discrete_k(lat_ctx *h, const double *Fk, double *S, int kind)
 {
     CTX_OR_FAIL(h);
+    ONE_MEMBER(c);
     if (!S || !Fk || (kind != 0 && kind != 1)) return -1;
     if (c.nx != c.ny || c.nx != c.nz) {
         c.last_error = "Discrete-tree spectra need a cubic lattice";
@@ -841,6 +963,7 @@ int lat_spectrum_discrete_k(lat_ctx *h, const double *Fk, double *S, int kind)
 int lat_crossspec_discrete_k(lat_ctx *h, const double *Fk1, const double *Fk2, double *S1, double *S2)
 {
     CTX_OR_FAIL(h);
+    ONE_MEMBER(c);
     if (!Fk1 || !Fk2 || !S1 || !S2) return -1;
     if (c.nx != c.ny || c.nx != c.nz) {
         c.last_error = "Discrete-tree spectra need a cubic lattice";
@@ -866,6 +989,7 @@ int lat_crossspec_discrete_k(lat_ctx *h, const double *Fk1, const double *Fk2, d
 int lat_crossspec_3d(lat_ctx *h, const double *Fk1, const double *Fk2, double *S_re, double *S_im)
 {
     CTX_OR_FAIL(h);
+    ONE_MEMBER(c);
     if (!Fk1 || !Fk2 || !S_re || !S_im) return -1;
     int rc = ensure_spec(c);
     if (rc) return rc;
@@ -914,7 +1038,7 @@ static int output_spectra_t(Ctx &c, int field, double *out)
         ScalarOps none{};
         if ((rc = launch_scalar(c, none, g))) return rc;
         if ((rc = pull_scalars(c))) return rc;
-        c.ke_valid = true;
+        set_ke_valid(c, true);
         const double nv = (double)c.nvol;
         const double fphi = nv * c.host_sc.mean_phi[field] + c.host_sc.red[RC_M1 + field];
         const double fpi = nv * c.host_sc.mean_pi[field] + c.host_sc.red[RC_S1 + field];
@@ -933,6 +1057,7 @@ extern "C" {
 int lat_output_spectra(lat_ctx *h, int field, double *out)
 {
     CTX_OR_FAIL(h);
+    ONE_MEMBER(c);
     if (!out || field < 0 || field >= c.nfld) return -1;
     int rc = ensure_spec(c);
     if (rc) return rc;
@@ -981,6 +1106,7 @@ extern "C" {
 int lat_set_gw(lat_ctx *h, const double *hij, const double *pi_hij)
 {
     CTX_OR_FAIL(h);
+    ONE_MEMBER(c);
     if (!hij || !pi_hij) return -1;
     if (!c.gw) {
         c.last_error = "context created without gw";
@@ -1004,6 +1130,7 @@ int lat_set_gw(lat_ctx *h, const double *hij, const double *pi_hij)
 int lat_get_gw(lat_ctx *h, double *hij, double *pi_hij)
 {
     CTX_OR_FAIL(h);
+    ONE_MEMBER(c);
     if (!hij || !pi_hij) return -1;
     if (!c.gw) {
         c.last_error = "context created without gw";
@@ -1029,6 +1156,7 @@ int lat_get_gw(lat_ctx *h, double *hij, double *pi_hij)
 int lat_gw_source(lat_ctx *h, double *Sij)
 {
     CTX_OR_FAIL(h);
+    ONE_MEMBER(c);
     if (!Sij) return -1;
     const size_t es = elem_size(c);
     int rc = ensure(c, &c.gwsrc, es * 2 * c.nvol, "hipMalloc gwsrc");
@@ -1047,6 +1175,7 @@ int lat_gw_source(lat_ctx *h, double *Sij)
 int lat_gw_spectrum(lat_ctx *h, double *out)
 {
     CTX_OR_FAIL(h);
+    ONE_MEMBER(c);
     if (!out) return -1;
     if (!c.gw) {
         c.last_error = "context created without gw";
@@ -1060,6 +1189,7 @@ int lat_gw_spectrum(lat_ctx *h, double *out)
 int lat_set_state(lat_ctx *h, const double *fld, const double *fldp, int ghost, double yscl, double ysclp)
 {
     CTX_OR_FAIL(h);
+    ONE_MEMBER(c);
     if (!fld || !fldp || ghost < 0 || ghost > 1) return -1;
     const int g = ghost, nf = c.nfld;
     const size_t sx = c.nx + 2 * g, sy = c.ny + 2 * g;
@@ -1107,13 +1237,14 @@ int lat_set_state(lat_ctx *h, const double *fld, const double *fldp, int ghost, 
         c.host_sc.mean_pi[l] = mpi[l];
         c.host_sc.nu[l] = c.host_sc.mu[l] = c.host_sc.sdphi[l] = 0.0;
     }
-    c.ke_valid = false;
+    set_ke_valid(c, false);
     return push_scalars(c);
 }
 
 int lat_get_state(lat_ctx *h, double *fld, double *fldp, int ghost, double *yscl, double *ysclp)
 {
     CTX_OR_FAIL(h);
+    ONE_MEMBER(c);
     if (ghost < 0 || ghost > 1) return -1;
     int rc = sync(c);
     if (rc) return rc;
@@ -1159,6 +1290,7 @@ int lat_get_state(lat_ctx *h, double *fld, double *fldp, int ghost, double *yscl
 int lat_get_scalars(lat_ctx *h, double *yscl, double *ysclp)
 {
     CTX_OR_FAIL(h);
+    ONE_MEMBER(c);
     int rc = pull_scalars(c);
     if (rc) return rc;
     if (yscl) *yscl = c.host_sc.yscl;
@@ -1169,6 +1301,7 @@ int lat_get_scalars(lat_ctx *h, double *yscl, double *ysclp)
 int lat_set_scalars(lat_ctx *h, double yscl, double ysclp)
 {
     CTX_OR_FAIL(h);
+    ONE_MEMBER(c);
     int rc = pull_scalars(c);
     if (rc) return rc;
     c.host_sc.yscl = yscl;
@@ -1227,6 +1360,7 @@ int lat_symp(lat_ctx *h, int order, double dt, int nsteps)
 int lat_calc_metric(lat_ctx *h)
 {
     CTX_OR_FAIL(h);
+    ONE_MEMBER(c);
     int rc = state_sums(c);
     if (rc) return rc;
     const double nv = (double)c.nvol, y = c.host_sc.yscl;
@@ -1245,6 +1379,7 @@ int lat_calc_metric(lat_ctx *h)
 int lat_energies(lat_ctx *h, double *out)
 {
     CTX_OR_FAIL(h);
+    ONE_MEMBER(c);
     if (!out) return -1;
     int rc = state_sums(c);
     if (rc) return rc;
@@ -1275,12 +1410,179 @@ int lat_energies(lat_ctx *h, double *out)
     return 0;
 }
 
+} // extern "C"
+
+// Fortran ES22.15 edit descriptor as written by '(30(ES22.15,2X))': one
+// digit before the point, 15 after, exponent E+dd (3-digit exponents drop the
+// E), right-justified in 22 columns.
+static std::string fortran_es22(double x)
+{
+    char buf[64];
+    if (std::isnan(x)) {
+        snprintf(buf, sizeof buf, "%22s", "NaN");
+        return buf;
+    }
+    if (std::isinf(x)) {
+        snprintf(buf, sizeof buf, "%22s", x > 0 ? "Infinity" : "-Infinity");
+        return buf;
+    }
+    snprintf(buf, sizeof buf, "%.15E", x);
+    std::string t(buf);
+    const size_t epos = t.find('E');
+    std::string mant = t.substr(0, epos), ex = t.substr(epos + 1); // ex = "+dd" or "+ddd"
+    std::string field = ex.size() > 3 ? mant + ex : mant + "E" + ex;
+    if (field.size() < 22) field = std::string(22 - field.size(), ' ') + field;
+    return field;
+}
+
+static int append_line(Ctx &c, const std::string &path, const std::vector<double> &vals)
+{
+    FILE *f = fopen(path.c_str(), "a");
+    if (!f) {
+        c.last_error = "cannot open " + path;
+        return -5;
+    }
+    std::string line;
+    for (size_t q = 0; q < vals.size(); ++q) {
+        if (q) line += "  ";
+        line += fortran_es22(vals[q]);
+    }
+    line += "\n";
+    fputs(line.c_str(), f);
+    fclose(f);
+    return 0;
+}
+
+extern "C" {
+
+int lat_make_output(lat_ctx *h, double time, int tree, const char *dir)
+{
+    CTX_OR_FAIL(h);
+    ONE_MEMBER(c);
+    if (!dir || (tree != 0 && tree != 1)) return -1;
+    double e[LAT_NE];
+    int rc = lat_energies(h, e);
+    if (rc) return rc;
+    const PhysSums ps = phys_sums(c);
+    const double a = e[LAT_E_A], ysp = c.host_sc.ysclp;
+    // dump_rho: openmp_dev/evolve_spectral.f90:403-405 ends with <phi_2> (an
+    // out-of-bounds read for one field; 0 here), mpi_dev/evolve_spectral.f90:311-313 with <pi_1>
+    std::vector<double> row = {time, a, e[LAT_E_RHO], e[LAT_E_KE], e[LAT_E_PE], e[LAT_E_GE], e[LAT_E_GRAV],
+                               (e[LAT_E_RHO] + e[LAT_E_GRAV]) / e[LAT_E_RHO],
+                               -(ysp * ysp) / 12.0 / std::pow(a, 4), e[LAT_E_MEAN1],
+                               tree == 0 ? e[LAT_E_MEAN2] : ps.mean_pi[0]};
+    const std::string d(dir);
+    if ((rc = append_line(c, d + "/energy_spec.out", row))) return rc;
+    // make_output spectra (evolve_spectral.f90:523-551): S_phi1, S_pi1, Re/Im cross;
+    // spec(ns, 2*nfld+2) has 2*nfld-2 further columns the reference never sets (0 here)
+    std::vector<double> sp(4 * (size_t)c.ns);
+    if ((rc = lat_output_spectra(h, 0, sp.data()))) return rc;
+    const int ncol = 2 * c.nfld + 2;
+    for (int b = 0; b < c.ns; ++b) {
+        std::vector<double> vals(1 + ncol, 0.0);
+        vals[0] = b * c.dk;
+        for (int q = 0; q < 4; ++q) vals[1 + q] = sp[(size_t)q * c.ns + b];
+        if ((rc = append_line(c, d + "/spectrum.out", vals))) return rc;
+    }
+    FILE *f = fopen((d + "/spectrum.out").c_str(), "a");
+    if (!f) return -5;
+    fputs("\n", f); // write(97,*): an empty record
+    fclose(f);
+    return 0;
+}
+
+// Checkpoint files of Discrete/Entropy/driver.f90:68-118 and scafld.f90:224-284:
+// fld_<i>.raw / fldp_<i>.raw hold field i (1-based) as native FP64, x fastest,
+// no ghosts; checkpt.dat holds "yscl ysclp", "time", "outframenum" as
+// list-directed records.
+int lat_checkpoint(lat_ctx *h, const char *dir, double time, int outframenum)
+{
+    CTX_OR_FAIL(h);
+    ONE_MEMBER(c);
+    if (!dir) return -1;
+    const size_t nv = c.nvol, nf = c.nfld;
+    std::vector<double> aos(nf * nv), aosp(nf * nv), raw(nv);
+    double y = 0, yp = 0;
+    int rc = lat_get_state(h, aos.data(), aosp.data(), 0, &y, &yp);
+    if (rc) return rc;
+    const std::string d(dir);
+    for (size_t l = 0; l < nf; ++l)
+        for (int which = 0; which < 2; ++which) {
+            const std::vector<double> &src = which ? aosp : aos;
+            for (size_t q = 0; q < nv; ++q) raw[q] = src[l + nf * q];
+            const std::string path = d + (which ? "/fldp_" : "/fld_") + std::to_string(l + 1) + ".raw";
+            FILE *f = fopen(path.c_str(), "wb");
+            if (!f) {
+                c.last_error = "cannot write " + path;
+                return -5;
+            }
+            const size_t w = fwrite(raw.data(), sizeof(double), nv, f);
+            fclose(f);
+            if (w != nv) {
+                c.last_error = "short write " + path;
+                return -5;
+            }
+        }
+    FILE *f = fopen((d + "/checkpt.dat").c_str(), "w");
+    if (!f) {
+        c.last_error = "cannot write checkpt.dat";
+        return -5;
+    }
+    fprintf(f, " %.17g %.17g\n %.17g\n %d\n", y, yp, time, outframenum);
+    fclose(f);
+    return 0;
+}
+
+int lat_restore(lat_ctx *h, const char *dir, double *time, int *outframenum)
+{
+    CTX_OR_FAIL(h);
+    ONE_MEMBER(c);
+    if (!dir) return -1;
+    const size_t nv = c.nvol, nf = c.nfld;
+    std::vector<double> aos(nf * nv), aosp(nf * nv), raw(nv);
+    const std::string d(dir);
+    for (size_t l = 0; l < nf; ++l)
+        for (int which = 0; which < 2; ++which) {
+            const std::string path = d + (which ? "/fldp_" : "/fld_") + std::to_string(l + 1) + ".raw";
+            FILE *f = fopen(path.c_str(), "rb");
+            if (!f) {
+                c.last_error = "cannot read " + path;
+                return -5;
+            }
+            const size_t r = fread(raw.data(), sizeof(double), nv, f);
+            fclose(f);
+            if (r != nv) {
+                c.last_error = "short read " + path;
+                return -5;
+            }
+            std::vector<double> &dst = which ? aosp : aos;
+            for (size_t q = 0; q < nv; ++q) dst[l + nf * q] = raw[q];
+        }
+    FILE *f = fopen((d + "/checkpt.dat").c_str(), "r");
+    if (!f) {
+        c.last_error = "cannot read checkpt.dat";
+        return -5;
+    }
+    double y = 0, yp = 0, t = 0;
+    int frame = 0;
+    const int got = fscanf(f, "%lf %lf %lf %d", &y, &yp, &t, &frame);
+    fclose(f);
+    if (got != 4) {
+        c.last_error = "malformed checkpt.dat";
+        return -5;
+    }
+    if (time) *time = t;
+    if (outframenum) *outframenum = frame;
+    return lat_set_state(h, aos.data(), aosp.data(), 0, y, yp);
+}
+
 int lat_device_state(lat_ctx *h, void **fld, void **fldp)
 {
     CTX_OR_FAIL(h);
+    ONE_MEMBER(c);
     if (fld) *fld = c.fld;
     if (fldp) *fldp = c.fldp;
-    c.ke_valid = false; // the caller may write the state
+    set_ke_valid(c, false); // the caller may write the state
     return 0;
 }
 
@@ -1288,6 +1590,7 @@ int lat_sample_field(lat_ctx *h, int field, int momentum, double mean, double ga
                      double kcut_fac, unsigned long long seed, unsigned long long stream)
 {
     CTX_OR_FAIL(h);
+    ONE_MEMBER(c);
     if (field < 0 || field >= c.nfld || momentum < 0 || momentum > 1 || !(mpl != 0.0)) return -1;
     int rc = ensure_io(c);
     if (rc) return rc;
@@ -1296,7 +1599,7 @@ int lat_sample_field(lat_ctx *h, int field, int momentum, double mean, double ga
     if (c.prec == LAT_F64) rc = store_io_field<double>(c, (double *)base + (size_t)field * c.nvol, mean);
     else rc = store_io_field<float>(c, (float *)base + (size_t)field * c.nvol, 0.0);
     if (rc) return rc;
-    c.ke_valid = false;
+    set_ke_valid(c, false);
     if (c.prec == LAT_MIXED) {
         // the homogeneous value goes to the FP64 part, the sample is the fluctuation
         if ((rc = pull_scalars(c))) return rc;
@@ -1331,6 +1634,326 @@ int lat_profile_read(lat_ctx *h, double *ms, long long *launches)
         if (launches) launches[s] = c.prof.cnt[s];
     }
     return 0;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------- copy bandwidth
+// Plain device copy with 16-byte accesses per lane: the measured HBM
+// bandwidth quoted beside the vendor peak, and the calibration kernel for
+// the HBM byte counters (read + write of a known size).
+__global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
+}
+
+extern "C" int lat_copy_bandwidth(lat_ctx *h, size_t bytes, int reps, double *gbps)
+{
+    CTX_OR_FAIL(h);
+    if (!gbps || reps < 1 || bytes < 16) return -1;
+    const size_t n = bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    LAT_HIP(&c, hipMalloc(&a, n * 16));
+    if (hipMalloc(&b, n * 16) != hipSuccess) {
+        (void)hipFree(a);
+        c.last_error = "hipMalloc copy buffer";
+        return -3;
+    }
+    hipEvent_t e0, e1;
+    int rc = 0;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = -3;
+    if (!rc) {
+        (void)hipMemsetAsync(a, 1, n * 16, c.stream);
+        hipLaunchKernelGGL(k_copy16, dim3(8192), dim3(256), 0, c.stream, (const uint4 *)a, (uint4 *)b, n);
+        (void)hipEventRecord(e0, c.stream);
+        for (int r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(k_copy16, dim3(8192), dim3(256), 0, c.stream, (const uint4 *)a, (uint4 *)b, n);
+        (void)hipEventRecord(e1, c.stream);
+        if (hipEventSynchronize(e1) != hipSuccess) rc = -3;
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        *gbps = 2.0 * (double)(n * 16) * reps / (ms * 1e-3) / 1e9;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (rc) c.last_error = "copy bandwidth timing failed";
+    return rc;
+}
+
+// ---------------------------------------------------------------- autotuner
+// Times the two launch shapes of each sweep on the context's own arrays (the
+// x R2C without drift and the y/z passes write only the k-space scratch; the
+// x C2R runs its reduce-only epilogue, which writes only the partials) and
+// keeps the faster.  Fields and momenta are not modified.
+template <typename T>
+static int autotune_t(Ctx &c, int reps)
+{
+    if (c.cfg.laplacian != LAT_LAP_SPECTRAL) return 0;
+    const cpx<T> *twx = (const cpx<T> *)(sizeof(T) == 8 ? (void *)c.twx64 : (void *)c.twx32);
+    const cpx<T> *twy = (const cpx<T> *)(sizeof(T) == 8 ? (void *)c.twy64 : (void *)c.twy32);
+    const cpx<T> *twz = (const cpx<T> *)(sizeof(T) == 8 ? (void *)c.twz64 : (void *)c.twz32);
+    cpx<T> *Fk = (cpx<T> *)c.Fk;
+    hipEvent_t e0, e1;
+    LAT_HIP(&c, hipEventCreate(&e0));
+    LAT_HIP(&c, hipEventCreate(&e1));
+    int *knobs[4] = {&c.tune.r2c, &c.tune.coly, &c.tune.colz, &c.tune.c2r};
+    int rc = 0;
+    for (int kind = 0; kind < 4 && !rc; ++kind) {
+        float best = 0.f;
+        int arg = 0;
+        for (int v = 0; v < 2 && !rc; ++v) {
+            *knobs[kind] = v;
+            auto run = [&]() -> int {
+                switch (kind) {
+                case 0: return launch_x_r2c<T>(c, twx, (T *)c.fld, nullptr, Fk, 0, 0);
+                case 1: return launch_col<T>(c, twy, Fk, 1, 0, c.nfld);
+                case 2: return launch_col<T>(c, twz, Fk, 2, 2, c.nfld);
+                default: {
+                    const int g = launch_x_c2r<T>(c, twx, Fk, c.nfld, EPI_REDUCE, nullptr, (const T *)c.fld,
+                                                  (T *)c.fldp);
+                    return g < 0 ? g : 0;
+                }
+                }
+            };
+            if ((rc = run())) break;
+            if (hipEventRecord(e0, c.stream) != hipSuccess) { rc = -3; break; }
+            for (int r = 0; r < reps && !rc; ++r) rc = run();
+            if (rc || hipEventRecord(e1, c.stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+                rc = rc ? rc : -3;
+                break;
+            }
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (v == 0 || ms < best) {
+                best = ms;
+                arg = v;
+            }
+        }
+        *knobs[kind] = arg;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    set_ke_valid(c, false);
+    return rc;
+}
+
+static int autotune(Ctx &c, int reps)
+{
+    return c.prec == LAT_F64 ? autotune_t<double>(c, reps) : autotune_t<float>(c, reps);
+}
+
+extern "C" {
+
+int lat_autotune(lat_ctx *h)
+{
+    CTX_OR_FAIL(h);
+    if (c.view) {
+        c.last_error = "tune the batch, not a member view";
+        return -2;
+    }
+    return autotune(c, 3);
+}
+
+int lat_set_tuning(lat_ctx *h, const int *in)
+{
+    CTX_OR_FAIL(h);
+    if (!in || c.view) return -1;
+    for (int q = 0; q < 4; ++q)
+        if (in[q] != 0 && in[q] != 1) return -1;
+    c.tune.r2c = in[0];
+    c.tune.coly = in[1];
+    c.tune.colz = in[2];
+    c.tune.c2r = in[3];
+    return 0;
+}
+
+int lat_tuning(lat_ctx *h, int *out)
+{
+    CTX_OR_FAIL(h);
+    if (!out) return -1;
+    out[0] = c.tune.r2c;
+    out[1] = c.tune.coly;
+    out[2] = c.tune.colz;
+    out[3] = c.tune.c2r;
+    return 0;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------- ensembles
+// Size classes of batches sharing one stream.  A Yoshida call over all
+// classes is captured once into a hipGraph (kernels only: no memsets, no host
+// synchronisation, profiling off) and replayed for every later call with the
+// same order, step count and time steps.
+struct lat_ensemble {
+    std::vector<lat_ctx *> cls;
+    std::vector<int> first; // index of each class's first member
+    hipStream_t stream = nullptr;
+    int device = 0;
+    bool use_graph = true;
+    struct Entry {
+        int order, nsteps;
+        std::vector<double> dt;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<Entry> graphs;
+    std::string last_error;
+};
+
+static int ens_fail(lat_ensemble *e, hipError_t err, const char *where)
+{
+    e->last_error = std::string(where) + ": " + hipGetErrorString(err);
+    return -3;
+}
+
+// One Yoshida call on every class; each starts from fresh reductions so that
+// the captured work is the same on every replay.
+static int ens_record(lat_ensemble *e, int order, const double *dt, int nsteps)
+{
+    for (size_t k = 0; k < e->cls.size(); ++k) {
+        Ctx &c = e->cls[k]->c;
+        std::vector<Op> ops;
+        if (build_symp(c, ops, order, dt[k], nsteps)) {
+            e->last_error = "symp order must be 2, 4, 6 or 8";
+            return -1;
+        }
+        set_ke_valid(c, false);
+        const bool prof = c.prof.on;
+        c.prof.on = false;
+        const int rc = run_ops(c, ops);
+        c.prof.on = prof;
+        if (rc) {
+            e->last_error = c.last_error;
+            return rc;
+        }
+    }
+    return 0;
+}
+
+extern "C" {
+
+int lat_ensemble_create(const lat_config *base, int nclass, const int *n, const int *count, const double *len,
+                        const double *model_params, lat_ensemble **out)
+{
+    if (!base || !n || !count || !out || nclass < 1) return -1;
+    *out = nullptr;
+    lat_ensemble *e = new lat_ensemble();
+    (void)hipGetDevice(&e->device);
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete e;
+        return -3;
+    }
+    int total = 0;
+    for (int k = 0; k < nclass; ++k) {
+        lat_config cfg = *base;
+        cfg.nx = cfg.ny = cfg.nz = n[k];
+        if (len) cfg.len = len[k];
+        cfg.gw = 0;
+        lat_ctx *h = nullptr;
+        const int rc = create_ctx(&cfg, count[k], model_params ? model_params + 4 * (size_t)total : nullptr, &h);
+        if (rc) {
+            lat_ensemble_destroy(e);
+            return rc;
+        }
+        Ctx &c = h->c;
+        (void)hipStreamSynchronize(c.stream);
+        (void)hipStreamDestroy(c.stream);
+        c.stream = e->stream;
+        c.owns_stream = false;
+        if (autotune(c, 3)) {
+            e->last_error = c.last_error;
+            lat_destroy(h);
+            lat_ensemble_destroy(e);
+            return -3;
+        }
+        e->cls.push_back(h);
+        e->first.push_back(total);
+        total += count[k];
+    }
+    *out = e;
+    return 0;
+}
+
+void lat_ensemble_destroy(lat_ensemble *e)
+{
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    for (auto &g : e->graphs) {
+        (void)hipGraphExecDestroy(g.exec);
+        (void)hipGraphDestroy(g.graph);
+    }
+    for (lat_ctx *h : e->cls) lat_destroy(h);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+const char *lat_ensemble_last_error(const lat_ensemble *e) { return e ? e->last_error.c_str() : "null ensemble"; }
+
+int lat_ensemble_size(const lat_ensemble *e)
+{
+    if (!e || e->cls.empty()) return -1;
+    return e->first.back() + e->cls.back()->c.B;
+}
+
+int lat_ensemble_member(lat_ensemble *e, int index, lat_ctx **view)
+{
+    if (!e || !view) return -1;
+    for (size_t k = 0; k < e->cls.size(); ++k)
+        if (index >= e->first[k] && index < e->first[k] + e->cls[k]->c.B)
+            return lat_member(e->cls[k], index - e->first[k], view);
+    e->last_error = "member index out of range";
+    return -1;
+}
+
+int lat_ensemble_use_graph(lat_ensemble *e, int enable)
+{
+    if (!e) return -1;
+    e->use_graph = enable != 0;
+    return 0;
+}
+
+int lat_ensemble_symp(lat_ensemble *e, int order, const double *dt, int nsteps)
+{
+    if (!e || !dt) return -1;
+    if (hipSetDevice(e->device) != hipSuccess) return -3;
+    if (nsteps <= 0) return 0;
+    if (!e->use_graph) return ens_record(e, order, dt, nsteps);
+    const std::vector<double> key(dt, dt + e->cls.size());
+    for (auto &g : e->graphs)
+        if (g.order == order && g.nsteps == nsteps && g.dt == key) {
+            hipError_t err = hipGraphLaunch(g.exec, e->stream);
+            return err == hipSuccess ? 0 : ens_fail(e, err, "hipGraphLaunch");
+        }
+    hipError_t err = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
+    if (err != hipSuccess) return ens_fail(e, err, "hipStreamBeginCapture");
+    const int rc = ens_record(e, order, dt, nsteps);
+    hipGraph_t graph = nullptr;
+    err = hipStreamEndCapture(e->stream, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (err != hipSuccess) return ens_fail(e, err, "hipStreamEndCapture");
+    hipGraphExec_t exec = nullptr;
+    err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (err != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return ens_fail(e, err, "hipGraphInstantiate");
+    }
+    e->graphs.push_back({order, nsteps, key, graph, exec});
+    err = hipGraphLaunch(exec, e->stream);
+    return err == hipSuccess ? 0 : ens_fail(e, err, "hipGraphLaunch");
+}
+
+int lat_ensemble_synchronize(lat_ensemble *e)
+{
+    if (!e) return -1;
+    hipError_t err = hipStreamSynchronize(e->stream);
+    return err == hipSuccess ? 0 : ens_fail(e, err, "hipStreamSynchronize");
 }
 
 } // extern "C"

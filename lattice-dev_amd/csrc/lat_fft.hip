@@ -93,12 +93,17 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NCH], double *out
 template <typename T, int N, int L, bool DRIFT, int MR = 4, bool TWL = false>
 __global__ __launch_bounds__(TPB) void k_x_r2c(T *__restrict__ f, const T *__restrict__ pi,
                                                cpx<T> *__restrict__ Fk, const cpx<T> *__restrict__ tw,
-                                               int rows, int P, const DevScalars *__restrict__ dsc, int field)
+                                               int rows, int P, const DevScalars *__restrict__ dsc, int field,
+                                               MemberStride ms)
 {
     constexpr int M = N / 2;
     constexpr int LS = M + 1;
     __shared__ cpx<T> buf[L * LS];
     LAT_TWIDDLES_TO_LDS(N, TPB)
+    f += blockIdx.y * ms.f;
+    if (pi) pi += blockIdx.y * ms.f;
+    Fk += blockIdx.y * ms.k;
+    dsc += blockIdx.y;
     int nd = 0;
     T dc0 = 0, dc1 = 0, nu = 0;
     if constexpr (DRIFT) {
@@ -190,9 +195,10 @@ template <typename T, int N, int TC, int MODE, int MR = 16, bool PF = true, int 
           bool TWL = false>
 __global__ __launch_bounds__(NT, MINW) void k_col(cpx<T> *__restrict__ Fk, const cpx<T> *__restrict__ tw, int nxh,
                                              int n_other, size_t s_axis, size_t s_other, size_t s_arr,
-                                             int n_arr, int ny, double kfac)
+                                             int n_arr, int ny, double kfac, size_t ms_k)
 {
     LAT_TWIDDLES_TO_LDS(N, NT)
+    Fk += blockIdx.y * ms_k;
     // Software pipeline per tile of TC adjacent columns:
     //   pass 0 straight from HBM into registers (radix R0, no twiddles),
     //   results to LDS, then the next tile's pass-0 loads are issued into the
@@ -284,13 +290,21 @@ __global__ __launch_bounds__(TPB) void k_x_c2r(const cpx<T> *__restrict__ Fk, co
                                                int rows, int P, size_t nk, size_t nvol, int nf, T *__restrict__ out,
                                                const T *__restrict__ fld, T *__restrict__ fldp,
                                                const DevScalars *__restrict__ dsc, ModelDev md,
-                                               double *__restrict__ partials)
+                                               double *__restrict__ partials, MemberStride ms)
 {
     constexpr int M = N / 2;
     constexpr int LS = M + 2; // even pitch: every row starts 16-byte aligned
     constexpr int MAXF = 2;
     __shared__ cpx<T> buf[MAXF * L * LS];
     LAT_TWIDDLES_TO_LDS(N, TPB)
+    Fk += blockIdx.y * ms.k;
+    if (out) out += blockIdx.y * ms.f;
+    if (fld) fld += blockIdx.y * ms.f;
+    if (fldp) fldp += blockIdx.y * ms.f;
+    dsc += blockIdx.y;
+    partials += (size_t)blockIdx.y * MAX_GRID * NRED;
+    md.p0 = dsc->mp[0];
+    md.p1 = dsc->mp[1];
     double acc[NRED] = {0, 0, 0, 0, 0, 0, 0, 0}; // RedChan order
     T hA = 0, hB = 0;
     if constexpr (EPI == EPI_KICK || EPI == EPI_GWKICK) {
@@ -511,24 +525,47 @@ struct ColCfg {
 
 #define LAT_FOR_SIZES(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(48) X(96) X(192)
 
+// Sizes up to 256 carry a second launch shape per sweep that the autotuner
+// (lat_autotune) may pick: half the rows per workgroup for the x sweeps, half
+// the columns per tile for the y/z sweeps.
+template <int N>
+constexpr bool tunable() { return N <= 256; }
+template <typename T, int N>
+constexpr int col_tc(int mode, int v) { return v ? clampi((mode == 2 ? ColCfg<T, N, 2>::TC : ColCfg<T, N, 0>::TC) / 2, 4, 64)
+                                              : (mode == 2 ? ColCfg<T, N, 2>::TC : ColCfg<T, N, 0>::TC); }
+
+template <typename T, int N, int L, int MR>
+static void r2c_go(Ctx &c, const cpx<T> *tw, T *f, const T *pi, cpx<T> *Fk, int drift, int field)
+{
+    const int rows = c.ny * c.nz;
+    const int g = grid_for((rows + L - 1) / L);
+    if (drift)
+        hipLaunchKernelGGL((k_x_r2c<T, N, L, true, MR>), dim3(g, c.B), dim3(TPB), 0, c.stream, f, pi, Fk, tw, rows,
+                           c.P, c.dsc, field, c.ms);
+    else
+        hipLaunchKernelGGL((k_x_r2c<T, N, L, false, MR>), dim3(g, c.B), dim3(TPB), 0, c.stream, f, pi, Fk, tw, rows,
+                           c.P, c.dsc, field, c.ms);
+}
+
+template <typename T, int N>
+static void r2c_size(Ctx &c, const cpx<T> *tw, T *f, const T *pi, cpx<T> *Fk, int drift, int field)
+{
+    constexpr int L = r2c_lines<N>();
+    constexpr int MR = r2c_maxradix<T>();
+    if constexpr (tunable<N>() && L > 1) {
+        if (c.tune.r2c == 1) return r2c_go<T, N, L / 2, MR>(c, tw, f, pi, Fk, drift, field);
+    }
+    r2c_go<T, N, L, MR>(c, tw, f, pi, Fk, drift, field);
+}
+
 template <typename T>
 int launch_x_r2c(Ctx &c, const cpx<T> *tw, T *f, const T *pi, cpx<T> *Fk, int drift, int field)
 {
-    const int rows = c.ny * c.nz;
     switch (c.nx) {
-#define CASE(NN)                                                                                         \
-    case NN: {                                                                                           \
-        constexpr int L = r2c_lines<NN>();                                                               \
-        constexpr int MR = r2c_maxradix<T>();                                                            \
-        const int g = grid_for((rows + L - 1) / L);                                                      \
-        if (drift)                                                                                       \
-            hipLaunchKernelGGL((k_x_r2c<T, NN, L, true, MR>), dim3(g), dim3(TPB), 0, c.stream, f, pi, Fk, \
-                               tw, rows, c.P, c.dsc, field);                                             \
-        else                                                                                             \
-            hipLaunchKernelGGL((k_x_r2c<T, NN, L, false, MR>), dim3(g), dim3(TPB), 0, c.stream, f, pi, Fk, \
-                               tw, rows, c.P, c.dsc, field);                                             \
-        break;                                                                                           \
-    }
+#define CASE(NN)                                          \
+    case NN:                                              \
+        r2c_size<T, NN>(c, tw, f, pi, Fk, drift, field);  \
+        break;
         LAT_FOR_SIZES(CASE)
 #undef CASE
     default:
@@ -537,6 +574,27 @@ int launch_x_r2c(Ctx &c, const cpx<T> *tw, T *f, const T *pi, cpx<T> *Fk, int dr
     }
     LAT_HIP(&c, hipGetLastError());
     return 0;
+}
+
+template <typename T, int N, int TC, int MODE>
+static void col_go(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int n_other, size_t s_axis, size_t s_other, int n_arrays,
+                   double kfac)
+{
+    using C = ColCfg<T, N, MODE == 2 ? 2 : 0>;
+    const int g = grid_for(((c.nxh + TC - 1) / TC) * n_other * n_arrays);
+    hipLaunchKernelGGL((k_col<T, N, TC, MODE, C::MR, C::PF, 1, COL_NT>), dim3(g, c.B), dim3(COL_NT), 0, c.stream, Fk,
+                       tw, c.nxh, n_other, s_axis, s_other, c.nk, n_arrays, c.ny, kfac, c.ms.k);
+}
+
+template <typename T, int N, int MODE>
+static void col_mode(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int n_other, size_t s_axis, size_t s_other,
+                     int n_arrays, double kfac, int v)
+{
+    constexpr int TC0 = col_tc<T, N>(MODE, 0), TC1 = col_tc<T, N>(MODE, 1);
+    if constexpr (tunable<N>() && TC1 != TC0) {
+        if (v == 1) return col_go<T, N, TC1, MODE>(c, tw, Fk, n_other, s_axis, s_other, n_arrays, kfac);
+    }
+    col_go<T, N, TC0, MODE>(c, tw, Fk, n_other, s_axis, s_other, n_arrays, kfac);
 }
 
 template <typename T>
@@ -547,24 +605,14 @@ int launch_col(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int axis, int mode, int n_a
     const size_t s_axis = axis == 1 ? (size_t)c.P : (size_t)c.P * c.ny;
     const size_t s_other = axis == 1 ? (size_t)c.P * c.ny : (size_t)c.P;
     const double kfac = c.dk * c.dk / (double)c.nvol;
+    const int v = axis == 1 ? c.tune.coly : c.tune.colz;
     switch (n) {
-#define CASE(NN)                                                                                          \
-    case NN: {                                                                                            \
-        using CY = ColCfg<T, NN, 0>;                                                                      \
-        using CZ = ColCfg<T, NN, 2>;                                                                      \
-        const int tc = mode == 2 ? CZ::TC : CY::TC;                                                       \
-        const int g = grid_for(((c.nxh + tc - 1) / tc) * n_other * n_arrays);                              \
-        if (mode == 0)                                                                                    \
-            hipLaunchKernelGGL((k_col<T, NN, CY::TC, 0, CY::MR, CY::PF, 1, COL_NT>), dim3(g), dim3(COL_NT), 0, \
-                               c.stream, Fk, tw, c.nxh, n_other, s_axis, s_other, c.nk, n_arrays, c.ny, kfac);   \
-        else if (mode == 1)                                                                               \
-            hipLaunchKernelGGL((k_col<T, NN, CY::TC, 1, CY::MR, CY::PF, 1, COL_NT>), dim3(g), dim3(COL_NT), 0, \
-                               c.stream, Fk, tw, c.nxh, n_other, s_axis, s_other, c.nk, n_arrays, c.ny, kfac);   \
-        else                                                                                              \
-            hipLaunchKernelGGL((k_col<T, NN, CZ::TC, 2, CZ::MR, CZ::PF, 1, COL_NT>), dim3(g), dim3(COL_NT), 0, \
-                               c.stream, Fk, tw, c.nxh, n_other, s_axis, s_other, c.nk, n_arrays, c.ny, kfac);   \
-        break;                                                                                            \
-    }
+#define CASE(NN)                                                                                  \
+    case NN:                                                                                      \
+        if (mode == 0) col_mode<T, NN, 0>(c, tw, Fk, n_other, s_axis, s_other, n_arrays, kfac, v); \
+        else if (mode == 1) col_mode<T, NN, 1>(c, tw, Fk, n_other, s_axis, s_other, n_arrays, kfac, v); \
+        else col_mode<T, NN, 2>(c, tw, Fk, n_other, s_axis, s_other, n_arrays, kfac, v);          \
+        break;
         LAT_FOR_SIZES(CASE)
 #undef CASE
     default:
@@ -575,41 +623,60 @@ int launch_col(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int axis, int mode, int n_a
     return 0;
 }
 
+template <typename T, int N, int L>
+static int c2r_go(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int nf, int epi, T *out, const T *fld, T *fldp,
+                  const ModelDev &md)
+{
+    const int rows = c.ny * c.nz;
+    const int g = grid_for((rows + L - 1) / L);
+    if (epi == EPI_STORE)
+        hipLaunchKernelGGL((k_x_c2r<T, N, L, EPI_STORE>), dim3(g, c.B), dim3(TPB), 0, c.stream, Fk, tw, rows, c.P,
+                           c.nk, c.nvol, nf, out, fld, fldp, c.dsc, md, c.partials, c.ms);
+    else if (epi == EPI_KICK)
+        hipLaunchKernelGGL((k_x_c2r<T, N, L, EPI_KICK>), dim3(g, c.B), dim3(TPB), 0, c.stream, Fk, tw, rows, c.P,
+                           c.nk, c.nvol, nf, out, fld, fldp, c.dsc, md, c.partials, c.ms);
+    else if (epi == EPI_GWKICK)
+        hipLaunchKernelGGL((k_x_c2r<T, N, L, EPI_GWKICK>), dim3(g, c.B), dim3(TPB), 0, c.stream, Fk, tw, rows, c.P,
+                           c.nk, c.nvol, nf, out, fld, fldp, c.dsc, md, c.partials, c.ms);
+    else
+        hipLaunchKernelGGL((k_x_c2r<T, N, L, EPI_REDUCE>), dim3(g, c.B), dim3(TPB), 0, c.stream, Fk, tw, rows, c.P,
+                           c.nk, c.nvol, nf, out, fld, fldp, c.dsc, md, c.partials, c.ms);
+    return g;
+}
+
+template <typename T, int N>
+static int c2r_size(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int nf, int epi, T *out, const T *fld, T *fldp,
+                    const ModelDev &md)
+{
+    constexpr int L = c2r_lines<N>();
+    if constexpr (tunable<N>() && L > 1) {
+        if (c.tune.c2r == 1) return c2r_go<T, N, L / 2>(c, tw, Fk, nf, epi, out, fld, fldp, md);
+    }
+    return c2r_go<T, N, L>(c, tw, Fk, nf, epi, out, fld, fldp, md);
+}
+
 template <typename T>
 int launch_x_c2r(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int nf, int epi, T *out, const T *fld, T *fldp)
 {
-    const int rows = c.ny * c.nz;
     ModelDev md{c.cfg.model, c.nfld, c.cfg.model_params[0], c.cfg.model_params[1], c.gw_coup};
     if (nf < 1 || nf > 2) {
         c.last_error = "nf must be 1 or 2";
         return -1;
     }
+    int g;
     switch (c.nx) {
-#define CASE(NN)                                                                                         \
-    case NN: {                                                                                           \
-        constexpr int L = c2r_lines<NN>();                                                               \
-        const int g = grid_for((rows + L - 1) / L);                                                      \
-        if (epi == EPI_STORE)                                                                            \
-            hipLaunchKernelGGL((k_x_c2r<T, NN, L, EPI_STORE>), dim3(g), dim3(TPB), 0, c.stream, Fk, tw, rows, \
-                               c.P, c.nk, c.nvol, nf, out, fld, fldp, c.dsc, md, c.partials);           \
-        else if (epi == EPI_KICK)                                                                        \
-            hipLaunchKernelGGL((k_x_c2r<T, NN, L, EPI_KICK>), dim3(g), dim3(TPB), 0, c.stream, Fk, tw, rows, \
-                               c.P, c.nk, c.nvol, nf, out, fld, fldp, c.dsc, md, c.partials);           \
-        else if (epi == EPI_GWKICK)                                                                      \
-            hipLaunchKernelGGL((k_x_c2r<T, NN, L, EPI_GWKICK>), dim3(g), dim3(TPB), 0, c.stream, Fk, tw,    \
-                               rows, c.P, c.nk, c.nvol, nf, out, fld, fldp, c.dsc, md, c.partials);     \
-        else                                                                                             \
-            hipLaunchKernelGGL((k_x_c2r<T, NN, L, EPI_REDUCE>), dim3(g), dim3(TPB), 0, c.stream, Fk, tw, rows, \
-                               c.P, c.nk, c.nvol, nf, out, fld, fldp, c.dsc, md, c.partials);           \
-        LAT_HIP(&c, hipGetLastError());                                                                  \
-        return g;                                                                                        \
-    }
+#define CASE(NN)                                                       \
+    case NN:                                                           \
+        g = c2r_size<T, NN>(c, tw, Fk, nf, epi, out, fld, fldp, md);   \
+        break;
         LAT_FOR_SIZES(CASE)
 #undef CASE
     default:
         c.last_error = "unsupported nx";
         return -2;
     }
+    LAT_HIP(&c, hipGetLastError());
+    return g;
 }
 
 template int launch_x_r2c<double>(Ctx &, const cpx<double> *, double *, const double *, cpx<double> *, int, int);

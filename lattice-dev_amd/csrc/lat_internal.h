@@ -46,6 +46,15 @@ struct DevScalars {
     double nu[2];             // re-centering: the next drift sweep subtracts nu from dphi
     double mu[2];             // re-centering: the next kick subtracts mu from dpi
     double sdphi[2];          // sum of dphi after the pending drifts (before nu)
+    double mp[2];             // model parameters of this member (lat_config.model_params)
+};
+
+// Member strides of a batch context (B realisations of one lattice size,
+// stored member-major): kernels launched with gridDim.y = B offset their
+// arrays by blockIdx.y times these element counts.
+struct MemberStride {
+    size_t f;  // fld / fldp elements per member (nfld * nvol)
+    size_t k;  // Fk complex elements per member
 };
 
 enum ScalarOpType {
@@ -83,6 +92,16 @@ struct Prof {
 struct Ctx {
     lat_config cfg;
     Prof prof;
+    int B = 1;              // members (realisations) of a batch context
+    MemberStride ms{0, 0};
+    bool view = false;      // a member view of a batch: borrows every buffer
+    Ctx *parent = nullptr;  // the batch a view belongs to
+    int member = 0;         // index of a view's member in its batch
+    bool owns_stream = true; // false when an ensemble shares its stream
+    struct Tune {            // launch-shape variants picked by lat_autotune
+        int r2c = 0, coly = 0, colz = 0, c2r = 0;
+    } tune;
+    std::vector<char> mvalid; // batch: per-member ke_valid
     int nx, ny, nz, nxh, P; // P = pitch (complex elements) of a half-complex row
     size_t nvol, nk;        // nk = P*ny*nz
     double dx, dk;

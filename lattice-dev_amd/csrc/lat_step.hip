@@ -56,8 +56,11 @@ __device__ __forceinline__ double site_v(const ModelDev &m, const double *pb64, 
 // recenter = 0: tensor components u_ij, never re-centred.
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_drift(T *__restrict__ f, const T *__restrict__ p, size_t nvol, int nf,
-                                               const DevScalars *__restrict__ dsc, int recenter)
+                                               const DevScalars *__restrict__ dsc, int recenter, size_t ms)
 {
+    f += blockIdx.y * ms;
+    p += blockIdx.y * ms;
+    dsc += blockIdx.y;
     const int nd = dsc->nd;
     T dc[4];
 #pragma unroll
@@ -84,7 +87,8 @@ int launch_drift(Ctx &c)
         if (g > 8192) g = 8192;
         T *f = (T *)(part == 0 ? c.fld : c.gwu);
         const T *p = (const T *)(part == 0 ? c.fldp : c.gwp);
-        hipLaunchKernelGGL((k_drift<T>), dim3(g), dim3(TPB), 0, c.stream, f, p, c.nvol, nf, c.dsc, part == 0 ? 1 : 0);
+        hipLaunchKernelGGL((k_drift<T>), dim3(g, c.B), dim3(TPB), 0, c.stream, f, p, c.nvol, nf, c.dsc,
+                           part == 0 ? 1 : 0, part == 0 ? c.ms.f : (size_t)6 * c.nvol);
     }
     prof_end(c, pi);
     LAT_HIP(&c, hipGetLastError());
@@ -159,8 +163,14 @@ int launch_gw_source(Ctx &c, int c0, int nc, T *dst)
 template <typename T>
 __global__ __launch_bounds__(TPB) void k_reduce_state(const T *__restrict__ f, const T *__restrict__ p, size_t nvol,
                                                       int nf, ModelDev md, const DevScalars *__restrict__ dsc,
-                                                      double *__restrict__ partials)
+                                                      double *__restrict__ partials, size_t ms)
 {
+    f += blockIdx.y * ms;
+    p += blockIdx.y * ms;
+    dsc += blockIdx.y;
+    partials += (size_t)blockIdx.y * MAX_GRID * NRED;
+    md.p0 = dsc->mp[0];
+    md.p1 = dsc->mp[1];
     double pb64[2] = {0, 0};
     if constexpr (sizeof(T) == 4) {
         pb64[0] = dsc->mean_phi[0];
@@ -189,8 +199,8 @@ int launch_reduce_state(Ctx &c)
     const size_t blocks = (c.nvol + TPB - 1) / TPB;
     int g = grid_for(blocks > (size_t)MAX_GRID ? MAX_GRID : (int)blocks);
     const int pi = prof_begin(c, LAT_ST_REDUCE);
-    hipLaunchKernelGGL((k_reduce_state<T>), dim3(g), dim3(TPB), 0, c.stream, (const T *)c.fld, (const T *)c.fldp,
-                       c.nvol, c.nfld, md, c.dsc, c.partials);
+    hipLaunchKernelGGL((k_reduce_state<T>), dim3(g, c.B), dim3(TPB), 0, c.stream, (const T *)c.fld,
+                       (const T *)c.fldp, c.nvol, c.nfld, md, c.dsc, c.partials, c.ms.f);
     prof_end(c, pi);
     LAT_HIP(&c, hipGetLastError());
     return g;
@@ -236,8 +246,15 @@ __device__ __forceinline__ T stencil_sum(const T *__restrict__ f, const int xs[3
 template <typename T, int EPI>
 __global__ __launch_bounds__(TPB) void k_stencil(const T *__restrict__ fld, T *__restrict__ fldp, T *__restrict__ out,
                                                  int nx, int ny, int nz, int nf, StencilC sc, ModelDev md,
-                                                 const DevScalars *__restrict__ dsc, double *__restrict__ partials)
+                                                 const DevScalars *__restrict__ dsc, double *__restrict__ partials,
+                                                 size_t ms)
 {
+    fld += blockIdx.y * ms;
+    fldp += blockIdx.y * ms;
+    dsc += blockIdx.y;
+    partials += (size_t)blockIdx.y * MAX_GRID * NRED;
+    md.p0 = dsc->mp[0];
+    md.p1 = dsc->mp[1];
     const size_t nvol = (size_t)nx * ny * nz;
     const T c0 = T(sc.c0), c1 = T(sc.c1), c2 = T(sc.c2), c3 = T(sc.c3), el = T(sc.elap);
     const bool has2 = sc.c2 != 0.0, has3 = sc.c3 != 0.0;
@@ -306,14 +323,14 @@ int launch_stencil(Ctx &c, int epi)
     int g = grid_for(blocks > (size_t)MAX_GRID ? MAX_GRID : (int)blocks);
     T *fld = (T *)c.fld, *fldp = (T *)c.fldp;
     if (epi == EPI_KICK)
-        hipLaunchKernelGGL((k_stencil<T, EPI_KICK>), dim3(g), dim3(TPB), 0, c.stream, fld, fldp, (T *)nullptr, c.nx,
-                           c.ny, c.nz, c.nfld, sc, md, c.dsc, c.partials);
+        hipLaunchKernelGGL((k_stencil<T, EPI_KICK>), dim3(g, c.B), dim3(TPB), 0, c.stream, fld, fldp, (T *)nullptr,
+                           c.nx, c.ny, c.nz, c.nfld, sc, md, c.dsc, c.partials, c.ms.f);
     else if (epi == EPI_REDUCE)
-        hipLaunchKernelGGL((k_stencil<T, EPI_REDUCE>), dim3(g), dim3(TPB), 0, c.stream, fld, fldp, (T *)nullptr,
-                           c.nx, c.ny, c.nz, c.nfld, sc, md, c.dsc, c.partials);
+        hipLaunchKernelGGL((k_stencil<T, EPI_REDUCE>), dim3(g, c.B), dim3(TPB), 0, c.stream, fld, fldp,
+                           (T *)nullptr, c.nx, c.ny, c.nz, c.nfld, sc, md, c.dsc, c.partials, c.ms.f);
     else
         hipLaunchKernelGGL((k_stencil<T, EPI_STORE>), dim3(g), dim3(TPB), 0, c.stream, fld, fldp, (T *)c.io64, c.nx,
-                           c.ny, c.nz, 1, sc, md, c.dsc, c.partials);
+                           c.ny, c.nz, 1, sc, md, c.dsc, c.partials, (size_t)0);
     LAT_HIP(&c, hipGetLastError());
     return g;
 }
@@ -413,6 +430,11 @@ struct ScalarEnv {
 __global__ __launch_bounds__(TPB) void k_scalar(DevScalars *__restrict__ sc, const double *__restrict__ partials,
                                                 int npart, ScalarOps ops, ScalarEnv env)
 {
+    // one workgroup per member of a batch
+    sc += blockIdx.x;
+    partials += (size_t)blockIdx.x * MAX_GRID * NRED;
+    env.md.p0 = sc->mp[0];
+    env.md.p1 = sc->mp[1];
     __shared__ double sh[NRED][32];
     const int ch = threadIdx.x >> 5, lane = threadIdx.x & 31; // 8 channels x 32 lanes
     if (npart > 0) {
@@ -548,7 +570,7 @@ int launch_scalar(Ctx &c, const ScalarOps &ops, int npart)
     ScalarEnv env{c.cfg.time_var, (double)c.nvol,
                   ModelDev{c.cfg.model, c.nfld, c.cfg.model_params[0], c.cfg.model_params[1]}};
     const int pi = prof_begin(c, LAT_ST_SCALAR);
-    hipLaunchKernelGGL(k_scalar, dim3(1), dim3(TPB), 0, c.stream, c.dsc, c.partials, npart, ops, env);
+    hipLaunchKernelGGL(k_scalar, dim3(c.B), dim3(TPB), 0, c.stream, c.dsc, c.partials, npart, ops, env);
     prof_end(c, pi);
     LAT_HIP(&c, hipGetLastError());
     return 0;

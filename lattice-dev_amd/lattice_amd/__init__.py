@@ -61,6 +61,11 @@ EXPORTS = [
     "lat_num_bins_discrete", "lat_spectrum_discrete",
     "lat_set_gw", "lat_get_gw", "lat_gw_source", "lat_gw_spectrum",
     "lat_spectrum_discrete_k", "lat_crossspec_discrete_k",
+    "lat_create_batch", "lat_num_members", "lat_member",
+    "lat_ensemble_create", "lat_ensemble_destroy", "lat_ensemble_last_error", "lat_ensemble_size",
+    "lat_ensemble_member", "lat_ensemble_use_graph", "lat_ensemble_symp", "lat_ensemble_synchronize",
+    "lat_autotune", "lat_tuning", "lat_set_tuning", "lat_copy_bandwidth", "lat_make_output",
+    "lat_checkpoint", "lat_restore",
 ]
 STAGES = ["x_r2c", "y_fwd", "z_lap", "y_inv", "x_c2r", "scalar", "drift", "stencil", "reduce", "gw_src"]
 
@@ -78,8 +83,11 @@ def lib():
         L.lat_destroy.argtypes = [ctypes.c_void_p]
         for name in EXPORTS:
             fn = getattr(L, name)
-            if name not in ("lat_last_error", "lat_destroy", "lat_create"):
+            if name not in ("lat_last_error", "lat_destroy", "lat_create", "lat_ensemble_destroy",
+                            "lat_ensemble_last_error"):
                 fn.restype = ctypes.c_int
+        L.lat_ensemble_last_error.restype = ctypes.c_char_p
+        L.lat_ensemble_destroy.argtypes = [ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -101,7 +109,7 @@ class Lattice:
 
     def __init__(self, n, nfld=2, length=10.0, precision=F64, laplacian=LAP_SPECTRAL,
                  stencil=STENCIL_7PT, time_var=COSMIC, model=MODEL_TWOFIELD, model_params=(1.0, 2.5e5),
-                 symp6_tree=1, gw=False, gw_mpl=1.0):
+                 symp6_tree=1, gw=False, gw_mpl=1.0, members=None):
         if isinstance(n, int):
             n = (n, n, n)
         self.nx, self.ny, self.nz = n
@@ -121,11 +129,56 @@ class Lattice:
         cfg.gw, cfg.gw_mpl = int(bool(gw)), gw_mpl
         self.cfg = cfg
         h = ctypes.c_void_p()
-        rc = lib().lat_create(ctypes.byref(cfg), ctypes.byref(h))
+        if members is None:
+            rc = lib().lat_create(ctypes.byref(cfg), ctypes.byref(h))
+        else:
+            # a batch: `members` = list of per-member model_params tuples
+            mp = np.zeros((len(members), 4))
+            for m, par in enumerate(members):
+                mp[m, :len(par)] = par
+            self._mp = mp
+            rc = lib().lat_create_batch(ctypes.byref(cfg), len(members), _p(mp), ctypes.byref(h))
         if rc != 0:
             raise LatticeError(f"lat_create failed with status {rc}")
         self._h = h
+        self._parent = None
         self.ns = num_bins(self.nx, self.ny, self.nz)
+
+    def autotune(self):
+        """Pick the faster launch shape of each sweep (sizes up to 256)."""
+        self._check(lib().lat_autotune(self._h), "lat_autotune")
+        return self.tuning()
+
+    def copy_bandwidth(self, nbytes=2 << 30, reps=5):
+        """Measured device-copy bandwidth in GB/s (read + write)."""
+        out = ctypes.c_double()
+        self._check(lib().lat_copy_bandwidth(self._h, ctypes.c_size_t(nbytes), reps, ctypes.byref(out)),
+                    "lat_copy_bandwidth")
+        return out.value
+
+    def set_tuning(self, x_r2c=0, y=0, z=0, x_c2r=0):
+        vals = (ctypes.c_int * 4)(x_r2c, y, z, x_c2r)
+        self._check(lib().lat_set_tuning(self._h, vals), "lat_set_tuning")
+
+    def tuning(self):
+        out = (ctypes.c_int * 4)()
+        self._check(lib().lat_tuning(self._h, out), "lat_tuning")
+        return dict(zip(("x_r2c", "y", "z", "x_c2r"), list(out)))
+
+    @property
+    def members(self):
+        return lib().lat_num_members(self._h)
+
+    def member(self, m):
+        """View of member m of a batch: the per-member API (state, IC,
+        energies, spectra) on that realisation.  Keeps the batch alive."""
+        h = ctypes.c_void_p()
+        self._check(lib().lat_member(self._h, m, ctypes.byref(h)), "lat_member")
+        v = object.__new__(Lattice)
+        v.__dict__.update({k: val for k, val in self.__dict__.items() if k not in ("_h", "_parent", "_mp")})
+        v._h = h
+        v._parent = self
+        return v
 
     # ---------------------------------------------------------------- plumbing
     def close(self):
@@ -291,6 +344,22 @@ class Lattice:
         self._check(lib().lat_gw_spectrum(self._h, _p(out)), "gw_spectrum")
         return out.reshape(2, self.ns)
 
+    def make_output(self, time, directory=".", tree=0):
+        """make_output/dump_rho: append to energy_spec.out and spectrum.out."""
+        self._check(lib().lat_make_output(self._h, ctypes.c_double(time), int(tree),
+                                          os.fsencode(str(directory))), "make_output")
+
+    def checkpoint(self, directory, time, outframenum):
+        self._check(lib().lat_checkpoint(self._h, os.fsencode(str(directory)), ctypes.c_double(time),
+                                         int(outframenum)), "checkpoint")
+
+    def restore(self, directory):
+        """Returns (time, outframenum) of the checkpoint."""
+        t, fr = ctypes.c_double(), ctypes.c_int()
+        self._check(lib().lat_restore(self._h, os.fsencode(str(directory)), ctypes.byref(t), ctypes.byref(fr)),
+                    "restore")
+        return t.value, fr.value
+
     # ---------------------------------------------------------------- state
     def set_state(self, fld, fldp, yscl=1.0, ysclp=0.0):
         """fld, fldp: (nfld, nz, ny, nx) SoA arrays; converted to the reference AoS."""
@@ -386,3 +455,97 @@ class Lattice:
         cnt = (ctypes.c_longlong * len(STAGES))()
         self._check(lib().lat_profile_read(self._h, ms, cnt), "profile_read")
         return {s: (ms[i], cnt[i]) for i, s in enumerate(STAGES)}
+
+
+class Ensemble:
+    """Realisations of uneven lattice sizes advanced together (one batched
+    launch per sweep per size class, Yoshida calls replayed from a hipGraph).
+    classes: list of (n, count, length); members: per-member model_params in
+    class-major order (len = total count), or None for `model_params`."""
+
+    def __init__(self, classes, nfld=2, precision=F64, laplacian=LAP_SPECTRAL, stencil=STENCIL_7PT,
+                 time_var=COSMIC, model=MODEL_TWOFIELD, model_params=(1.0, 2.5e5), members=None):
+        cfg = Config()
+        cfg.nx = cfg.ny = cfg.nz = classes[0][0]
+        cfg.nfld, cfg.len, cfg.precision, cfg.laplacian = nfld, classes[0][2], precision, laplacian
+        for i, v in enumerate(stencil):
+            cfg.stencil[i] = v
+        cfg.time_var, cfg.model = time_var, model
+        for i, v in enumerate(model_params):
+            cfg.model_params[i] = v
+        cfg.symp6_tree = 1
+        self.classes = [tuple(c) for c in classes]
+        n = (ctypes.c_int * len(classes))(*[c[0] for c in classes])
+        cnt = (ctypes.c_int * len(classes))(*[c[1] for c in classes])
+        ln = np.array([c[2] for c in classes], np.float64)
+        mp = None
+        if members is not None:
+            mp = np.zeros((len(members), 4))
+            for m, par in enumerate(members):
+                mp[m, :len(par)] = par
+        self._mp = mp
+        h = ctypes.c_void_p()
+        rc = lib().lat_ensemble_create(ctypes.byref(cfg), len(classes), n, cnt, _p(ln),
+                                       _p(mp) if mp is not None else None, ctypes.byref(h))
+        if rc != 0:
+            raise LatticeError(f"lat_ensemble_create failed with status {rc}")
+        self._h = h
+        self._tmpl = dict(nfld=nfld)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lat_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = lib().lat_ensemble_last_error(self._h)
+            raise LatticeError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    @property
+    def size(self):
+        return lib().lat_ensemble_size(self._h)
+
+    def member(self, index):
+        """Lattice view of member `index` (class-major)."""
+        h = ctypes.c_void_p()
+        self._check(lib().lat_ensemble_member(self._h, index, ctypes.byref(h)), "lat_ensemble_member")
+        k = 0
+        first = 0
+        while index >= first + self.classes[k][1]:
+            first += self.classes[k][1]
+            k += 1
+        nn, _, length = self.classes[k]
+        v = object.__new__(Lattice)
+        v.nx = v.ny = v.nz = nn
+        v.nfld = self._tmpl["nfld"]
+        v.length = length
+        v.dx = length / nn
+        v.dk = 2.0 * np.pi / length
+        v.ns = num_bins(nn, nn, nn)
+        v._h = h
+        v._parent = self
+        return v
+
+    def use_graph(self, enable):
+        self._check(lib().lat_ensemble_use_graph(self._h, int(bool(enable))), "use_graph")
+
+    def symp(self, order, dt, nsteps):
+        """dt: one time step per class."""
+        dt = np.ascontiguousarray(dt, np.float64)
+        self._check(lib().lat_ensemble_symp(self._h, order, _p(dt), nsteps), "lat_ensemble_symp")
+
+    def synchronize(self):
+        self._check(lib().lat_ensemble_synchronize(self._h), "lat_ensemble_synchronize")

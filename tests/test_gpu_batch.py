@@ -1,0 +1,173 @@
+"""GPU batches of realisations (ensembles / coupling scans): B members of one
+lattice size advanced by batched launches must reproduce, member by member,
+single-context runs with the same initial conditions and couplings, and the
+oracle."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+PHI0, DPHI0 = 1.0093430384226378929425913902459, -0.7137133070120812430962278466136
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def member_ic(n, L, seed, mpl=2e5):
+    fld = np.empty((2, n, n, n))
+    fldp = np.empty_like(fld)
+    fld[0] = PHI0 + O.sample(n, L, -0.25, 1.0, mpl, seed, 0)
+    fldp[0] = DPHI0 + O.sample(n, L, 0.25, 1.0, mpl, seed, 1)
+    fld[1] = O.sample(n, L, -0.25, 1.0, mpl, seed, 2)
+    fldp[1] = O.sample(n, L, 0.25, 1.0, mpl, seed, 3)
+    return fld, fldp
+
+
+@pytest.mark.parametrize("lap", ["spectral", "fd7"])
+@pytest.mark.parametrize("precision", ["f64", "mixed"])
+def test_batch_members_match_single_runs(gpu, lap, precision):
+    print("case batch", lap, precision, flush=True)
+    n, L, dt, nsteps = 16, 10.0, 1e-3, 4
+    g2s = [1e4, 1e5, 2.5e5]
+    kw = dict(nfld=2, length=L, time_var=gpu.COSMIC, model=gpu.MODEL_TWOFIELD,
+              laplacian=gpu.LAP_SPECTRAL if lap == "spectral" else gpu.LAP_STENCIL,
+              precision=gpu.F64 if precision == "f64" else gpu.MIXED)
+    ics = [member_ic(n, L, 100 + m) for m in range(len(g2s))]
+    singles = []
+    for m, g2 in enumerate(g2s):
+        with gpu.Lattice(n, model_params=(1.0, g2), **kw) as lat:
+            lat.set_state(*ics[m], 1.0, 0.0)
+            lat.calc_metric()
+            lat.symp4(dt, nsteps)
+            singles.append(lat.get_state() + (lat.energies(), lat.output_spectra(1)))
+    with gpu.Lattice(n, members=[(1.0, g2) for g2 in g2s], **kw) as batch:
+        assert batch.members == len(g2s)
+        views = [batch.member(m) for m in range(len(g2s))]
+        for m, v in enumerate(views):
+            v.set_state(*ics[m], 1.0, 0.0)
+            v.calc_metric()
+        with pytest.raises(gpu.LatticeError):
+            batch.energies()
+        batch.symp4(dt, nsteps)
+        got = [v.get_state() + (v.energies(), v.output_spectra(1)) for v in views]
+        for v in views:
+            v.close()
+    for m in range(len(g2s)):
+        gf, gp, gy, gpy, ge, gs = got[m]
+        sf, sp, sy, spy, se, ss = singles[m]
+        # same kernels, same per-member grids and reduction order: identical
+        assert np.array_equal(gf, sf) and np.array_equal(gp, sp), m
+        assert gy == sy and gpy == spy, m
+        assert ge["rho"] == se["rho"], m
+        assert np.array_equal(gs, ss), m
+    # and the members differ from each other (distinct couplings and seeds)
+    assert rel(got[0][0], got[2][0]) > 1e-9
+
+
+def test_batch_member_matches_oracle(gpu):
+    print("case batch oracle", flush=True)
+    n, L, dt, nsteps = 16, 10.0, 1e-3, 3
+    g2s = [5e4, 2e5]
+    with gpu.Lattice(n, nfld=2, length=L, members=[(1.0, g2) for g2 in g2s]) as batch:
+        views = [batch.member(m) for m in range(2)]
+        ics = [member_ic(n, L, 7 + m) for m in range(2)]
+        for m, v in enumerate(views):
+            v.set_state(*ics[m], 1.0, 0.0)
+            v.calc_metric()
+        batch.symp(6, dt, nsteps)
+        got = [v.get_state() for v in views]
+    for m, g2 in enumerate(g2s):
+        sim = O.Simulation(n, 2, L, O.COSMIC, model=O.MODEL_TWOFIELD, mp=(1.0, g2))
+        sim.set_state(*ics[m], 1.0, 0.0)
+        sim.calc_metric()
+        sim.symp(6, dt, nsteps)
+        assert rel(got[m][0], sim.fld) < 1e-12
+        assert rel(got[m][1], sim.fldp) < 1e-12
+        assert abs(got[m][2] - sim.yscl) < 1e-13
+
+
+def test_batch_rejects_per_member_calls_and_gw(gpu):
+    with gpu.Lattice(8, nfld=2, length=5.0, members=[(1.0, 1.0), (1.0, 2.0)]) as batch:
+        with pytest.raises(gpu.LatticeError):
+            batch.set_state(np.zeros((2, 8, 8, 8)), np.zeros((2, 8, 8, 8)))
+        with pytest.raises(gpu.LatticeError):
+            batch.member(2)
+    with pytest.raises(gpu.LatticeError):
+        gpu.Lattice(8, nfld=2, length=5.0, gw=True, members=[(1.0, 1.0), (1.0, 2.0)])
+
+
+def _ens_run(gpu, classes, params, ics, dts, calls, graph):
+    with gpu.Ensemble(classes, members=params) as ens:
+        ens.use_graph(graph)
+        views = [ens.member(i) for i in range(ens.size)]
+        for v, ic in zip(views, ics):
+            v.set_state(*ic, 1.0, 0.0)
+            v.calc_metric()
+        for nsteps in calls:
+            ens.symp(4, dts, nsteps)
+        ens.synchronize()
+        out = [v.get_state() for v in views]
+        for v in views:
+            v.close()
+    return out
+
+
+def test_ensemble_graph_replays_match_eager_and_single_runs(gpu):
+    """Two size classes, per-member couplings and seeds; hipGraph capture then
+    replay of the same call must equal the eager path bitwise, and each member
+    must match a single-context run."""
+    print("case ensemble", flush=True)
+    classes = [(16, 3, 10.0), (32, 2, 10.0)]
+    g2s = [1e4, 5e4, 2.5e5, 1e5, 2e5]
+    params = [(1.0, g) for g in g2s]
+    sizes = [16, 16, 16, 32, 32]
+    ics = [member_ic(n, 10.0, 300 + i) for i, n in enumerate(sizes)]
+    dts = [1e-3, 5e-4]
+    calls = [2, 2, 3]  # the second call replays the graph captured by the first
+    eager = _ens_run(gpu, classes, params, ics, dts, calls, graph=False)
+    graph = _ens_run(gpu, classes, params, ics, dts, calls, graph=True)
+    for i in range(len(g2s)):
+        for a, b in zip(eager[i], graph[i]):
+            assert np.array_equal(np.asarray(a), np.asarray(b)), i
+    for i, (n, g2) in enumerate(zip(sizes, g2s)):
+        with gpu.Lattice(n, nfld=2, length=10.0, model_params=(1.0, g2)) as lat:
+            lat.set_state(*ics[i], 1.0, 0.0)
+            lat.calc_metric()
+            dt = dts[0] if n == 16 else dts[1]
+            for nsteps in calls:
+                lat.symp4(dt, nsteps)
+            ref = lat.get_state()
+        assert rel(graph[i][0], ref[0]) < 1e-13, i
+        assert rel(graph[i][1], ref[1]) < 1e-12, i
+        assert abs(graph[i][2] - ref[2]) < 1e-14, i
+
+
+@pytest.mark.parametrize("n", [32, 64, 192])
+@pytest.mark.parametrize("precision", ["f64", "mixed"])
+def test_tuned_launch_shapes_keep_results(gpu, n, precision):
+    """The alternative launch shapes (fewer rows per x workgroup, fewer
+    columns per y/z tile) compute the same transforms; only the order of the
+    block partial sums changes."""
+    print("case tuning", n, precision, flush=True)
+    L, dt, nsteps = 10.0, 2e-4, 2
+    fld, fldp = member_ic(n, L, 55)
+    prec = gpu.F64 if precision == "f64" else gpu.MIXED
+    out = []
+    for shape in ((0, 0, 0, 0), (1, 1, 1, 1)):
+        with gpu.Lattice(n, nfld=2, length=L, precision=prec) as lat:
+            lat.set_tuning(*shape)
+            assert tuple(lat.tuning().values()) == shape
+            lat.set_state(fld, fldp, 1.0, 0.0)
+            lat.calc_metric()
+            lat.symp4(dt, nsteps)
+            out.append(lat.get_state())
+    tol = 1e-13 if precision == "f64" else 1e-6
+    assert rel(out[1][0], out[0][0]) < tol
+    assert rel(out[1][1], out[0][1]) < tol
+    assert abs(out[1][2] - out[0][2]) < 1e-13
+    with gpu.Lattice(n, nfld=2, length=L, precision=prec) as lat:
+        t = lat.autotune()
+        assert set(t.values()) <= {0, 1}
