@@ -534,17 +534,25 @@ template <typename T, int N>
 constexpr int col_tc(int mode, int v) { return v ? clampi((mode == 2 ? ColCfg<T, N, 2>::TC : ColCfg<T, N, 0>::TC) / 2, 4, 64)
                                               : (mode == 2 ? ColCfg<T, N, 2>::TC : ColCfg<T, N, 0>::TC); }
 
+// Twiddles in LDS (TWL) where the 1024^3 sweeps measured a gain: the x C2R
+// (both precisions), the FP32 x R2C, the y passes and the FP64 z pass.
+template <typename T>
+constexpr bool r2c_twl() { return sizeof(T) == 4; }
+template <typename T, int MODE>
+constexpr bool col_twl() { return MODE != 2 || sizeof(T) == 8; }
+
 template <typename T, int N, int L, int MR>
 static void r2c_go(Ctx &c, const cpx<T> *tw, T *f, const T *pi, cpx<T> *Fk, int drift, int field)
 {
     const int rows = c.ny * c.nz;
     const int g = grid_for((rows + L - 1) / L);
+    constexpr bool TW = r2c_twl<T>();
     if (drift)
-        hipLaunchKernelGGL((k_x_r2c<T, N, L, true, MR>), dim3(g, c.B), dim3(TPB), 0, c.stream, f, pi, Fk, tw, rows,
-                           c.P, c.dsc, field, c.ms);
+        hipLaunchKernelGGL((k_x_r2c<T, N, L, true, MR, TW>), dim3(g, c.B), dim3(TPB), 0, c.stream, f, pi, Fk, tw,
+                           rows, c.P, c.dsc, field, c.ms);
     else
-        hipLaunchKernelGGL((k_x_r2c<T, N, L, false, MR>), dim3(g, c.B), dim3(TPB), 0, c.stream, f, pi, Fk, tw, rows,
-                           c.P, c.dsc, field, c.ms);
+        hipLaunchKernelGGL((k_x_r2c<T, N, L, false, MR, TW>), dim3(g, c.B), dim3(TPB), 0, c.stream, f, pi, Fk, tw,
+                           rows, c.P, c.dsc, field, c.ms);
 }
 
 template <typename T, int N>
@@ -582,8 +590,9 @@ static void col_go(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int n_other, size_t s_a
 {
     using C = ColCfg<T, N, MODE == 2 ? 2 : 0>;
     const int g = grid_for(((c.nxh + TC - 1) / TC) * n_other * n_arrays);
-    hipLaunchKernelGGL((k_col<T, N, TC, MODE, C::MR, C::PF, 1, COL_NT>), dim3(g, c.B), dim3(COL_NT), 0, c.stream, Fk,
-                       tw, c.nxh, n_other, s_axis, s_other, c.nk, n_arrays, c.ny, kfac, c.ms.k);
+    hipLaunchKernelGGL((k_col<T, N, TC, MODE, C::MR, C::PF, 1, COL_NT, col_twl<T, MODE>()>), dim3(g, c.B),
+                       dim3(COL_NT), 0, c.stream, Fk, tw, c.nxh, n_other, s_axis, s_other, c.nk, n_arrays, c.ny, kfac,
+                       c.ms.k);
 }
 
 template <typename T, int N, int MODE>
@@ -630,17 +639,17 @@ static int c2r_go(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int nf, int epi, T *out,
     const int rows = c.ny * c.nz;
     const int g = grid_for((rows + L - 1) / L);
     if (epi == EPI_STORE)
-        hipLaunchKernelGGL((k_x_c2r<T, N, L, EPI_STORE>), dim3(g, c.B), dim3(TPB), 0, c.stream, Fk, tw, rows, c.P,
-                           c.nk, c.nvol, nf, out, fld, fldp, c.dsc, md, c.partials, c.ms);
+        hipLaunchKernelGGL((k_x_c2r<T, N, L, EPI_STORE, 4, true, true>), dim3(g, c.B), dim3(TPB), 0, c.stream, Fk, tw,
+                           rows, c.P, c.nk, c.nvol, nf, out, fld, fldp, c.dsc, md, c.partials, c.ms);
     else if (epi == EPI_KICK)
-        hipLaunchKernelGGL((k_x_c2r<T, N, L, EPI_KICK>), dim3(g, c.B), dim3(TPB), 0, c.stream, Fk, tw, rows, c.P,
-                           c.nk, c.nvol, nf, out, fld, fldp, c.dsc, md, c.partials, c.ms);
+        hipLaunchKernelGGL((k_x_c2r<T, N, L, EPI_KICK, 4, true, true>), dim3(g, c.B), dim3(TPB), 0, c.stream, Fk, tw,
+                           rows, c.P, c.nk, c.nvol, nf, out, fld, fldp, c.dsc, md, c.partials, c.ms);
     else if (epi == EPI_GWKICK)
-        hipLaunchKernelGGL((k_x_c2r<T, N, L, EPI_GWKICK>), dim3(g, c.B), dim3(TPB), 0, c.stream, Fk, tw, rows, c.P,
-                           c.nk, c.nvol, nf, out, fld, fldp, c.dsc, md, c.partials, c.ms);
+        hipLaunchKernelGGL((k_x_c2r<T, N, L, EPI_GWKICK, 4, true, true>), dim3(g, c.B), dim3(TPB), 0, c.stream, Fk, tw,
+                           rows, c.P, c.nk, c.nvol, nf, out, fld, fldp, c.dsc, md, c.partials, c.ms);
     else
-        hipLaunchKernelGGL((k_x_c2r<T, N, L, EPI_REDUCE>), dim3(g, c.B), dim3(TPB), 0, c.stream, Fk, tw, rows, c.P,
-                           c.nk, c.nvol, nf, out, fld, fldp, c.dsc, md, c.partials, c.ms);
+        hipLaunchKernelGGL((k_x_c2r<T, N, L, EPI_REDUCE, 4, true, true>), dim3(g, c.B), dim3(TPB), 0, c.stream, Fk, tw,
+                           rows, c.P, c.nk, c.nvol, nf, out, fld, fldp, c.dsc, md, c.partials, c.ms);
     return g;
 }
 
