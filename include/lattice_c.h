@@ -222,6 +222,37 @@ int  lat_make_output(lat_ctx *ctx, double time, int tree, const char *dir);
 int  lat_checkpoint(lat_ctx *ctx, const char *dir, double time, int outframenum);
 int  lat_restore(lat_ctx *ctx, const char *dir, double *time, int *outframenum);
 
+/* ---- learned corrector (no reference counterpart) ----
+ * A 4-layer 3x3x3 periodic convolutional network (4 -> 32 -> 32 -> 32 -> 4
+ * channels, SiLU, BF16 weights and activations, FP32 accumulation on MFMA)
+ * that maps the state after one coarse leapfrog step of coarse_factor * dt to
+ * the residual of coarse_factor Yoshida-4 steps of dt.  It is trained on the
+ * GPU (hand-written forward, backward and Adam kernels) on pairs that the
+ * plain integrator generates, and the plain integrator is the fallback when
+ * the Friedmann-constraint change of a corrected step exceeds a threshold.
+ * Contexts: single, two-field, FP64, cubic with n a multiple of 16.
+ * Weights of layer l: cout x 27 cin with k = tap * cin + ci, tap =
+ * (dz+1)*9 + (dy+1)*3 + (dx+1) for the neighbour at (x+dx, y+dy, z+dz);
+ * channels (phi, chi, pi_phi, pi_chi); normalisation input (v - mean) / sd,
+ * output residual / rsd per channel. */
+typedef struct lat_corrector lat_corrector;
+int  lat_corrector_create(unsigned long long seed, lat_corrector **out);
+void lat_corrector_destroy(lat_corrector *corr);
+const char *lat_corrector_last_error(const lat_corrector *corr);
+int  lat_corrector_set_layer(lat_corrector *corr, int layer, const float *w, const float *b);
+int  lat_corrector_get_layer(lat_corrector *corr, int layer, float *w, float *b);
+int  lat_corrector_set_norm(lat_corrector *corr, const double *mean, const double *sd, const double *rsd);
+int  lat_corrector_get_norm(const lat_corrector *corr, double *mean, double *sd, double *rsd);
+/* normalised network output [site][4] for the current state of ctx */
+int  lat_corrector_infer(lat_corrector *corr, lat_ctx *ctx, float *out);
+/* loss (MSE) and gradients (per layer: weights then biases) against target [site][4] */
+int  lat_corrector_gradients(lat_corrector *corr, lat_ctx *ctx, const float *target, double *loss, float *grads);
+int  lat_corrector_train(lat_corrector *corr, lat_ctx *ctx, double dt, int coarse_factor, int npairs, double lr,
+                         double *losses);
+/* stats[3] = fallbacks, largest accepted constraint change, force evaluations */
+int  lat_corrector_step(lat_corrector *corr, lat_ctx *ctx, double dt, int coarse_factor, int nsteps,
+                        double threshold, double *stats);
+
 /* ---- state (module Hamiltonian globals fld, fldp, yscl, ysclp,
  *      hamiltonian_conformal.f90:22-24; hamiltonian_cosmic.f90:27,31) ---- */
 int  lat_set_state(lat_ctx *ctx, const double *fld, const double *fldp, int ghost,

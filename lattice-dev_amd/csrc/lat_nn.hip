@@ -1,0 +1,855 @@
+// lat_nn.hip -- the learned corrector: a 3-D convolutional network on MFMA.
+//
+// Network: 4 layers of 3x3x3 periodic convolutions, 4 -> W -> W -> W -> 4
+// channels (W = 32), SiLU between layers, BF16 weights and activations, FP32
+// accumulation on the matrix cores (v_mfma_f32_16x16x16_bf16).  Input: the
+// normalised (phi, chi, pi_phi, pi_chi) after one coarse leapfrog step;
+// output: the normalised residual to the fine Yoshida-4 trajectory.
+//
+// Layout: activations are site-major with channels fastest, X[site][C]
+// (C a multiple of 4), so the four consecutive reduction indices an MFMA
+// lane supplies are one 8-byte load.  A convolution is an implicit GEMM
+//   Y[s][co] = sum_k W[co][k] X[nbr(s, tap(k))][ci(k)],  k = tap * CIN + ci,
+// with M = output channels (16-row tiles), N = 16 sites per wave tile and
+// K = 27 CIN padded to a multiple of 16.  The weight matrix of a layer is
+// staged in LDS once per workgroup.
+//
+// Backward: the data gradient is the same convolution with the transposed,
+// tap-reversed weights Wt[ci][t * COUT + co] = W[co][(26 - t) * CIN + ci];
+// the weight gradient is a GEMM over sites (K = sites) accumulated per chunk
+// of sites into partials that are reduced in a fixed order (deterministic).
+#include <cmath>
+
+#include "lat_internal.h"
+
+namespace lat {
+namespace nn {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short bf16;
+
+// round-to-nearest-even FP32 -> BF16 (finite inputs)
+__host__ __device__ inline bf16 to_bf16(float f)
+{
+    unsigned u = __builtin_bit_cast(unsigned, f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (bf16)(u >> 16);
+}
+__host__ __device__ inline float from_bf16(bf16 h) { return __builtin_bit_cast(float, (unsigned)h << 16); }
+
+__device__ inline float silu(float v) { return v / (1.0f + expf(-v)); }
+__device__ inline float dsilu(float v)
+{
+    const float s = 1.0f / (1.0f + expf(-v));
+    return s * (1.0f + v * (1.0f - s));
+}
+
+constexpr int KPAD(int cin) { return (27 * cin + 15) / 16 * 16; }
+
+// epilogue modes of the convolution
+enum { CONV_SILU = 0,  // Z = acc + b (BF16), Y = silu(Z) (BF16)
+       CONV_LINEAR = 1, // Yf = acc + b (FP32)
+       CONV_DSILU = 2 };  // Y = acc * silu'(Zin) (BF16): data gradient through the activation
+
+// Neighbour of site (x, y, z) for tap t = (dz+1)*9 + (dy+1)*3 + (dx+1).
+__device__ inline size_t nbr_site(int x, int y, int z, int t, int n)
+{
+    int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
+    int xx = x + dx, yy = y + dy, zz = z + dz;
+    xx = xx < 0 ? xx + n : (xx >= n ? xx - n : xx);
+    yy = yy < 0 ? yy + n : (yy >= n ? yy - n : yy);
+    zz = zz < 0 ? zz + n : (zz >= n ? zz - n : zz);
+    return (size_t)xx + (size_t)n * ((size_t)yy + (size_t)n * zz);
+}
+
+// One wave per 16-site tile; 4 waves per workgroup; grid-stride over tiles.
+// W: [MT*16][KP] BF16 (rows >= COUT and k >= 27*CIN are zero).
+template <int CIN, int COUT, int MODE>
+__global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ X, const bf16 *__restrict__ W,
+                                              const float *__restrict__ bias, bf16 *__restrict__ Y,
+                                              float *__restrict__ Yf, bf16 *__restrict__ Z,
+                                              const bf16 *__restrict__ Zin, int n)
+{
+    constexpr int MT = (COUT + 15) / 16;
+    constexpr int KP = KPAD(CIN);
+    __shared__ bf16 ws[MT * 16 * KP];
+    for (int q = threadIdx.x; q < MT * 16 * KP / 4; q += 256)
+        reinterpret_cast<bf16x4 *>(ws)[q] = reinterpret_cast<const bf16x4 *>(W)[q];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int j = lane & 15, kg = lane >> 4;
+    const size_t nvol = (size_t)n * n * n, ntile = nvol / 16;
+    for (size_t tile = (size_t)blockIdx.x * 4 + wave; tile < ntile; tile += (size_t)gridDim.x * 4) {
+        const size_t s = tile * 16 + j;
+        const int x = (int)(s % n), y = (int)((s / n) % n), z = (int)(s / ((size_t)n * n));
+        f32x4 acc[MT];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int k0 = 0; k0 < KP; k0 += 16) {
+            const int k = k0 + 4 * kg;
+            const int t = k / CIN, ci = k - t * CIN;
+            bf16x4 b = bf16x4{0, 0, 0, 0};
+            if (t < 27) b = *reinterpret_cast<const bf16x4 *>(X + nbr_site(x, y, z, t, n) * CIN + ci);
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const bf16x4 a = *reinterpret_cast<const bf16x4 *>(ws + (mt * 16 + j) * KP + k);
+                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[mt], 0, 0, 0);
+            }
+        }
+        // lane holds D[co][site] for co = mt*16 + 4*kg + e, site = tile*16 + j
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            const int co0 = mt * 16 + 4 * kg;
+            if (co0 >= COUT) continue;
+            if constexpr (MODE == CONV_LINEAR) {
+                float4 o;
+                o.x = acc[mt][0] + bias[co0];
+                o.y = acc[mt][1] + bias[co0 + 1];
+                o.z = acc[mt][2] + bias[co0 + 2];
+                o.w = acc[mt][3] + bias[co0 + 3];
+                *reinterpret_cast<float4 *>(Yf + s * COUT + co0) = o;
+            } else if constexpr (MODE == CONV_SILU) {
+                bf16x4 zo, yo;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const float v = acc[mt][e] + bias[co0 + e];
+                    const bf16 zb = to_bf16(v);
+                    zo[e] = (short)zb;
+                    yo[e] = (short)to_bf16(silu(from_bf16(zb)));
+                }
+                *reinterpret_cast<bf16x4 *>(Z + s * COUT + co0) = zo;
+                *reinterpret_cast<bf16x4 *>(Y + s * COUT + co0) = yo;
+            } else {
+                const bf16x4 zi = *reinterpret_cast<const bf16x4 *>(Zin + s * COUT + co0);
+                bf16x4 yo;
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    yo[e] = (short)to_bf16(acc[mt][e] * dsilu(from_bf16((bf16)zi[e])));
+                *reinterpret_cast<bf16x4 *>(Y + s * COUT + co0) = yo;
+            }
+        }
+    }
+}
+
+// Weight gradient: dW[co][t*CIN + ci] = sum_s G[s][co] X[nbr(s,t)][ci] over a
+// chunk of CHUNK sites per wave; one wave per (co tile, column tile, chunk).
+// part: [nchunk][MT*16][KP] FP32.  Also db[co] partial in column KP (part has
+// KP + 16 columns: the bias gradient uses a column of ones).
+constexpr int GCHUNK = 4096;
+template <int CIN, int COUT>
+__global__ __launch_bounds__(64) void k_conv_dw(const bf16 *__restrict__ G, const bf16 *__restrict__ X,
+                                                float *__restrict__ part, int n)
+{
+    constexpr int MT = (COUT + 15) / 16;
+    constexpr int KP = KPAD(CIN);
+    constexpr int CT = KP / 16 + 1; // the last column tile carries the bias column
+    constexpr int LD = KP + 16;
+    const int lane = threadIdx.x, j = lane & 15, kg = lane >> 4;
+    const int ct = blockIdx.x % CT, mt = (blockIdx.x / CT) % MT, chunk = blockIdx.x / (CT * MT);
+    const size_t nvol = (size_t)n * n * n;
+    const size_t s0 = (size_t)chunk * GCHUNK;
+    const int co = mt * 16 + j; // A row supplied by this lane
+    const int col = ct * 16 + j; // B column supplied by this lane
+    const bool bias_col = ct == CT - 1;
+    const int t = col / CIN, ci = col - t * CIN;
+    f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (size_t sb = s0; sb < s0 + GCHUNK && sb < nvol; sb += 16) {
+        bf16x4 a, b;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const size_t s = sb + 4 * kg + q;
+            a[q] = (short)((co < COUT) ? G[s * COUT + co] : (bf16)0);
+            bf16 bv = 0;
+            if (bias_col) {
+                bv = (j == 0) ? to_bf16(1.0f) : (bf16)0;
+            } else if (t < 27) {
+                const int x = (int)(s % n), y = (int)((s / n) % n), z = (int)(s / ((size_t)n * n));
+                bv = X[nbr_site(x, y, z, t, n) * CIN + ci];
+            }
+            b[q] = (short)bv;
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    // D[row = 4*kg + e][col = j]
+    float *dst = part + (size_t)chunk * (MT * 16) * LD;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int r = mt * 16 + 4 * kg + e;
+        const int cidx = bias_col ? KP + j : ct * 16 + j;
+        dst[(size_t)r * LD + cidx] = acc[e];
+    }
+}
+
+// out[q] = sum over chunks (in order) of part[chunk][q]
+__global__ __launch_bounds__(256) void k_sum_chunks(const float *__restrict__ part, int nchunk, int nq,
+                                                    float *__restrict__ out)
+{
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= nq) return;
+    float s = 0.f;
+    for (int c = 0; c < nchunk; ++c) s += part[(size_t)c * nq + q];
+    out[q] = s;
+}
+
+// Adam on the FP32 master weights of one layer; grad [MP][KP+16] (bias in
+// column KP, row co).  Writes the BF16 copy W and the transposed, tap-reversed
+// copy Wt[ci][t*COUT + co] (rows padded to a multiple of 16, K padded to KPt).
+template <int CIN, int COUT>
+__global__ __launch_bounds__(256) void k_adam(float *__restrict__ w, float *__restrict__ b, float *__restrict__ mw,
+                                              float *__restrict__ vw, float *__restrict__ mb,
+                                              float *__restrict__ vb, const float *__restrict__ grad, float lr,
+                                              float b1, float b2, float eps, float c1, float c2,
+                                              bf16 *__restrict__ Wb, bf16 *__restrict__ Wt)
+{
+    constexpr int KP = KPAD(CIN), LD = KP + 16;
+    constexpr int KPt = KPAD(COUT);
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q < COUT * 27 * CIN) {
+        const int co = q / (27 * CIN), k = q - co * (27 * CIN);
+        const float g = grad[(size_t)co * LD + k];
+        const float m = b1 * mw[q] + (1.f - b1) * g;
+        const float v = b2 * vw[q] + (1.f - b2) * g * g;
+        mw[q] = m;
+        vw[q] = v;
+        const float wn = w[q] - lr * (m / c1) / (sqrtf(v / c2) + eps);
+        w[q] = wn;
+        const bf16 hb = to_bf16(wn);
+        Wb[(size_t)co * KP + k] = hb;
+        const int t = k / CIN, ci = k - t * CIN;
+        Wt[(size_t)ci * KPt + (26 - t) * COUT + co] = hb;
+    } else if (q < COUT * 27 * CIN + COUT) {
+        const int co = q - COUT * 27 * CIN;
+        const float g = grad[(size_t)co * LD + KP];
+        const float m = b1 * mb[co] + (1.f - b1) * g;
+        const float v = b2 * vb[co] + (1.f - b2) * g * g;
+        mb[co] = m;
+        vb[co] = v;
+        b[co] = b[co] - lr * (m / c1) / (sqrtf(v / c2) + eps);
+    }
+}
+
+} // namespace nn
+} // namespace lat
+
+// ======================================================================
+// host side: network state, training on pairs from the plain integrator,
+// corrected coarse stepping with the plain integrator as fallback
+// ======================================================================
+#include <cstring>
+#include <random>
+#include <vector>
+
+namespace lat {
+namespace nn {
+
+constexpr int NL = 4;             // layers
+constexpr int WID = 32;           // hidden channels
+constexpr int CINS[NL] = {4, WID, WID, WID};
+constexpr int COUTS[NL] = {WID, WID, WID, 4};
+
+struct Layer {
+    int cin, cout, kp, mp, kpt, mpt; // padded shapes of W [mp][kp] and Wt [mpt][kpt]
+    float *w = nullptr, *b = nullptr, *mw = nullptr, *vw = nullptr, *mb = nullptr, *vb = nullptr; // FP32 master
+    bf16 *wb = nullptr, *wt = nullptr;                                                           // BF16 copies
+    float *grad = nullptr;                                                                       // [mp][kp+16]
+};
+
+struct Net {
+    Layer L[NL];
+    int step = 0;
+    // normalisation: input (v - mean) / sd, target residual / rsd, per channel
+    double mean[4] = {0, 0, 0, 0}, sd[4] = {1, 1, 1, 1}, rsd[4] = {1, 1, 1, 1};
+    bool stats = false;
+    // activation buffers for lattice size n
+    int n = 0;
+    bf16 *x0 = nullptr, *a[3] = {nullptr, nullptr, nullptr}, *z[3] = {nullptr, nullptr, nullptr};
+    bf16 *g4 = nullptr, *gA = nullptr, *gB = nullptr;
+    float *y = nullptr, *tgt = nullptr, *part = nullptr, *lossp = nullptr;
+    void *save_f = nullptr, *save_p = nullptr, *coarse_f = nullptr, *coarse_p = nullptr;
+    DevScalars *save_s = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+};
+
+static size_t layer_wcount(const Layer &l) { return (size_t)l.cout * 27 * l.cin; }
+
+} // namespace nn
+} // namespace lat
+
+struct lat_corrector {
+    lat::nn::Net net;
+};
+
+namespace lat {
+namespace nn {
+
+template <typename T>
+static int alloc(Net &net, T **p, size_t count, const char *what)
+{
+    if (hipMalloc((void **)p, sizeof(T) * count) != hipSuccess) {
+        net.err = std::string("hipMalloc ") + what;
+        return -3;
+    }
+    return 0;
+}
+
+static void free_buffers(Net &net)
+{
+    void *ps[] = {net.x0, net.a[0], net.a[1], net.a[2], net.z[0], net.z[1], net.z[2], net.g4, net.gA, net.gB,
+                  net.y, net.tgt, net.part, net.lossp, net.save_f, net.save_p, net.coarse_f, net.coarse_p,
+                  net.save_s};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    net.x0 = net.g4 = net.gA = net.gB = nullptr;
+    for (int q = 0; q < 3; ++q) net.a[q] = net.z[q] = nullptr;
+    net.y = net.tgt = net.part = net.lossp = nullptr;
+    net.save_f = net.save_p = net.coarse_f = net.coarse_p = nullptr;
+    net.save_s = nullptr;
+    net.n = 0;
+}
+
+static size_t nchunks(size_t nvol) { return (nvol + GCHUNK - 1) / GCHUNK; }
+
+static int ensure_buffers(Net &net, const Ctx &c)
+{
+    if (net.n == c.nx) return 0;
+    free_buffers(net);
+    const size_t nv = c.nvol;
+    int rc = 0;
+    rc |= alloc(net, &net.x0, nv * 4, "x0");
+    for (int q = 0; q < 3; ++q) {
+        rc |= alloc(net, &net.a[q], nv * WID, "act");
+        rc |= alloc(net, &net.z[q], nv * WID, "preact");
+    }
+    rc |= alloc(net, &net.g4, nv * 4, "g4");
+    rc |= alloc(net, &net.gA, nv * WID, "gradA");
+    rc |= alloc(net, &net.gB, nv * WID, "gradB");
+    rc |= alloc(net, &net.y, nv * 4, "out");
+    rc |= alloc(net, &net.tgt, nv * 4, "target");
+    size_t maxpart = 0;
+    for (int l = 0; l < NL; ++l) {
+        const size_t q = (size_t)net.L[l].mp * (net.L[l].kp + 16);
+        maxpart = q > maxpart ? q : maxpart;
+    }
+    rc |= alloc(net, &net.part, maxpart * nchunks(nv), "dW partials");
+    rc |= alloc(net, &net.lossp, 1024, "loss partials");
+    const size_t es = elem_size(c);
+    rc |= alloc(net, (char **)&net.save_f, es * c.ms.f, "saved fields");
+    rc |= alloc(net, (char **)&net.save_p, es * c.ms.f, "saved momenta");
+    rc |= alloc(net, (char **)&net.coarse_f, es * c.ms.f, "coarse fields");
+    rc |= alloc(net, (char **)&net.coarse_p, es * c.ms.f, "coarse momenta");
+    rc |= alloc(net, &net.save_s, 1, "saved scalars");
+    if (rc) {
+        free_buffers(net);
+        return rc;
+    }
+    net.n = c.nx;
+    return 0;
+}
+
+// ---------------------------------------------------------------- data kernels
+// X0[s][c] = (v_c(s) - mean_c) / sd_c for (phi, chi, pi_phi, pi_chi)
+__global__ __launch_bounds__(256) void k_pack(const double *__restrict__ f, const double *__restrict__ p,
+                                              size_t nvol, double4 mean, double4 isd, bf16 *__restrict__ X)
+{
+    for (size_t s = blockIdx.x * (size_t)256 + threadIdx.x; s < nvol; s += (size_t)gridDim.x * 256) {
+        bf16x4 o;
+        o[0] = (short)to_bf16((float)((f[s] - mean.x) * isd.x));
+        o[1] = (short)to_bf16((float)((f[nvol + s] - mean.y) * isd.y));
+        o[2] = (short)to_bf16((float)((p[s] - mean.z) * isd.z));
+        o[3] = (short)to_bf16((float)((p[nvol + s] - mean.w) * isd.w));
+        *reinterpret_cast<bf16x4 *>(X + s * 4) = o;
+    }
+}
+
+// tgt[s][c] = (fine_c - coarse_c) / rsd_c
+__global__ __launch_bounds__(256) void k_target(const double *__restrict__ ff, const double *__restrict__ fp,
+                                                const double *__restrict__ cf, const double *__restrict__ cp,
+                                                size_t nvol, double4 irsd, float *__restrict__ tgt)
+{
+    for (size_t s = blockIdx.x * (size_t)256 + threadIdx.x; s < nvol; s += (size_t)gridDim.x * 256) {
+        float4 o;
+        o.x = (float)((ff[s] - cf[s]) * irsd.x);
+        o.y = (float)((ff[nvol + s] - cf[nvol + s]) * irsd.y);
+        o.z = (float)((fp[s] - cp[s]) * irsd.z);
+        o.w = (float)((fp[nvol + s] - cp[nvol + s]) * irsd.w);
+        *reinterpret_cast<float4 *>(tgt + s * 4) = o;
+    }
+}
+
+// G4 = 2 (y - tgt) / (4 nvol) in BF16; per-block partial sums of (y - tgt)^2
+__global__ __launch_bounds__(256) void k_loss(const float *__restrict__ y, const float *__restrict__ tgt,
+                                              size_t nvol, bf16 *__restrict__ g, float *__restrict__ part)
+{
+    __shared__ float red[256];
+    const float scale = 2.0f / (4.0f * (float)nvol);
+    float acc = 0.f;
+    for (size_t q = blockIdx.x * (size_t)256 + threadIdx.x; q < nvol * 4; q += (size_t)gridDim.x * 256) {
+        const float d = y[q] - tgt[q];
+        acc += d * d;
+        g[q] = to_bf16(d * scale);
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// f_c += rsd_c * y_c (apply the predicted residual)
+__global__ __launch_bounds__(256) void k_apply(double *__restrict__ f, double *__restrict__ p,
+                                               const float *__restrict__ y, size_t nvol, double4 rsd)
+{
+    for (size_t s = blockIdx.x * (size_t)256 + threadIdx.x; s < nvol; s += (size_t)gridDim.x * 256) {
+        const float4 v = *reinterpret_cast<const float4 *>(y + s * 4);
+        f[s] += rsd.x * v.x;
+        f[nvol + s] += rsd.y * v.y;
+        p[s] += rsd.z * v.z;
+        p[nvol + s] += rsd.w * v.w;
+    }
+}
+
+// ---------------------------------------------------------------- network passes
+template <int CIN, int COUT, int MODE>
+static void conv(Net &net, const bf16 *X, const bf16 *W, const float *bias, bf16 *Y, float *Yf, bf16 *Z,
+                 const bf16 *Zin)
+{
+    const size_t ntile = (size_t)net.n * net.n * net.n / 16;
+    size_t g = (ntile + 3) / 4;
+    if (g > 8192) g = 8192;
+    hipLaunchKernelGGL((k_conv<CIN, COUT, MODE>), dim3((unsigned)g), dim3(256), 0, net.stream, X, W, bias, Y, Yf, Z,
+                       Zin, net.n);
+}
+
+static void forward(Net &net)
+{
+    conv<4, WID, CONV_SILU>(net, net.x0, net.L[0].wb, net.L[0].b, net.a[0], nullptr, net.z[0], nullptr);
+    conv<WID, WID, CONV_SILU>(net, net.a[0], net.L[1].wb, net.L[1].b, net.a[1], nullptr, net.z[1], nullptr);
+    conv<WID, WID, CONV_SILU>(net, net.a[1], net.L[2].wb, net.L[2].b, net.a[2], nullptr, net.z[2], nullptr);
+    conv<WID, 4, CONV_LINEAR>(net, net.a[2], net.L[3].wb, net.L[3].b, nullptr, net.y, nullptr, nullptr);
+}
+
+template <int CIN, int COUT>
+static void weight_grad(Net &net, Layer &l, const bf16 *G, const bf16 *X)
+{
+    const size_t nv = (size_t)net.n * net.n * net.n;
+    const int MT = (COUT + 15) / 16, CT = KPAD(CIN) / 16 + 1;
+    const size_t nch = nchunks(nv);
+    hipLaunchKernelGGL((k_conv_dw<CIN, COUT>), dim3((unsigned)(nch * MT * CT)), dim3(64), 0, net.stream, G, X,
+                       net.part, net.n);
+    const int nq = l.mp * (l.kp + 16);
+    hipLaunchKernelGGL(k_sum_chunks, dim3((nq + 255) / 256), dim3(256), 0, net.stream, net.part, (int)nch, nq,
+                       l.grad);
+}
+
+// gradients of all layers for the current x0 / tgt; returns after launching
+static void backward(Net &net)
+{
+    const size_t nv = (size_t)net.n * net.n * net.n;
+    hipLaunchKernelGGL(k_loss, dim3(1024), dim3(256), 0, net.stream, net.y, net.tgt, nv, net.g4, net.lossp);
+    weight_grad<WID, 4>(net, net.L[3], net.g4, net.a[2]);
+    conv<4, WID, CONV_DSILU>(net, net.g4, net.L[3].wt, nullptr, net.gA, nullptr, nullptr, net.z[2]);
+    weight_grad<WID, WID>(net, net.L[2], net.gA, net.a[1]);
+    conv<WID, WID, CONV_DSILU>(net, net.gA, net.L[2].wt, nullptr, net.gB, nullptr, nullptr, net.z[1]);
+    weight_grad<WID, WID>(net, net.L[1], net.gB, net.a[0]);
+    conv<WID, WID, CONV_DSILU>(net, net.gB, net.L[1].wt, nullptr, net.gA, nullptr, nullptr, net.z[0]);
+    weight_grad<4, WID>(net, net.L[0], net.gA, net.x0);
+}
+
+template <int CIN, int COUT>
+static void adam_layer(Net &net, Layer &l, float lr, float c1, float c2)
+{
+    const int nq = COUT * 27 * CIN + COUT;
+    hipLaunchKernelGGL((k_adam<CIN, COUT>), dim3((nq + 255) / 256), dim3(256), 0, net.stream, l.w, l.b, l.mw, l.vw,
+                       l.mb, l.vb, l.grad, lr, 0.9f, 0.999f, 1e-8f, c1, c2, l.wb, l.wt);
+}
+
+static void adam(Net &net, float lr)
+{
+    ++net.step;
+    const float c1 = 1.f - powf(0.9f, (float)net.step), c2 = 1.f - powf(0.999f, (float)net.step);
+    adam_layer<4, WID>(net, net.L[0], lr, c1, c2);
+    adam_layer<WID, WID>(net, net.L[1], lr, c1, c2);
+    adam_layer<WID, WID>(net, net.L[2], lr, c1, c2);
+    adam_layer<WID, 4>(net, net.L[3], lr, c1, c2);
+}
+
+// Upload FP32 master weights (cout x 27 cin, k = tap*cin + ci) and biases of
+// layer l, rebuilding the BF16 copies; the Adam moments are reset.
+static int set_layer(Net &net, int li, const float *w, const float *b)
+{
+    Layer &l = net.L[li];
+    const size_t nw = layer_wcount(l);
+    std::vector<bf16> wb((size_t)l.mp * l.kp, 0), wt((size_t)l.mpt * l.kpt, 0);
+    for (int co = 0; co < l.cout; ++co)
+        for (int k = 0; k < 27 * l.cin; ++k) {
+            const bf16 h = to_bf16(w[(size_t)co * 27 * l.cin + k]);
+            wb[(size_t)co * l.kp + k] = h;
+            const int t = k / l.cin, ci = k - t * l.cin;
+            wt[(size_t)ci * l.kpt + (26 - t) * l.cout + co] = h;
+        }
+    std::vector<float> zeros(nw > (size_t)l.cout ? nw : l.cout, 0.f);
+    bool ok = hipMemcpy(l.w, w, sizeof(float) * nw, hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(l.b, b, sizeof(float) * l.cout, hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(l.mw, zeros.data(), sizeof(float) * nw, hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(l.vw, zeros.data(), sizeof(float) * nw, hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(l.mb, zeros.data(), sizeof(float) * l.cout, hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(l.vb, zeros.data(), sizeof(float) * l.cout, hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(l.wb, wb.data(), sizeof(bf16) * wb.size(), hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(l.wt, wt.data(), sizeof(bf16) * wt.size(), hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) {
+        net.err = "weight upload failed";
+        return -3;
+    }
+    return 0;
+}
+
+} // namespace nn
+} // namespace lat
+
+// ---------------------------------------------------------------- C ABI
+using namespace lat;
+using namespace lat::nn;
+
+static int corr_fail(Net &net, const char *msg)
+{
+    net.err = msg;
+    return -2;
+}
+
+static int check_ctx(Net &net, Ctx &c)
+{
+    if (c.B != 1 || c.view) return corr_fail(net, "the corrector runs on single contexts");
+    if (c.prec != LAT_F64 || c.nfld != 2) return corr_fail(net, "the corrector needs a two-field FP64 context");
+    if (c.nx != c.ny || c.nx != c.nz || c.nx % 16) return corr_fail(net, "the corrector needs a cubic lattice, n % 16 == 0");
+    net.stream = c.stream;
+    return ensure_buffers(net, c);
+}
+
+static double4 d4(const double *v) { return double4{v[0], v[1], v[2], v[3]}; }
+static double4 inv4(const double *v) { return double4{1.0 / v[0], 1.0 / v[1], 1.0 / v[2], 1.0 / v[3]}; }
+
+static int grid_sites(size_t nvol)
+{
+    size_t g = (nvol + 255) / 256;
+    return (int)(g > 8192 ? 8192 : g);
+}
+
+static void pack(Net &net, Ctx &c)
+{
+    hipLaunchKernelGGL(k_pack, dim3(grid_sites(c.nvol)), dim3(256), 0, c.stream, (const double *)c.fld,
+                       (const double *)c.fldp, c.nvol, d4(net.mean), inv4(net.sd), net.x0);
+}
+
+static int save_state(Net &net, Ctx &c)
+{
+    const size_t bytes = sizeof(double) * c.ms.f;
+    LAT_HIP(&c, hipMemcpyAsync(net.save_f, c.fld, bytes, hipMemcpyDeviceToDevice, c.stream));
+    LAT_HIP(&c, hipMemcpyAsync(net.save_p, c.fldp, bytes, hipMemcpyDeviceToDevice, c.stream));
+    LAT_HIP(&c, hipMemcpyAsync(net.save_s, c.dsc, sizeof(DevScalars), hipMemcpyDeviceToDevice, c.stream));
+    return 0;
+}
+
+static int restore_state(Net &net, Ctx &c)
+{
+    const size_t bytes = sizeof(double) * c.ms.f;
+    LAT_HIP(&c, hipMemcpyAsync(c.fld, net.save_f, bytes, hipMemcpyDeviceToDevice, c.stream));
+    LAT_HIP(&c, hipMemcpyAsync(c.fldp, net.save_p, bytes, hipMemcpyDeviceToDevice, c.stream));
+    LAT_HIP(&c, hipMemcpyAsync(c.dsc, net.save_s, sizeof(DevScalars), hipMemcpyDeviceToDevice, c.stream));
+    c.ke_valid = false;
+    return 0;
+}
+
+// per-channel mean and standard deviation of (phi, chi, pi_phi, pi_chi) of two device state copies (host pass)
+static int channel_stats(Ctx &c, const void *f, const void *p, double *mean, double *sd)
+{
+    const size_t nv = c.nvol;
+    std::vector<double> hf(2 * nv), hp(2 * nv);
+    LAT_HIP(&c, hipStreamSynchronize(c.stream));
+    LAT_HIP(&c, hipMemcpy(hf.data(), f, sizeof(double) * 2 * nv, hipMemcpyDeviceToHost));
+    LAT_HIP(&c, hipMemcpy(hp.data(), p, sizeof(double) * 2 * nv, hipMemcpyDeviceToHost));
+    for (int ch = 0; ch < 4; ++ch) {
+        const double *v = (ch < 2 ? hf.data() : hp.data()) + (size_t)(ch & 1) * nv;
+        double m = 0, q = 0;
+        for (size_t s = 0; s < nv; ++s) m += v[s];
+        m /= (double)nv;
+        for (size_t s = 0; s < nv; ++s) q += (v[s] - m) * (v[s] - m);
+        mean[ch] = m;
+        sd[ch] = std::sqrt(q / (double)nv);
+        if (!(sd[ch] > 0.0)) sd[ch] = 1.0;
+    }
+    return 0;
+}
+
+// residual scale: RMS of (fine - coarse) per channel
+static int residual_stats(Ctx &c, const void *ff, const void *fp, const void *cf, const void *cp, double *rsd)
+{
+    const size_t nv = c.nvol;
+    std::vector<double> a(2 * nv), b(2 * nv), d(2 * nv), e(2 * nv);
+    LAT_HIP(&c, hipStreamSynchronize(c.stream));
+    LAT_HIP(&c, hipMemcpy(a.data(), ff, sizeof(double) * 2 * nv, hipMemcpyDeviceToHost));
+    LAT_HIP(&c, hipMemcpy(b.data(), fp, sizeof(double) * 2 * nv, hipMemcpyDeviceToHost));
+    LAT_HIP(&c, hipMemcpy(d.data(), cf, sizeof(double) * 2 * nv, hipMemcpyDeviceToHost));
+    LAT_HIP(&c, hipMemcpy(e.data(), cp, sizeof(double) * 2 * nv, hipMemcpyDeviceToHost));
+    for (int ch = 0; ch < 4; ++ch) {
+        const double *x = (ch < 2 ? a.data() : b.data()) + (size_t)(ch & 1) * nv;
+        const double *y = (ch < 2 ? d.data() : e.data()) + (size_t)(ch & 1) * nv;
+        double q = 0;
+        for (size_t s = 0; s < nv; ++s) q += (x[s] - y[s]) * (x[s] - y[s]);
+        rsd[ch] = std::sqrt(q / (double)nv);
+        if (!(rsd[ch] > 0.0)) rsd[ch] = 1.0;
+    }
+    return 0;
+}
+
+extern "C" {
+
+int lat_corrector_create(unsigned long long seed, lat_corrector **out)
+{
+    if (!out) return -1;
+    *out = nullptr;
+    lat_corrector *h = new lat_corrector();
+    Net &net = h->net;
+    std::mt19937_64 rng(seed);
+    for (int li = 0; li < NL; ++li) {
+        Layer &l = net.L[li];
+        l.cin = CINS[li];
+        l.cout = COUTS[li];
+        l.kp = KPAD(l.cin);
+        l.mp = (l.cout + 15) / 16 * 16;
+        l.kpt = KPAD(l.cout);
+        l.mpt = (l.cin + 15) / 16 * 16;
+        const size_t nw = layer_wcount(l);
+        int rc = 0;
+        rc |= alloc(net, &l.w, nw, "w");
+        rc |= alloc(net, &l.mw, nw, "adam m");
+        rc |= alloc(net, &l.vw, nw, "adam v");
+        rc |= alloc(net, &l.b, l.cout, "b");
+        rc |= alloc(net, &l.mb, l.cout, "adam mb");
+        rc |= alloc(net, &l.vb, l.cout, "adam vb");
+        rc |= alloc(net, &l.wb, (size_t)l.mp * l.kp, "w bf16");
+        rc |= alloc(net, &l.wt, (size_t)l.mpt * l.kpt, "wt bf16");
+        rc |= alloc(net, &l.grad, (size_t)l.mp * (l.kp + 16), "grad");
+        if (rc) {
+            lat_corrector_destroy(h);
+            return -3;
+        }
+        // He-normal initialisation; the last layer starts small so that the
+        // untrained corrector is close to the plain coarse step
+        const double sd = std::sqrt(2.0 / (27.0 * l.cin)) * (li == NL - 1 ? 0.1 : 1.0);
+        std::normal_distribution<double> nd(0.0, sd);
+        std::vector<float> w(nw), b(l.cout, 0.f);
+        for (auto &x : w) x = (float)nd(rng);
+        if (set_layer(net, li, w.data(), b.data())) {
+            lat_corrector_destroy(h);
+            return -3;
+        }
+    }
+    *out = h;
+    return 0;
+}
+
+void lat_corrector_destroy(lat_corrector *h)
+{
+    if (!h) return;
+    Net &net = h->net;
+    if (net.stream) (void)hipStreamSynchronize(net.stream);
+    free_buffers(net);
+    for (auto &l : net.L) {
+        void *ps[] = {l.w, l.b, l.mw, l.vw, l.mb, l.vb, l.wb, l.wt, l.grad};
+        for (void *p : ps)
+            if (p) (void)hipFree(p);
+    }
+    delete h;
+}
+
+const char *lat_corrector_last_error(const lat_corrector *h) { return h ? h->net.err.c_str() : "null corrector"; }
+
+int lat_corrector_set_layer(lat_corrector *h, int layer, const float *w, const float *b)
+{
+    if (!h || !w || !b || layer < 0 || layer >= NL) return -1;
+    return set_layer(h->net, layer, w, b);
+}
+
+int lat_corrector_get_layer(lat_corrector *h, int layer, float *w, float *b)
+{
+    if (!h || !w || !b || layer < 0 || layer >= NL) return -1;
+    Layer &l = h->net.L[layer];
+    if (h->net.stream) (void)hipStreamSynchronize(h->net.stream);
+    if (hipMemcpy(w, l.w, sizeof(float) * layer_wcount(l), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(b, l.b, sizeof(float) * l.cout, hipMemcpyDeviceToHost) != hipSuccess)
+        return -3;
+    return 0;
+}
+
+int lat_corrector_set_norm(lat_corrector *h, const double *mean, const double *sd, const double *rsd)
+{
+    if (!h || !mean || !sd || !rsd) return -1;
+    for (int q = 0; q < 4; ++q) {
+        if (!(sd[q] > 0.0) || !(rsd[q] > 0.0)) return -1;
+        h->net.mean[q] = mean[q];
+        h->net.sd[q] = sd[q];
+        h->net.rsd[q] = rsd[q];
+    }
+    h->net.stats = true;
+    return 0;
+}
+
+int lat_corrector_get_norm(const lat_corrector *h, double *mean, double *sd, double *rsd)
+{
+    if (!h || !mean || !sd || !rsd) return -1;
+    for (int q = 0; q < 4; ++q) {
+        mean[q] = h->net.mean[q];
+        sd[q] = h->net.sd[q];
+        rsd[q] = h->net.rsd[q];
+    }
+    return 0;
+}
+
+// Network output (normalised residual, [site][4] FP32) for the current state of ctx.
+int lat_corrector_infer(lat_corrector *h, lat_ctx *ctx, float *out)
+{
+    if (!h || !ctx || !out) return -1;
+    Net &net = h->net;
+    Ctx &c = ctx->c;
+    int rc = check_ctx(net, c);
+    if (rc) return rc;
+    pack(net, c);
+    forward(net);
+    LAT_HIP(&c, hipMemcpyAsync(out, net.y, sizeof(float) * 4 * c.nvol, hipMemcpyDeviceToHost, c.stream));
+    LAT_HIP(&c, hipStreamSynchronize(c.stream));
+    return 0;
+}
+
+// Loss and gradients for the current state of ctx as input and `target`
+// ([site][4] FP32, normalised residual): *loss = mean squared error; grads
+// (nullable) receives, per layer, cout x 27 cin weight gradients then cout
+// bias gradients.  No weight update.
+int lat_corrector_gradients(lat_corrector *h, lat_ctx *ctx, const float *target, double *loss, float *grads)
+{
+    if (!h || !ctx || !target) return -1;
+    Net &net = h->net;
+    Ctx &c = ctx->c;
+    int rc = check_ctx(net, c);
+    if (rc) return rc;
+    LAT_HIP(&c, hipMemcpyAsync(net.tgt, target, sizeof(float) * 4 * c.nvol, hipMemcpyHostToDevice, c.stream));
+    pack(net, c);
+    forward(net);
+    backward(net);
+    std::vector<float> lp(1024);
+    LAT_HIP(&c, hipMemcpyAsync(lp.data(), net.lossp, sizeof(float) * 1024, hipMemcpyDeviceToHost, c.stream));
+    LAT_HIP(&c, hipStreamSynchronize(c.stream));
+    double s = 0;
+    for (float v : lp) s += v;
+    if (loss) *loss = s / (4.0 * (double)c.nvol);
+    if (grads) {
+        size_t off = 0;
+        for (auto &l : net.L) {
+            std::vector<float> g((size_t)l.mp * (l.kp + 16));
+            LAT_HIP(&c, hipMemcpy(g.data(), l.grad, sizeof(float) * g.size(), hipMemcpyDeviceToHost));
+            for (int co = 0; co < l.cout; ++co)
+                for (int k = 0; k < 27 * l.cin; ++k) grads[off++] = g[(size_t)co * (l.kp + 16) + k];
+            for (int co = 0; co < l.cout; ++co) grads[off++] = g[(size_t)co * (l.kp + 16) + l.kp];
+        }
+    }
+    return 0;
+}
+
+// Train on npairs snapshot pairs generated along the trajectory of ctx by the
+// plain integrator: from the current state S, coarse = one leapfrog step of
+// coarse_factor * dt, fine = coarse_factor Yoshida-4 steps of dt; input =
+// coarse, target = (fine - coarse) / rsd; one Adam step per pair; the
+// trajectory continues from the fine state.  losses (nullable): per pair.
+int lat_corrector_train(lat_corrector *h, lat_ctx *ctx, double dt, int coarse_factor, int npairs, double lr,
+                        double *losses)
+{
+    if (!h || !ctx || coarse_factor < 1 || npairs < 0) return -1;
+    Net &net = h->net;
+    Ctx &c = ctx->c;
+    int rc = check_ctx(net, c);
+    if (rc) return rc;
+    const size_t bytes = sizeof(double) * c.ms.f;
+    for (int it = 0; it < npairs; ++it) {
+        if ((rc = save_state(net, c))) return rc;
+        if ((rc = lat_symp(ctx, 2, coarse_factor * dt, 1))) return rc;
+        LAT_HIP(&c, hipMemcpyAsync(net.coarse_f, c.fld, bytes, hipMemcpyDeviceToDevice, c.stream));
+        LAT_HIP(&c, hipMemcpyAsync(net.coarse_p, c.fldp, bytes, hipMemcpyDeviceToDevice, c.stream));
+        if ((rc = restore_state(net, c))) return rc;
+        if ((rc = lat_symp(ctx, 4, dt, coarse_factor))) return rc;
+        if (!net.stats) {
+            if ((rc = channel_stats(c, net.coarse_f, net.coarse_p, net.mean, net.sd))) return rc;
+            if ((rc = residual_stats(c, c.fld, c.fldp, net.coarse_f, net.coarse_p, net.rsd))) return rc;
+            net.stats = true;
+        }
+        hipLaunchKernelGGL(k_pack, dim3(grid_sites(c.nvol)), dim3(256), 0, c.stream, (const double *)net.coarse_f,
+                           (const double *)net.coarse_p, c.nvol, d4(net.mean), inv4(net.sd), net.x0);
+        hipLaunchKernelGGL(k_target, dim3(grid_sites(c.nvol)), dim3(256), 0, c.stream, (const double *)c.fld,
+                           (const double *)c.fldp, (const double *)net.coarse_f, (const double *)net.coarse_p,
+                           c.nvol, inv4(net.rsd), net.tgt);
+        forward(net);
+        backward(net);
+        adam(net, (float)lr);
+        if (losses) {
+            std::vector<float> lp(1024);
+            LAT_HIP(&c, hipMemcpyAsync(lp.data(), net.lossp, sizeof(float) * 1024, hipMemcpyDeviceToHost, c.stream));
+            LAT_HIP(&c, hipStreamSynchronize(c.stream));
+            double s = 0;
+            for (float v : lp) s += v;
+            losses[it] = s / (4.0 * (double)c.nvol);
+        }
+    }
+    LAT_HIP(&c, hipGetLastError());
+    return 0;
+}
+
+// nsteps corrected coarse steps: save S, leapfrog(coarse_factor * dt), add
+// the predicted residual, then check the Friedmann constraint: if
+// |resid_after - resid_before| > threshold, restore S and take
+// coarse_factor Yoshida-4 steps of dt instead (the plain integrator is the
+// fallback).  stats (nullable, 3 doubles): fallbacks, max accepted
+// constraint change, force evaluations used (a coarse step costs 1, a
+// Yoshida-4 step 3).
+int lat_corrector_step(lat_corrector *h, lat_ctx *ctx, double dt, int coarse_factor, int nsteps, double threshold,
+                       double *stats)
+{
+    if (!h || !ctx || coarse_factor < 1 || nsteps < 0) return -1;
+    Net &net = h->net;
+    Ctx &c = ctx->c;
+    if (!net.stats) return corr_fail(net, "the corrector has not been trained (no normalisation)");
+    int rc = check_ctx(net, c);
+    if (rc) return rc;
+    double fb = 0, maxv = 0, forces = 0, e[LAT_NE];
+    for (int it = 0; it < nsteps; ++it) {
+        if ((rc = lat_energies(ctx, e))) return rc;
+        const double r0 = e[LAT_E_RESID];
+        if ((rc = save_state(net, c))) return rc;
+        if ((rc = lat_symp(ctx, 2, coarse_factor * dt, 1))) return rc;
+        pack(net, c);
+        forward(net);
+        hipLaunchKernelGGL(k_apply, dim3(grid_sites(c.nvol)), dim3(256), 0, c.stream, (double *)c.fld,
+                           (double *)c.fldp, net.y, c.nvol, d4(net.rsd));
+        c.ke_valid = false;
+        forces += 1;
+        if ((rc = lat_energies(ctx, e))) return rc;
+        const double v = std::fabs(e[LAT_E_RESID] - r0);
+        if (!(v <= threshold)) {
+            if ((rc = restore_state(net, c))) return rc;
+            if ((rc = lat_symp(ctx, 4, dt, coarse_factor))) return rc;
+            fb += 1;
+            forces += 3.0 * coarse_factor;
+        } else if (v > maxv) {
+            maxv = v;
+        }
+    }
+    if (stats) {
+        stats[0] = fb;
+        stats[1] = maxv;
+        stats[2] = forces;
+    }
+    return 0;
+}
+
+} // extern "C"

@@ -66,6 +66,9 @@ EXPORTS = [
     "lat_ensemble_member", "lat_ensemble_use_graph", "lat_ensemble_symp", "lat_ensemble_synchronize",
     "lat_autotune", "lat_tuning", "lat_set_tuning", "lat_copy_bandwidth", "lat_make_output",
     "lat_checkpoint", "lat_restore",
+    "lat_corrector_create", "lat_corrector_destroy", "lat_corrector_last_error", "lat_corrector_set_layer",
+    "lat_corrector_get_layer", "lat_corrector_set_norm", "lat_corrector_get_norm", "lat_corrector_infer",
+    "lat_corrector_gradients", "lat_corrector_train", "lat_corrector_step",
 ]
 STAGES = ["x_r2c", "y_fwd", "z_lap", "y_inv", "x_c2r", "scalar", "drift", "stencil", "reduce", "gw_src"]
 
@@ -84,8 +87,10 @@ def lib():
         for name in EXPORTS:
             fn = getattr(L, name)
             if name not in ("lat_last_error", "lat_destroy", "lat_create", "lat_ensemble_destroy",
-                            "lat_ensemble_last_error"):
+                            "lat_ensemble_last_error", "lat_corrector_destroy", "lat_corrector_last_error"):
                 fn.restype = ctypes.c_int
+        L.lat_corrector_last_error.restype = ctypes.c_char_p
+        L.lat_corrector_destroy.argtypes = [ctypes.c_void_p]
         L.lat_ensemble_last_error.restype = ctypes.c_char_p
         L.lat_ensemble_destroy.argtypes = [ctypes.c_void_p]
         _lib = L
@@ -549,3 +554,101 @@ class Ensemble:
 
     def synchronize(self):
         self._check(lib().lat_ensemble_synchronize(self._h), "lat_ensemble_synchronize")
+
+
+_fp = ctypes.POINTER(ctypes.c_float)
+
+
+def _pf(a):
+    return a.ctypes.data_as(_fp)
+
+
+class Corrector:
+    """The learned corrector (lat_corrector_*): a 4-layer BF16 3-D conv net
+    on MFMA trained on the GPU on pairs from the plain integrator."""
+
+    SHAPES = [(4, 32), (32, 32), (32, 32), (32, 4)]   # (cin, cout) per layer
+
+    def __init__(self, seed=1):
+        h = ctypes.c_void_p()
+        rc = lib().lat_corrector_create(ctypes.c_ulonglong(seed), ctypes.byref(h))
+        if rc != 0:
+            raise LatticeError(f"lat_corrector_create failed with status {rc}")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lat_corrector_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = lib().lat_corrector_last_error(self._h)
+            raise LatticeError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def set_layer(self, layer, w, b):
+        """w: (cout, 27, cin) with tap = (dz+1)*9 + (dy+1)*3 + (dx+1); b: (cout,)."""
+        w = np.ascontiguousarray(w, np.float32)
+        b = np.ascontiguousarray(b, np.float32)
+        self._check(lib().lat_corrector_set_layer(self._h, layer, _pf(w), _pf(b)), "set_layer")
+
+    def get_layer(self, layer):
+        cin, cout = self.SHAPES[layer]
+        w = np.empty((cout, 27, cin), np.float32)
+        b = np.empty(cout, np.float32)
+        self._check(lib().lat_corrector_get_layer(self._h, layer, _pf(w), _pf(b)), "get_layer")
+        return w, b
+
+    def set_norm(self, mean, sd, rsd):
+        arrs = [np.ascontiguousarray(x, np.float64) for x in (mean, sd, rsd)]
+        self._check(lib().lat_corrector_set_norm(self._h, *[_p(a) for a in arrs]), "set_norm")
+
+    def get_norm(self):
+        arrs = [np.empty(4) for _ in range(3)]
+        self._check(lib().lat_corrector_get_norm(self._h, *[_p(a) for a in arrs]), "get_norm")
+        return tuple(arrs)
+
+    def infer(self, lat):
+        out = np.empty((lat.nz, lat.ny, lat.nx, 4), np.float32)
+        self._check(lib().lat_corrector_infer(self._h, lat._h, _pf(out)), "infer")
+        return out
+
+    def gradients(self, lat, target):
+        """target: (nz, ny, nx, 4) float32; returns (loss, [(dW, db) per layer])."""
+        target = np.ascontiguousarray(target, np.float32)
+        total = sum(cout * 27 * cin + cout for cin, cout in self.SHAPES)
+        g = np.empty(total, np.float32)
+        loss = ctypes.c_double()
+        self._check(lib().lat_corrector_gradients(self._h, lat._h, _pf(target), ctypes.byref(loss), _pf(g)),
+                    "gradients")
+        out, off = [], 0
+        for cin, cout in self.SHAPES:
+            nw = cout * 27 * cin
+            out.append((g[off:off + nw].reshape(cout, 27, cin), g[off + nw:off + nw + cout]))
+            off += nw + cout
+        return loss.value, out
+
+    def train(self, lat, dt, coarse_factor=8, npairs=1, lr=1e-3):
+        losses = np.empty(npairs)
+        self._check(lib().lat_corrector_train(self._h, lat._h, ctypes.c_double(dt), coarse_factor, npairs,
+                                              ctypes.c_double(lr), _p(losses)), "train")
+        return losses
+
+    def step(self, lat, dt, coarse_factor=8, nsteps=1, threshold=1e-6):
+        """Corrected coarse steps; returns dict(fallbacks, max_violation, force_evals)."""
+        st = np.zeros(3)
+        self._check(lib().lat_corrector_step(self._h, lat._h, ctypes.c_double(dt), coarse_factor, nsteps,
+                                             ctypes.c_double(threshold), _p(st)), "step")
+        return {"fallbacks": int(st[0]), "max_violation": float(st[1]), "force_evals": float(st[2])}
