@@ -43,9 +43,9 @@ def make_lattice(la, n, precision):
                       time_var=la.COSMIC, model=la.MODEL_TWOFIELD, model_params=(1.0, G2))
 
 
-def init_state(lat, seed=20240601):
+def init_state(lat, seed=20240601, g2=G2):
     """Vacuum fluctuations (gamma = -1/4 fields, +1/4 momenta) on the background."""
-    m2 = [1.0, G2 * PHI0 ** 2]
+    m2 = [1.0, g2 * PHI0 ** 2]
     lat.sample_field(0, 0, PHI0, -0.25, m2[0], MPL, seed, 0)
     lat.sample_field(0, 1, DPHI0, 0.25, m2[0], MPL, seed, 1)
     lat.sample_field(1, 0, 0.0, -0.25, m2[1], MPL, seed, 2)
@@ -151,6 +151,50 @@ def spectrum_error(sa, sb, floor):
     return max(errs)
 
 
+def corrector_leg(la, args):
+    """The learned corrector: train at --corr-train-n on pairs the plain
+    integrator generates (several couplings), then run at --corr-n with
+    coarse steps of 8 dt, each checked against the Friedmann constraint and
+    replaced by 8 Yoshida-4 steps when its change exceeds the threshold.
+    Reports force evaluations against the plain run over the same time."""
+    g2s = [float(x) for x in args.corr_g2.split(",")]
+    cf = 8
+    out = {"metric": "fine-equivalent steps saved by the learned corrector", "coarse_factor": cf,
+           "threshold": args.corr_threshold, "g2": g2s}
+    t0 = time.perf_counter()
+    losses = []
+    with la.Corrector(seed=7) as corr:
+        ntr = args.corr_train_n
+        dt_tr = (10.0 / ntr) / 40.0
+        per = max(1, args.corr_pairs // len(g2s))
+        for q, g2 in enumerate(g2s):
+            with la.Lattice(ntr, nfld=2, length=10.0, model_params=(1.0, g2)) as lat:
+                init_state(lat, seed=1000 + q, g2=g2)
+                lat.symp4(dt_tr, 8)
+                losses.extend(corr.train(lat, dt_tr, cf, npairs=per, lr=args.corr_lr).tolist())
+        out["train"] = {"n": ntr, "pairs": per * len(g2s), "loss_first": float(np.mean(losses[:5])),
+                        "loss_last": float(np.mean(losses[-5:])), "seconds": round(time.perf_counter() - t0, 1)}
+        n = args.corr_n
+        dt = (10.0 / n) / 40.0
+        res = []
+        for q, g2 in enumerate(g2s):
+            with la.Lattice(n, nfld=2, length=10.0, model_params=(1.0, g2)) as lat:
+                init_state(lat, seed=2000 + q, g2=g2)
+                e0 = lat.energies()["resid"]
+                st = corr.step(lat, dt, cf, nsteps=args.corr_steps, threshold=args.corr_threshold)
+                e1 = lat.energies()["resid"]
+            with la.Lattice(n, nfld=2, length=10.0, model_params=(1.0, g2)) as ref:
+                init_state(ref, seed=2000 + q, g2=g2)
+                ref.symp4(dt, cf * args.corr_steps)
+                r1 = ref.energies()["resid"]
+            plain = 3.0 * cf * args.corr_steps
+            res.append({"g2": g2, "fallbacks": st["fallbacks"], "force_evals": st["force_evals"],
+                        "plain_force_evals": plain, "saving": plain / st["force_evals"],
+                        "resid_change_corrected": e1 - e0, "resid_change_plain": r1 - e0})
+        out["eval"] = {"n": n, "coarse_steps": args.corr_steps, "runs": res}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,7 +209,20 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the library's plain 2 GiB device copy (4 launches) to calibrate HBM byte "
                          "counters")
+    ap.add_argument("--corrector", action="store_true", help="only run the learned-corrector leg")
+    ap.add_argument("--corr-train-n", type=int, default=128)
+    ap.add_argument("--corr-n", type=int, default=256)
+    ap.add_argument("--corr-pairs", type=int, default=300)
+    ap.add_argument("--corr-steps", type=int, default=20)
+    ap.add_argument("--corr-lr", type=float, default=1e-3)
+    ap.add_argument("--corr-threshold", type=float, default=1e-6)
+    ap.add_argument("--corr-g2", default="1e4,1e5,1e6")
     args = ap.parse_args()
+    if args.corrector:
+        import lattice_amd as la
+
+        corrector_leg(la, args)
+        return
     if args.copy_calibration:
         # the library's plain copy kernel (k_copy16) over 2 GiB, 1 + 3 launches:
         # per launch it reads and writes exactly 2 GiB

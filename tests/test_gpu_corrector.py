@@ -145,7 +145,8 @@ def test_corrector_training_and_fallback(gpu):
         assert np.all(sd > 0) and np.all(rsd > 0)
         start = lat.get_state()
         st = corr.step(lat, dt, cf, nsteps=2, threshold=0.0)
-        assert st["fallbacks"] == 2 and st["force_evals"] == 2 * 3 * cf
+        # each step: the rejected coarse attempt (1 force evaluation) + cf Yoshida-4 steps
+        assert st["fallbacks"] == 2 and st["force_evals"] == 2 * (1 + 3 * cf)
         got = lat.get_state()
         st2 = corr.step(lat, dt, cf, nsteps=2, threshold=1.0)
         assert st2["fallbacks"] == 0 and st2["force_evals"] == 2
