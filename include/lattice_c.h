@@ -233,16 +233,21 @@ int  lat_restore(lat_ctx *ctx, const char *dir, double *time, int *outframenum);
  * Contexts: single, two-field, FP64, cubic with n a multiple of 16.
  * Weights of layer l: cout x 27 cin with k = tap * cin + ci, tap =
  * (dz+1)*9 + (dy+1)*3 + (dx+1) for the neighbour at (x+dx, y+dy, z+dz);
- * channels (phi, chi, pi_phi, pi_chi); normalisation input (v - mean) / sd,
- * output residual / rsd per channel. */
+ * channels (phi, chi, pi_phi, pi_chi).  The network sees and corrects
+ * fluctuations only, at unit scale: input (v - <v>) / s_v, target
+ * (r - <r>) / (ratio s_v) per channel, with <.> the spatial mean and s_v the
+ * spatial standard deviation of the coarse state at hand; a corrected step
+ * adds ratio s_v (y - <y>), leaving the homogeneous mode at the coarse step's
+ * value.  ratio[4] (residual fluctuation RMS / s_v) is fixed at the first
+ * training pair, or by set_norm. */
 typedef struct lat_corrector lat_corrector;
 int  lat_corrector_create(unsigned long long seed, lat_corrector **out);
 void lat_corrector_destroy(lat_corrector *corr);
 const char *lat_corrector_last_error(const lat_corrector *corr);
 int  lat_corrector_set_layer(lat_corrector *corr, int layer, const float *w, const float *b);
 int  lat_corrector_get_layer(lat_corrector *corr, int layer, float *w, float *b);
-int  lat_corrector_set_norm(lat_corrector *corr, const double *mean, const double *sd, const double *rsd);
-int  lat_corrector_get_norm(const lat_corrector *corr, double *mean, double *sd, double *rsd);
+int  lat_corrector_set_norm(lat_corrector *corr, const double *ratio);
+int  lat_corrector_get_norm(const lat_corrector *corr, double *ratio);
 /* normalised network output [site][4] for the current state of ctx */
 int  lat_corrector_infer(lat_corrector *corr, lat_ctx *ctx, float *out);
 /* loss (MSE) and gradients (per layer: weights then biases) against target [site][4] */

@@ -66,6 +66,19 @@ constexpr int ns_at(int n, int p, int mr = 16)
     return ns;
 }
 constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
+// The same radices in reverse order (REV): a transform whose first radix
+// equals the last radix of the forward schedule, so the forward transform's
+// last pass and the inverse transform's first pass can share registers.
+constexpr int sched_radix(int n, int p, int mr, bool rev)
+{
+    return rev ? radix_at(n, num_passes(n, mr) - 1 - p, mr) : radix_at(n, p, mr);
+}
+constexpr int sched_ns(int n, int p, int mr, bool rev)
+{
+    int ns = 1;
+    for (int i = 0; i < p; ++i) ns *= sched_radix(n, i, mr, rev);
+    return ns;
+}
 
 // ---------------------------------------------------------------- radix kernels
 // multiply by e^{SIGN 2 pi i m / 16} for a compile-time m
@@ -254,24 +267,27 @@ __device__ __forceinline__ void lds_fft(cpx<T> *buf, int lstr, int estr, const c
     }
 }
 
-// Passes [P, PEND) in LDS (schedule with max radix MR).
-template <typename T, int N, int SIGN, int L, int TPB, bool COLMAJOR, int TWS, int P, int PEND, int MR = 16>
+// Passes [P, PEND) in LDS (schedule with max radix MR, reversed with REV).
+template <typename T, int N, int SIGN, int L, int TPB, bool COLMAJOR, int TWS, int P, int PEND, int MR = 16,
+          bool REV = false>
 __device__ __forceinline__ void lds_fft_range(cpx<T> *buf, int lstr, int estr, const cpx<T> *__restrict__ tw)
 {
     if constexpr (P < PEND) {
-        stockham_pass<T, N, radix_at(N, P, MR), ns_at(N, P, MR), SIGN, L, TPB, COLMAJOR, TWS>(buf, lstr, estr, tw);
-        lds_fft_range<T, N, SIGN, L, TPB, COLMAJOR, TWS, P + 1, PEND, MR>(buf, lstr, estr, tw);
+        stockham_pass<T, N, sched_radix(N, P, MR, REV), sched_ns(N, P, MR, REV), SIGN, L, TPB, COLMAJOR, TWS>(
+            buf, lstr, estr, tw);
+        lds_fft_range<T, N, SIGN, L, TPB, COLMAJOR, TWS, P + 1, PEND, MR, REV>(buf, lstr, estr, tw);
     }
 }
 
 // Pass P read from LDS, result handed to st(line, position, value) instead of
 // being written back (used for the last pass: results go straight to HBM).
 // No trailing barrier: the caller synchronises before LDS is reused.
-template <typename T, int N, int P, int SIGN, int L, int TPB, bool COLMAJOR, int TWS, int MR, typename Store>
+template <typename T, int N, int P, int SIGN, int L, int TPB, bool COLMAJOR, int TWS, int MR, bool REV = false,
+          typename Store>
 __device__ __forceinline__ void stockham_pass_out(const cpx<T> *buf, int lstr, int estr,
                                                   const cpx<T> *__restrict__ tw, Store st)
 {
-    constexpr int R = radix_at(N, P, MR), NS = ns_at(N, P, MR);
+    constexpr int R = sched_radix(N, P, MR, REV), NS = sched_ns(N, P, MR, REV);
     constexpr int NR = N / R;
     constexpr int TOTAL = L * NR;
     constexpr int BPT = cdiv(TOTAL, TPB);
@@ -303,6 +319,55 @@ __device__ __forceinline__ void stockham_pass_out(const cpx<T> *buf, int lstr, i
             for (int q = 0; q < R; ++q) st(line, o + q * NS, v[q]);
         }
     }
+}
+
+// The last pass of a forward transform (schedule MR), a pointwise factor
+// mul(line, n) on its natural-order outputs n, and the first pass of the
+// inverse transform in the reversed schedule, without the LDS round trip in
+// between: the last forward pass of butterfly (line, j) yields the elements
+// n = j + q N/R, which are exactly the inputs of inverse butterfly (line, j)
+// (first pass, radix R, no twiddles).  Column-major lines (ESTR = L).
+template <typename T, int N, int L, int TPB, int MR, typename Mul>
+__device__ __forceinline__ void fused_last_first(cpx<T> *buf, const cpx<T> *__restrict__ tw, Mul mul)
+{
+    constexpr int NP = num_passes(N, MR);
+    constexpr int R = radix_at(N, NP - 1, MR), NS = ns_at(N, NP - 1, MR);
+    static_assert(NS * R == N, "last pass spans the line");
+    constexpr int TOTAL = L * NS;
+    constexpr int BPT = cdiv(TOTAL, TPB);
+    cpx<T> v[BPT][R];
+    int pos[BPT];
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+        const int g = threadIdx.x + b * TPB;
+        if (TOTAL % TPB == 0 || g < TOTAL) {
+            const int line = g % L, j = g / L;
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                cpx<T> a = buf[(j + q * NS) * L + line];
+                if (q > 0 && NS > 1) a = cmul(a, tw[(j * q) % N]);
+                v[b][q] = a;
+            }
+            dft_small<R, -1>(v[b]);
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const T m = mul(line, j + q * NS);
+                v[b][q] = {v[b][q].x * m, v[b][q].y * m};
+            }
+            dft_small<R, +1>(v[b]);
+            pos[b] = j * R * L + line;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+        const int g = threadIdx.x + b * TPB;
+        if (TOTAL % TPB == 0 || g < TOTAL) {
+#pragma unroll
+            for (int q = 0; q < R; ++q) buf[pos[b] + q * L] = v[b][q];
+        }
+    }
+    __syncthreads();
 }
 
 } // namespace lat

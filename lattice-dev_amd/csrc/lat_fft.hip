@@ -191,8 +191,11 @@ __global__ __launch_bounds__(TPB) void k_x_r2c(T *__restrict__ f, const T *__res
 
 // ---------------------------------------------------------------- y / z columns
 // MODE 0 forward, 1 inverse, 2 forward * (-k^2 kfac) inverse (z axis only).
+// FUSE (MODE 2): the last forward pass, the -k^2 factor and the first pass of
+// the inverse (radices in reverse order) run in registers in one step, which
+// saves one LDS write + read of the tile and a barrier pair.
 template <typename T, int N, int TC, int MODE, int MR = 16, bool PF = true, int MINW = 1, int NT = TPB,
-          bool TWL = false>
+          bool TWL = false, bool FUSE = false>
 __global__ __launch_bounds__(NT, MINW) void k_col(cpx<T> *__restrict__ Fk, const cpx<T> *__restrict__ tw, int nxh,
                                              int n_other, size_t s_axis, size_t s_other, size_t s_arr,
                                              int n_arr, int ny, double kfac, size_t ms_k)
@@ -258,7 +261,19 @@ __global__ __launch_bounds__(NT, MINW) void k_col(cpx<T> *__restrict__ Fk, const
         auto store = [&](int line, int pos, cpx<T> a) {
             if (line < ncol) base[(size_t)pos * s_axis + line] = a;
         };
-        if constexpr (MODE == 2) {
+        if constexpr (MODE == 2 && FUSE && NP >= 2) {
+            lds_fft_range<T, N, -1, TC, NT, true, 1, 1, NP - 1, MR>(buf, 1, TC, twp);
+            const int jj = wmag(o, ny);
+            auto mul = [&](int c, int n) -> T {
+                const int ii = ct * TC + c;
+                const int kk = wmag(n, N);
+                const double rad2 = double(ii * ii) + double(jj * jj) + double(kk * kk);
+                return T(-rad2 * kfac);
+            };
+            fused_last_first<T, N, TC, NT, MR>(buf, twp, mul);
+            lds_fft_range<T, N, +1, TC, NT, true, 1, 1, NP - 1, MR, true>(buf, 1, TC, twp);
+            stockham_pass_out<T, N, NP - 1, +1, TC, NT, true, 1, MR, true>(buf, 1, TC, twp, store);
+        } else if constexpr (MODE == 2) {
             lds_fft_range<T, N, -1, TC, NT, true, 1, 1, NP, MR>(buf, 1, TC, twp);
             const int jj = wmag(o, ny);
             for (int idx = threadIdx.x; idx < TC * N; idx += NT) {
@@ -513,14 +528,20 @@ constexpr int c2r_lines() { return clampi(1024 / (N / 2), 1, 256); }
 template <typename T>
 constexpr int r2c_maxradix() { return sizeof(T) == 8 ? 4 : 8; }
 
-// y/z column passes: TC adjacent kx columns per tile, 512 threads
-constexpr int COL_NT = 512;
+// y/z column passes: TC adjacent kx columns per tile.  y passes: 8 columns
+// at N = 1024 (128 B rows in FP64), radix <= 16, 512 threads.  Fused z pass:
+// radix <= 16 with the last forward / first inverse pass in registers
+// (FUSE); FP32 8 columns, 512 threads; FP64 4 columns, 256 threads, with the
+// next tile's first-pass loads in flight during the LDS passes (PF).
 template <typename T, int N, int MODE>
 struct ColCfg {
-    // y passes: 8 columns at N = 1024 (128 B rows in FP64); fused z pass: 4
-    static constexpr int TC = MODE == 2 ? clampi(4096 / N, 4, 64) : clampi(8192 / N, 8, 64);
-    static constexpr int MR = MODE == 2 ? 8 : 16;
-    static constexpr bool PF = MODE == 2 && sizeof(T) == 8;
+    static constexpr bool Z = MODE == 2;
+    static constexpr bool F64 = sizeof(T) == 8;
+    static constexpr int TC = (Z && F64) ? clampi(4096 / N, 4, 64) : clampi(8192 / N, 8, 64);
+    static constexpr int MR = 16;
+    static constexpr bool PF = Z && F64;
+    static constexpr int NT = (Z && F64) ? 256 : 512;
+    static constexpr bool FUSE = Z;
 };
 
 #define LAT_FOR_SIZES(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(48) X(96) X(192)
@@ -535,11 +556,9 @@ constexpr int col_tc(int mode, int v) { return v ? clampi((mode == 2 ? ColCfg<T,
                                               : (mode == 2 ? ColCfg<T, N, 2>::TC : ColCfg<T, N, 0>::TC); }
 
 // Twiddles in LDS (TWL) where the 1024^3 sweeps measured a gain: the x C2R
-// (both precisions), the FP32 x R2C, the y passes and the FP64 z pass.
+// (both precisions), the FP32 x R2C, the y passes and the fused z pass.
 template <typename T>
 constexpr bool r2c_twl() { return sizeof(T) == 4; }
-template <typename T, int MODE>
-constexpr bool col_twl() { return MODE != 2 || sizeof(T) == 8; }
 
 template <typename T, int N, int L, int MR>
 static void r2c_go(Ctx &c, const cpx<T> *tw, T *f, const T *pi, cpx<T> *Fk, int drift, int field)
@@ -590,8 +609,8 @@ static void col_go(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int n_other, size_t s_a
 {
     using C = ColCfg<T, N, MODE == 2 ? 2 : 0>;
     const int g = grid_for(((c.nxh + TC - 1) / TC) * n_other * n_arrays);
-    hipLaunchKernelGGL((k_col<T, N, TC, MODE, C::MR, C::PF, 1, COL_NT, col_twl<T, MODE>()>), dim3(g, c.B),
-                       dim3(COL_NT), 0, c.stream, Fk, tw, c.nxh, n_other, s_axis, s_other, c.nk, n_arrays, c.ny, kfac,
+    hipLaunchKernelGGL((k_col<T, N, TC, MODE, C::MR, C::PF, 1, C::NT, true, C::FUSE>), dim3(g, c.B),
+                       dim3(C::NT), 0, c.stream, Fk, tw, c.nxh, n_other, s_axis, s_other, c.nk, n_arrays, c.ny, kfac,
                        c.ms.k);
 }
 

@@ -247,6 +247,7 @@ constexpr int NL = 4;             // layers
 constexpr int WID = 32;           // hidden channels
 constexpr int CINS[NL] = {4, WID, WID, WID};
 constexpr int COUTS[NL] = {WID, WID, WID, 4};
+constexpr int CM_BLOCKS = 512;    // blocks of the channel-mean reduction
 
 struct Layer {
     int cin, cout, kp, mp, kpt, mpt; // padded shapes of W [mp][kp] and Wt [mpt][kpt]
@@ -258,14 +259,24 @@ struct Layer {
 struct Net {
     Layer L[NL];
     int step = 0;
-    // normalisation: input (v - mean) / sd, target residual / rsd, per channel
-    double mean[4] = {0, 0, 0, 0}, sd[4] = {1, 1, 1, 1}, rsd[4] = {1, 1, 1, 1};
+    // normalisation per channel, from the snapshot at hand (computed on the
+    // GPU each time): input (v - <v>) / s_v, target (r - <r>) / (ratio s_v),
+    // with <.> the spatial mean and s_v the spatial standard deviation of the
+    // coarse state.  The network sees and corrects fluctuations only, at
+    // unit scale, so neither the drift of the homogeneous mode nor the growth
+    // of the fluctuations over a run leaves the range it was trained on.
+    // ratio = RMS of the residual fluctuation / s_v, fixed at the first pair.
+    double ratio[4] = {1, 1, 1, 1};
     bool stats = false;
     // activation buffers for lattice size n
     int n = 0;
     bf16 *x0 = nullptr, *a[3] = {nullptr, nullptr, nullptr}, *z[3] = {nullptr, nullptr, nullptr};
     bf16 *g4 = nullptr, *gA = nullptr, *gB = nullptr;
     float *y = nullptr, *tgt = nullptr, *part = nullptr, *lossp = nullptr;
+    // channel statistics: [0,4) input mean, [4,8) residual mean, [8,12)
+    // output mean, [12,16) input standard deviation, [16,20) residual
+    // standard deviation (first training pair); block partials
+    double *cm = nullptr, *cpart = nullptr;
     void *save_f = nullptr, *save_p = nullptr, *coarse_f = nullptr, *coarse_p = nullptr;
     DevScalars *save_s = nullptr;
     hipStream_t stream = nullptr;
@@ -298,12 +309,13 @@ static void free_buffers(Net &net)
 {
     void *ps[] = {net.x0, net.a[0], net.a[1], net.a[2], net.z[0], net.z[1], net.z[2], net.g4, net.gA, net.gB,
                   net.y, net.tgt, net.part, net.lossp, net.save_f, net.save_p, net.coarse_f, net.coarse_p,
-                  net.save_s};
+                  net.save_s, net.cm, net.cpart};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     net.x0 = net.g4 = net.gA = net.gB = nullptr;
     for (int q = 0; q < 3; ++q) net.a[q] = net.z[q] = nullptr;
     net.y = net.tgt = net.part = net.lossp = nullptr;
+    net.cm = net.cpart = nullptr;
     net.save_f = net.save_p = net.coarse_f = net.coarse_p = nullptr;
     net.save_s = nullptr;
     net.n = 0;
@@ -334,6 +346,8 @@ static int ensure_buffers(Net &net, const Ctx &c)
     }
     rc |= alloc(net, &net.part, maxpart * nchunks(nv), "dW partials");
     rc |= alloc(net, &net.lossp, 1024, "loss partials");
+    rc |= alloc(net, &net.cm, 20, "channel statistics");
+    rc |= alloc(net, &net.cpart, 4 * CM_BLOCKS, "channel-mean partials");
     const size_t es = elem_size(c);
     rc |= alloc(net, (char **)&net.save_f, es * c.ms.f, "saved fields");
     rc |= alloc(net, (char **)&net.save_p, es * c.ms.f, "saved momenta");
@@ -349,10 +363,76 @@ static int ensure_buffers(Net &net, const Ctx &c)
 }
 
 // ---------------------------------------------------------------- data kernels
-// X0[s][c] = (v_c(s) - mean_c) / sd_c for (phi, chi, pi_phi, pi_chi)
-__global__ __launch_bounds__(256) void k_pack(const double *__restrict__ f, const double *__restrict__ p,
-                                              size_t nvol, double4 mean, double4 isd, bf16 *__restrict__ X)
+// Spatial channel sums, deterministic: CM_BLOCKS fixed blocks write partials,
+// one block adds them in block order.  Source: the FP64 state (f, p), minus
+// (cf, cp) when given, or the network output y [site][4]; with sub, the sums
+// are of (v - sub)^2 instead (second pass of a standard deviation).
+__global__ __launch_bounds__(256) void k_chsum(const double *__restrict__ f, const double *__restrict__ p,
+                                               const double *__restrict__ cf, const double *__restrict__ cp,
+                                               const float *__restrict__ y, const double *__restrict__ sub,
+                                               size_t nvol, double *__restrict__ part)
 {
+    double a[4] = {0, 0, 0, 0};
+    double m[4] = {0, 0, 0, 0};
+    if (sub) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) m[q] = sub[q];
+    }
+    for (size_t s = blockIdx.x * (size_t)256 + threadIdx.x; s < nvol; s += (size_t)gridDim.x * 256) {
+        double v[4];
+        if (y) {
+            const float4 t = *reinterpret_cast<const float4 *>(y + s * 4);
+            v[0] = t.x;
+            v[1] = t.y;
+            v[2] = t.z;
+            v[3] = t.w;
+        } else {
+            v[0] = cf ? f[s] - cf[s] : f[s];
+            v[1] = cf ? f[nvol + s] - cf[nvol + s] : f[nvol + s];
+            v[2] = cp ? p[s] - cp[s] : p[s];
+            v[3] = cp ? p[nvol + s] - cp[nvol + s] : p[nvol + s];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a[q] += sub ? (v[q] - m[q]) * (v[q] - m[q]) : v[q];
+    }
+    __shared__ double red[4][256];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = a[q];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) part[blockIdx.x * 4 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// out[c] = sum / nvol, or with root the standard deviation sqrt(sum / nvol)
+// (1 where it is 0: a channel without fluctuations is left unscaled)
+__global__ __launch_bounds__(64) void k_chmean(const double *__restrict__ part, int nb, size_t nvol, int root,
+                                               double *__restrict__ out)
+{
+    if (threadIdx.x < 4) {
+        double t = 0;
+        for (int b = 0; b < nb; ++b) t += part[b * 4 + threadIdx.x];
+        t /= (double)nvol;
+        if (root) {
+            t = sqrt(t);
+            if (!(t > 0.0)) t = 1.0;
+        }
+        out[threadIdx.x] = t;
+    }
+}
+
+// X0[s][c] = (v_c(s) - <v_c>) / s_c for (phi, chi, pi_phi, pi_chi); st =
+// channel statistics (means at 0, standard deviations at 12)
+__global__ __launch_bounds__(256) void k_pack(const double *__restrict__ f, const double *__restrict__ p,
+                                              size_t nvol, const double *__restrict__ st, bf16 *__restrict__ X)
+{
+    const double4 mean = {st[0], st[1], st[2], st[3]};
+    const double4 isd = {1.0 / st[12], 1.0 / st[13], 1.0 / st[14], 1.0 / st[15]};
     for (size_t s = blockIdx.x * (size_t)256 + threadIdx.x; s < nvol; s += (size_t)gridDim.x * 256) {
         bf16x4 o;
         o[0] = (short)to_bf16((float)((f[s] - mean.x) * isd.x));
@@ -363,17 +443,21 @@ __global__ __launch_bounds__(256) void k_pack(const double *__restrict__ f, cons
     }
 }
 
-// tgt[s][c] = (fine_c - coarse_c) / rsd_c
+// tgt[s][c] = (r_c(s) - <r_c>) / (ratio_c s_c) with r = fine - coarse
 __global__ __launch_bounds__(256) void k_target(const double *__restrict__ ff, const double *__restrict__ fp,
                                                 const double *__restrict__ cf, const double *__restrict__ cp,
-                                                size_t nvol, double4 irsd, float *__restrict__ tgt)
+                                                size_t nvol, const double *__restrict__ st, double4 ratio,
+                                                float *__restrict__ tgt)
 {
+    const double4 m = {st[4], st[5], st[6], st[7]};
+    const double4 irsd = {1.0 / (ratio.x * st[12]), 1.0 / (ratio.y * st[13]), 1.0 / (ratio.z * st[14]),
+                          1.0 / (ratio.w * st[15])};
     for (size_t s = blockIdx.x * (size_t)256 + threadIdx.x; s < nvol; s += (size_t)gridDim.x * 256) {
         float4 o;
-        o.x = (float)((ff[s] - cf[s]) * irsd.x);
-        o.y = (float)((ff[nvol + s] - cf[nvol + s]) * irsd.y);
-        o.z = (float)((fp[s] - cp[s]) * irsd.z);
-        o.w = (float)((fp[nvol + s] - cp[nvol + s]) * irsd.w);
+        o.x = (float)((ff[s] - cf[s] - m.x) * irsd.x);
+        o.y = (float)((ff[nvol + s] - cf[nvol + s] - m.y) * irsd.y);
+        o.z = (float)((fp[s] - cp[s] - m.z) * irsd.z);
+        o.w = (float)((fp[nvol + s] - cp[nvol + s] - m.w) * irsd.w);
         *reinterpret_cast<float4 *>(tgt + s * 4) = o;
     }
 }
@@ -399,16 +483,20 @@ __global__ __launch_bounds__(256) void k_loss(const float *__restrict__ y, const
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-// f_c += rsd_c * y_c (apply the predicted residual)
+// f_c += ratio_c s_c (y_c - <y_c>): the predicted residual fluctuation; the
+// homogeneous mode keeps the coarse step's value
 __global__ __launch_bounds__(256) void k_apply(double *__restrict__ f, double *__restrict__ p,
-                                               const float *__restrict__ y, size_t nvol, double4 rsd)
+                                               const float *__restrict__ y, size_t nvol,
+                                               const double *__restrict__ st, double4 ratio)
 {
+    const double4 m = {st[8], st[9], st[10], st[11]};
+    const double4 rsd = {ratio.x * st[12], ratio.y * st[13], ratio.z * st[14], ratio.w * st[15]};
     for (size_t s = blockIdx.x * (size_t)256 + threadIdx.x; s < nvol; s += (size_t)gridDim.x * 256) {
         const float4 v = *reinterpret_cast<const float4 *>(y + s * 4);
-        f[s] += rsd.x * v.x;
-        f[nvol + s] += rsd.y * v.y;
-        p[s] += rsd.z * v.z;
-        p[nvol + s] += rsd.w * v.w;
+        f[s] += rsd.x * (v.x - m.x);
+        f[nvol + s] += rsd.y * (v.y - m.y);
+        p[s] += rsd.z * (v.z - m.z);
+        p[nvol + s] += rsd.w * (v.w - m.w);
     }
 }
 
@@ -445,11 +533,16 @@ static void weight_grad(Net &net, Layer &l, const bf16 *G, const bf16 *X)
                        l.grad);
 }
 
-// gradients of all layers for the current x0 / tgt; returns after launching
-static void backward(Net &net)
+// output gradient G4 and the loss partials for the current y / tgt
+static void loss_grad(Net &net)
 {
     const size_t nv = (size_t)net.n * net.n * net.n;
     hipLaunchKernelGGL(k_loss, dim3(1024), dim3(256), 0, net.stream, net.y, net.tgt, nv, net.g4, net.lossp);
+}
+
+// gradients of all layers from G4; returns after launching
+static void backward_layers(Net &net)
+{
     weight_grad<WID, 4>(net, net.L[3], net.g4, net.a[2]);
     conv<4, WID, CONV_DSILU>(net, net.g4, net.L[3].wt, nullptr, net.gA, nullptr, nullptr, net.z[2]);
     weight_grad<WID, WID>(net, net.L[2], net.gA, net.a[1]);
@@ -457,6 +550,12 @@ static void backward(Net &net)
     weight_grad<WID, WID>(net, net.L[1], net.gB, net.a[0]);
     conv<WID, WID, CONV_DSILU>(net, net.gB, net.L[1].wt, nullptr, net.gA, nullptr, nullptr, net.z[0]);
     weight_grad<4, WID>(net, net.L[0], net.gA, net.x0);
+}
+
+static void backward(Net &net)
+{
+    loss_grad(net);
+    backward_layers(net);
 }
 
 template <int CIN, int COUT>
@@ -530,7 +629,6 @@ static int check_ctx(Net &net, Ctx &c)
 }
 
 static double4 d4(const double *v) { return double4{v[0], v[1], v[2], v[3]}; }
-static double4 inv4(const double *v) { return double4{1.0 / v[0], 1.0 / v[1], 1.0 / v[2], 1.0 / v[3]}; }
 
 static int grid_sites(size_t nvol)
 {
@@ -538,10 +636,26 @@ static int grid_sites(size_t nvol)
     return (int)(g > 8192 ? 8192 : g);
 }
 
-static void pack(Net &net, Ctx &c)
+// channel means into net.cm + 4 * slot (see Net::cm)
+static void channel_means(Net &net, Ctx &c, const void *f, const void *p, const void *cf, const void *cp,
+                          const float *y, int slot)
 {
-    hipLaunchKernelGGL(k_pack, dim3(grid_sites(c.nvol)), dim3(256), 0, c.stream, (const double *)c.fld,
-                       (const double *)c.fldp, c.nvol, d4(net.mean), inv4(net.sd), net.x0);
+    hipLaunchKernelGGL(k_chsum, dim3(CM_BLOCKS), dim3(256), 0, c.stream, (const double *)f, (const double *)p,
+                       (const double *)cf, (const double *)cp, y, (const double *)nullptr, c.nvol, net.cpart);
+    hipLaunchKernelGGL(k_chmean, dim3(1), dim3(64), 0, c.stream, net.cpart, CM_BLOCKS, c.nvol, 0,
+                       net.cm + 4 * slot);
+}
+
+// input means (slot 0) and standard deviations (slot 3) of the state (f, p), then X0
+static void pack(Net &net, Ctx &c, const void *f, const void *p)
+{
+    channel_means(net, c, f, p, nullptr, nullptr, nullptr, 0);
+    hipLaunchKernelGGL(k_chsum, dim3(CM_BLOCKS), dim3(256), 0, c.stream, (const double *)f, (const double *)p,
+                       (const double *)nullptr, (const double *)nullptr, (const float *)nullptr,
+                       (const double *)net.cm, c.nvol, net.cpart);
+    hipLaunchKernelGGL(k_chmean, dim3(1), dim3(64), 0, c.stream, net.cpart, CM_BLOCKS, c.nvol, 1, net.cm + 12);
+    hipLaunchKernelGGL(k_pack, dim3(grid_sites(c.nvol)), dim3(256), 0, c.stream, (const double *)f,
+                       (const double *)p, c.nvol, (const double *)net.cm, net.x0);
 }
 
 static int save_state(Net &net, Ctx &c)
@@ -560,48 +674,6 @@ static int restore_state(Net &net, Ctx &c)
     LAT_HIP(&c, hipMemcpyAsync(c.fldp, net.save_p, bytes, hipMemcpyDeviceToDevice, c.stream));
     LAT_HIP(&c, hipMemcpyAsync(c.dsc, net.save_s, sizeof(DevScalars), hipMemcpyDeviceToDevice, c.stream));
     c.ke_valid = false;
-    return 0;
-}
-
-// per-channel mean and standard deviation of (phi, chi, pi_phi, pi_chi) of two device state copies (host pass)
-static int channel_stats(Ctx &c, const void *f, const void *p, double *mean, double *sd)
-{
-    const size_t nv = c.nvol;
-    std::vector<double> hf(2 * nv), hp(2 * nv);
-    LAT_HIP(&c, hipStreamSynchronize(c.stream));
-    LAT_HIP(&c, hipMemcpy(hf.data(), f, sizeof(double) * 2 * nv, hipMemcpyDeviceToHost));
-    LAT_HIP(&c, hipMemcpy(hp.data(), p, sizeof(double) * 2 * nv, hipMemcpyDeviceToHost));
-    for (int ch = 0; ch < 4; ++ch) {
-        const double *v = (ch < 2 ? hf.data() : hp.data()) + (size_t)(ch & 1) * nv;
-        double m = 0, q = 0;
-        for (size_t s = 0; s < nv; ++s) m += v[s];
-        m /= (double)nv;
-        for (size_t s = 0; s < nv; ++s) q += (v[s] - m) * (v[s] - m);
-        mean[ch] = m;
-        sd[ch] = std::sqrt(q / (double)nv);
-        if (!(sd[ch] > 0.0)) sd[ch] = 1.0;
-    }
-    return 0;
-}
-
-// residual scale: RMS of (fine - coarse) per channel
-static int residual_stats(Ctx &c, const void *ff, const void *fp, const void *cf, const void *cp, double *rsd)
-{
-    const size_t nv = c.nvol;
-    std::vector<double> a(2 * nv), b(2 * nv), d(2 * nv), e(2 * nv);
-    LAT_HIP(&c, hipStreamSynchronize(c.stream));
-    LAT_HIP(&c, hipMemcpy(a.data(), ff, sizeof(double) * 2 * nv, hipMemcpyDeviceToHost));
-    LAT_HIP(&c, hipMemcpy(b.data(), fp, sizeof(double) * 2 * nv, hipMemcpyDeviceToHost));
-    LAT_HIP(&c, hipMemcpy(d.data(), cf, sizeof(double) * 2 * nv, hipMemcpyDeviceToHost));
-    LAT_HIP(&c, hipMemcpy(e.data(), cp, sizeof(double) * 2 * nv, hipMemcpyDeviceToHost));
-    for (int ch = 0; ch < 4; ++ch) {
-        const double *x = (ch < 2 ? a.data() : b.data()) + (size_t)(ch & 1) * nv;
-        const double *y = (ch < 2 ? d.data() : e.data()) + (size_t)(ch & 1) * nv;
-        double q = 0;
-        for (size_t s = 0; s < nv; ++s) q += (x[s] - y[s]) * (x[s] - y[s]);
-        rsd[ch] = std::sqrt(q / (double)nv);
-        if (!(rsd[ch] > 0.0)) rsd[ch] = 1.0;
-    }
     return 0;
 }
 
@@ -685,26 +757,22 @@ int lat_corrector_get_layer(lat_corrector *h, int layer, float *w, float *b)
     return 0;
 }
 
-int lat_corrector_set_norm(lat_corrector *h, const double *mean, const double *sd, const double *rsd)
+int lat_corrector_set_norm(lat_corrector *h, const double *ratio)
 {
-    if (!h || !mean || !sd || !rsd) return -1;
+    if (!h || !ratio) return -1;
     for (int q = 0; q < 4; ++q) {
-        if (!(sd[q] > 0.0) || !(rsd[q] > 0.0)) return -1;
-        h->net.mean[q] = mean[q];
-        h->net.sd[q] = sd[q];
-        h->net.rsd[q] = rsd[q];
+        if (!(ratio[q] > 0.0)) return -1;
+        h->net.ratio[q] = ratio[q];
     }
     h->net.stats = true;
     return 0;
 }
 
-int lat_corrector_get_norm(const lat_corrector *h, double *mean, double *sd, double *rsd)
+int lat_corrector_get_norm(const lat_corrector *h, double *ratio)
 {
-    if (!h || !mean || !sd || !rsd) return -1;
+    if (!h || !ratio) return -1;
     for (int q = 0; q < 4; ++q) {
-        mean[q] = h->net.mean[q];
-        sd[q] = h->net.sd[q];
-        rsd[q] = h->net.rsd[q];
+        ratio[q] = h->net.ratio[q];
     }
     return 0;
 }
@@ -717,7 +785,7 @@ int lat_corrector_infer(lat_corrector *h, lat_ctx *ctx, float *out)
     Ctx &c = ctx->c;
     int rc = check_ctx(net, c);
     if (rc) return rc;
-    pack(net, c);
+    pack(net, c, c.fld, c.fldp);
     forward(net);
     LAT_HIP(&c, hipMemcpyAsync(out, net.y, sizeof(float) * 4 * c.nvol, hipMemcpyDeviceToHost, c.stream));
     LAT_HIP(&c, hipStreamSynchronize(c.stream));
@@ -736,7 +804,7 @@ int lat_corrector_gradients(lat_corrector *h, lat_ctx *ctx, const float *target,
     int rc = check_ctx(net, c);
     if (rc) return rc;
     LAT_HIP(&c, hipMemcpyAsync(net.tgt, target, sizeof(float) * 4 * c.nvol, hipMemcpyHostToDevice, c.stream));
-    pack(net, c);
+    pack(net, c, c.fld, c.fldp);
     forward(net);
     backward(net);
     std::vector<float> lp(1024);
@@ -761,8 +829,11 @@ int lat_corrector_gradients(lat_corrector *h, lat_ctx *ctx, const float *target,
 // Train on npairs snapshot pairs generated along the trajectory of ctx by the
 // plain integrator: from the current state S, coarse = one leapfrog step of
 // coarse_factor * dt, fine = coarse_factor Yoshida-4 steps of dt; input =
-// coarse, target = (fine - coarse) / rsd; one Adam step per pair; the
-// trajectory continues from the fine state.  losses (nullable): per pair.
+// coarse, target = its residual fluctuation over (ratio x the coarse
+// state's standard deviation), ratio fixed at the first pair of a fresh
+// corrector; one Adam step per pair; the
+// trajectory continues from the fine state.  losses (nullable): per pair,
+// NaN for a pair whose coarse step was not finite (no update from it).
 int lat_corrector_train(lat_corrector *h, lat_ctx *ctx, double dt, int coarse_factor, int npairs, double lr,
                         double *losses)
 {
@@ -779,27 +850,53 @@ int lat_corrector_train(lat_corrector *h, lat_ctx *ctx, double dt, int coarse_fa
         LAT_HIP(&c, hipMemcpyAsync(net.coarse_p, c.fldp, bytes, hipMemcpyDeviceToDevice, c.stream));
         if ((rc = restore_state(net, c))) return rc;
         if ((rc = lat_symp(ctx, 4, dt, coarse_factor))) return rc;
+        pack(net, c, net.coarse_f, net.coarse_p);
+        channel_means(net, c, c.fld, c.fldp, net.coarse_f, net.coarse_p, nullptr, 1);
+        bool usable = true;
         if (!net.stats) {
-            if ((rc = channel_stats(c, net.coarse_f, net.coarse_p, net.mean, net.sd))) return rc;
-            if ((rc = residual_stats(c, c.fld, c.fldp, net.coarse_f, net.coarse_p, net.rsd))) return rc;
-            net.stats = true;
+            // ratio = residual fluctuation RMS / input standard deviation
+            hipLaunchKernelGGL(k_chsum, dim3(CM_BLOCKS), dim3(256), 0, c.stream, (const double *)c.fld,
+                               (const double *)c.fldp, (const double *)net.coarse_f, (const double *)net.coarse_p,
+                               (const float *)nullptr, (const double *)(net.cm + 4), c.nvol, net.cpart);
+            hipLaunchKernelGGL(k_chmean, dim3(1), dim3(64), 0, c.stream, net.cpart, CM_BLOCKS, c.nvol, 1,
+                               net.cm + 16);
+            double st[20];
+            LAT_HIP(&c, hipMemcpyAsync(st, net.cm, sizeof(st), hipMemcpyDeviceToHost, c.stream));
+            LAT_HIP(&c, hipStreamSynchronize(c.stream));
+            double r[4];
+            for (int q = 0; q < 4; ++q) {
+                r[q] = st[16 + q] / st[12 + q];
+                // a non-finite coarse step (beyond the leapfrog's stability
+                // limit) gives no usable scale: wait for a usable pair
+                if (!std::isfinite(r[q]) || !std::isfinite(st[q]) || !std::isfinite(st[4 + q])) usable = false;
+                if (!(r[q] > 0.0)) r[q] = 1.0;
+            }
+            if (usable) {
+                for (int q = 0; q < 4; ++q) net.ratio[q] = r[q];
+                net.stats = true;
+            }
         }
-        hipLaunchKernelGGL(k_pack, dim3(grid_sites(c.nvol)), dim3(256), 0, c.stream, (const double *)net.coarse_f,
-                           (const double *)net.coarse_p, c.nvol, d4(net.mean), inv4(net.sd), net.x0);
+        if (!usable) {
+            if (losses) losses[it] = std::nan("");
+            continue;
+        }
         hipLaunchKernelGGL(k_target, dim3(grid_sites(c.nvol)), dim3(256), 0, c.stream, (const double *)c.fld,
                            (const double *)c.fldp, (const double *)net.coarse_f, (const double *)net.coarse_p,
-                           c.nvol, inv4(net.rsd), net.tgt);
+                           c.nvol, (const double *)net.cm, d4(net.ratio), net.tgt);
         forward(net);
-        backward(net);
+        loss_grad(net);
+        std::vector<float> lp(1024);
+        LAT_HIP(&c, hipMemcpyAsync(lp.data(), net.lossp, sizeof(float) * 1024, hipMemcpyDeviceToHost, c.stream));
+        LAT_HIP(&c, hipStreamSynchronize(c.stream));
+        double s = 0;
+        for (float v : lp) s += v;
+        const double loss = s / (4.0 * (double)c.nvol);
+        if (losses) losses[it] = loss;
+        // a pair whose coarse step left the stability region (non-finite
+        // loss) would poison the weights: no update from it
+        if (!std::isfinite(loss)) continue;
+        backward_layers(net);
         adam(net, (float)lr);
-        if (losses) {
-            std::vector<float> lp(1024);
-            LAT_HIP(&c, hipMemcpyAsync(lp.data(), net.lossp, sizeof(float) * 1024, hipMemcpyDeviceToHost, c.stream));
-            LAT_HIP(&c, hipStreamSynchronize(c.stream));
-            double s = 0;
-            for (float v : lp) s += v;
-            losses[it] = s / (4.0 * (double)c.nvol);
-        }
     }
     LAT_HIP(&c, hipGetLastError());
     return 0;
@@ -827,10 +924,11 @@ int lat_corrector_step(lat_corrector *h, lat_ctx *ctx, double dt, int coarse_fac
         const double r0 = e[LAT_E_RESID];
         if ((rc = save_state(net, c))) return rc;
         if ((rc = lat_symp(ctx, 2, coarse_factor * dt, 1))) return rc;
-        pack(net, c);
+        pack(net, c, c.fld, c.fldp);
         forward(net);
+        channel_means(net, c, nullptr, nullptr, nullptr, nullptr, net.y, 2);
         hipLaunchKernelGGL(k_apply, dim3(grid_sites(c.nvol)), dim3(256), 0, c.stream, (double *)c.fld,
-                           (double *)c.fldp, net.y, c.nvol, d4(net.rsd));
+                           (double *)c.fldp, net.y, c.nvol, (const double *)net.cm, d4(net.ratio));
         c.ke_valid = false;
         forces += 1;
         if ((rc = lat_energies(ctx, e))) return rc;

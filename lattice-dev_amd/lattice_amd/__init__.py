@@ -611,14 +611,16 @@ class Corrector:
         self._check(lib().lat_corrector_get_layer(self._h, layer, _pf(w), _pf(b)), "get_layer")
         return w, b
 
-    def set_norm(self, mean, sd, rsd):
-        arrs = [np.ascontiguousarray(x, np.float64) for x in (mean, sd, rsd)]
-        self._check(lib().lat_corrector_set_norm(self._h, *[_p(a) for a in arrs]), "set_norm")
+    def set_norm(self, ratio):
+        """Per-channel residual scale: RMS of the residual fluctuation over the
+        standard deviation of the coarse state (phi, chi, pi_phi, pi_chi)."""
+        r = np.ascontiguousarray(ratio, np.float64)
+        self._check(lib().lat_corrector_set_norm(self._h, _p(r)), "set_norm")
 
     def get_norm(self):
-        arrs = [np.empty(4) for _ in range(3)]
-        self._check(lib().lat_corrector_get_norm(self._h, *[_p(a) for a in arrs]), "get_norm")
-        return tuple(arrs)
+        r = np.empty(4)
+        self._check(lib().lat_corrector_get_norm(self._h, _p(r)), "get_norm")
+        return r
 
     def infer(self, lat):
         out = np.empty((lat.nz, lat.ny, lat.nx, 4), np.float32)

@@ -66,9 +66,11 @@ def torch_forward(x0, layers, rounding=True):
     return h[0].permute(1, 2, 3, 0), params
 
 
-def packed_input(fld, fldp, mean, sd):
+def packed_input(fld, fldp):
+    """Fluctuations about the per-channel spatial mean over the per-channel
+    spatial standard deviation, in BF16."""
     v = np.stack([fld[0], fld[1], fldp[0], fldp[1]], axis=-1)
-    return bf16(((v - mean) / sd).astype(np.float32))
+    return bf16(((v - v.mean(axis=(0, 1, 2))) / v.std(axis=(0, 1, 2))).astype(np.float32))
 
 
 def test_corrector_forward_matches_torch(gpu):
@@ -76,18 +78,16 @@ def test_corrector_forward_matches_torch(gpu):
     n, L = 16, 10.0
     rng = np.random.default_rng(1)
     fld, fldp = state(n, L, 4)
-    mean = np.array([1.0, 0.0, -0.7, 0.0])
-    sd = np.array([fld[0].std(), fld[1].std(), fldp[0].std(), fldp[1].std()])
     layers = random_layers(rng)
     with gpu.Lattice(n, nfld=2, length=L, model_params=(1.0, 1e4)) as lat, gpu.Corrector(3) as corr:
         lat.set_state(fld, fldp, 1.0, 0.0)
         for li, (w, b) in enumerate(layers):
             corr.set_layer(li, w, b)
-        corr.set_norm(mean, sd, np.ones(4))
+        corr.set_norm(np.ones(4))
         out = corr.infer(lat)
         w0, b0 = corr.get_layer(0)
     assert np.array_equal(w0, layers[0][0]) and np.array_equal(b0, layers[0][1])
-    ref, _ = torch_forward(packed_input(fld, fldp, mean, sd), layers)
+    ref, _ = torch_forward(packed_input(fld, fldp), layers)
     ref = ref.detach().numpy()
     err = np.abs(out - ref)
     # same BF16 rounding points; only the FP32 summation order differs, which
@@ -103,17 +103,15 @@ def test_corrector_gradients_match_torch(gpu):
     n, L = 16, 10.0
     rng = np.random.default_rng(2)
     fld, fldp = state(n, L, 6)
-    mean = np.array([1.0, 0.0, -0.7, 0.0])
-    sd = np.array([fld[0].std(), fld[1].std(), fldp[0].std(), fldp[1].std()])
     layers = random_layers(rng, scale=0.5)
     target = (rng.standard_normal((n, n, n, 4)) * 0.5).astype(np.float32)
     with gpu.Lattice(n, nfld=2, length=L, model_params=(1.0, 1e4)) as lat, gpu.Corrector(3) as corr:
         lat.set_state(fld, fldp, 1.0, 0.0)
         for li, (w, b) in enumerate(layers):
             corr.set_layer(li, w, b)
-        corr.set_norm(mean, sd, np.ones(4))
+        corr.set_norm(np.ones(4))
         loss, grads = corr.gradients(lat, target)
-    out, params = torch_forward(packed_input(fld, fldp, mean, sd), layers)
+    out, params = torch_forward(packed_input(fld, fldp), layers)
     tl = ((out - torch.from_numpy(target)) ** 2).mean()
     tl.backward()
     assert abs(loss - tl.item()) < 1e-2 * tl.item()
@@ -141,18 +139,31 @@ def test_corrector_training_and_fallback(gpu):
         losses = corr.train(lat, dt, cf, npairs=40, lr=2e-3)
         assert np.all(np.isfinite(losses))
         assert losses[-5:].mean() < 0.95 * losses[:5].mean()
-        mean, sd, rsd = corr.get_norm()
-        assert np.all(sd > 0) and np.all(rsd > 0)
+        ratio = corr.get_norm()
+        assert np.all(ratio > 0) and np.all(ratio < 1)
         start = lat.get_state()
         st = corr.step(lat, dt, cf, nsteps=2, threshold=0.0)
         # each step: the rejected coarse attempt (1 force evaluation) + cf Yoshida-4 steps
         assert st["fallbacks"] == 2 and st["force_evals"] == 2 * (1 + 3 * cf)
         got = lat.get_state()
-        st2 = corr.step(lat, dt, cf, nsteps=2, threshold=1.0)
-        assert st2["fallbacks"] == 0 and st2["force_evals"] == 2
+        st2 = corr.step(lat, dt, cf, nsteps=1, threshold=1.0)
+        assert st2["fallbacks"] == 0 and st2["force_evals"] == 1
+        acc = lat.get_state()
     with gpu.Lattice(n, **kw) as ref:
         ref.set_state(start[0], start[1], start[2], start[3])
         ref.symp4(dt, 2 * cf)
         want = ref.get_state()
     assert np.linalg.norm(got[0] - want[0]) < 1e-12 * np.linalg.norm(want[0])
     assert np.linalg.norm(got[1] - want[1]) < 1e-11 * np.linalg.norm(want[1])
+    # an accepted step = one leapfrog step of cf*dt plus a correction with no
+    # homogeneous part: scale factor identical, channel means unchanged
+    with gpu.Lattice(n, **kw) as ref:
+        ref.set_state(got[0], got[1], got[2], got[3])
+        ref.symp(2, cf * dt, 1)
+        leap = ref.get_state()
+    assert acc[2] == leap[2] and acc[3] == leap[3]
+    for k in (0, 1):
+        for i in (0, 1):
+            d = acc[k][i] - leap[k][i]
+            assert abs(d.mean()) < 1e-12 * (abs(leap[k][i].mean()) + leap[k][i].std())
+            assert np.linalg.norm(d) > 0
