@@ -38,9 +38,9 @@ __device__ __forceinline__ void tile_range(int T, int &start, int &end, int &ste
     }
 }
 
-int grid_for(int tiles)
+int grid_for(int tiles, int cap)
 {
-    int g = tiles < MAX_GRID ? tiles : MAX_GRID;
+    int g = tiles < cap ? tiles : cap;
     if (g >= 8) g &= ~7;
     return g < 1 ? 1 : g;
 }
@@ -528,6 +528,15 @@ constexpr int c2r_lines() { return clampi(1024 / (N / 2), 1, 256); }
 template <typename T>
 constexpr int r2c_maxradix() { return sizeof(T) == 8 ? 4 : 8; }
 
+// Grid sizes (1024^3 sweep of 1024 ... all tiles): the x R2C runs fastest
+// with one block per tile; the x C2R + kick with 8192 blocks in FP32 and
+// 1024 in FP64 (it writes one reduction partial per block, at most
+// MAX_GRID); the y passes with 65536; the fused z pass with 8192 / 1024.
+constexpr int R2C_GRID = 1 << 30;
+template <typename T>
+constexpr int c2r_grid() { return sizeof(T) == 8 ? 1024 : 8192; }
+static_assert(c2r_grid<float>() <= MAX_GRID && c2r_grid<double>() <= MAX_GRID, "one partial per block");
+
 // y/z column passes: TC adjacent kx columns per tile.  y passes: 8 columns
 // at N = 1024 (128 B rows in FP64), radix <= 16, 512 threads.  Fused z pass:
 // radix <= 16 with the last forward / first inverse pass in registers
@@ -542,6 +551,7 @@ struct ColCfg {
     static constexpr bool PF = Z && F64;
     static constexpr int NT = (Z && F64) ? 256 : 512;
     static constexpr bool FUSE = Z;
+    static constexpr int GRID = Z ? (F64 ? 1024 : 8192) : 65536;
 };
 
 #define LAT_FOR_SIZES(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(48) X(96) X(192)
@@ -564,7 +574,7 @@ template <typename T, int N, int L, int MR>
 static void r2c_go(Ctx &c, const cpx<T> *tw, T *f, const T *pi, cpx<T> *Fk, int drift, int field)
 {
     const int rows = c.ny * c.nz;
-    const int g = grid_for((rows + L - 1) / L);
+    const int g = grid_for((rows + L - 1) / L, R2C_GRID);
     constexpr bool TW = r2c_twl<T>();
     if (drift)
         hipLaunchKernelGGL((k_x_r2c<T, N, L, true, MR, TW>), dim3(g, c.B), dim3(TPB), 0, c.stream, f, pi, Fk, tw,
@@ -608,7 +618,7 @@ static void col_go(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int n_other, size_t s_a
                    double kfac)
 {
     using C = ColCfg<T, N, MODE == 2 ? 2 : 0>;
-    const int g = grid_for(((c.nxh + TC - 1) / TC) * n_other * n_arrays);
+    const int g = grid_for(((c.nxh + TC - 1) / TC) * n_other * n_arrays, C::GRID);
     hipLaunchKernelGGL((k_col<T, N, TC, MODE, C::MR, C::PF, 1, C::NT, true, C::FUSE>), dim3(g, c.B),
                        dim3(C::NT), 0, c.stream, Fk, tw, c.nxh, n_other, s_axis, s_other, c.nk, n_arrays, c.ny, kfac,
                        c.ms.k);
@@ -656,7 +666,7 @@ static int c2r_go(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int nf, int epi, T *out,
                   const ModelDev &md)
 {
     const int rows = c.ny * c.nz;
-    const int g = grid_for((rows + L - 1) / L);
+    const int g = grid_for((rows + L - 1) / L, c2r_grid<T>());
     if (epi == EPI_STORE)
         hipLaunchKernelGGL((k_x_c2r<T, N, L, EPI_STORE, 4, true, true>), dim3(g, c.B), dim3(TPB), 0, c.stream, Fk, tw,
                            rows, c.P, c.nk, c.nvol, nf, out, fld, fldp, c.dsc, md, c.partials, c.ms);

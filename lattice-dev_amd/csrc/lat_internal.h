@@ -14,7 +14,8 @@
 namespace lat {
 
 constexpr int TPB = 256;       // threads per workgroup for all lattice kernels
-constexpr int MAX_GRID = 4096; // cap of grid-stride launches (= max reduction partials)
+constexpr int MAX_GRID = 8192; // most reduction partials (blocks) per member of a launch
+constexpr int GRID_CAP = 4096; // default cap of grid-stride launches
 constexpr int NRED = 8;        // reduction channels per partial
 
 // Reduction channels written by the sweep kernels (FP64 partial per block).
@@ -174,7 +175,7 @@ int launch_reduce_state(Ctx &c);        // partials: PE, KE, means (no Laplacian
 template <typename T>
 int launch_stencil(Ctx &c, int epi);    // FD Laplacian + kick / reduce epilogue
 int launch_scalar(Ctx &c, const ScalarOps &ops, int npart);
-int grid_for(int tiles);
+int grid_for(int tiles, int cap = GRID_CAP);
 
 // ---- spectra (lat_spec.hip) ----
 // mode 0: |F1|^2; 1: F1 conj(F2); 2: output channels; 3: sum_ij |Lambda U|^2 of
