@@ -108,7 +108,8 @@ def roofline(prof, n, nf, w):
     achieved = b / avg_s / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{stage}_{n}.json")
-    if os.path.exists(tpath):
+    if w == 4 and os.path.exists(tpath):
+        # measured for the mixed form (FP32 sweeps); no FP64 counterpart
         with open(tpath) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
     return {"bound": "hbm", "kernel": stage, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
