@@ -299,9 +299,114 @@ __global__ __launch_bounds__(NT, MINW) void k_col(cpx<T> *__restrict__ Fk, const
     }
 }
 
+// ---------------------------------------------------------------- y / z columns, streamed tiles
+// Same transforms as k_col, with the tiles streamed into LDS by the
+// LDS-direct loads of gfx950 (global_load_lds_dwordx4: each lane's 16 bytes
+// land at consecutive LDS addresses of its wave, no registers involved) and
+// double-buffered: tile t+1 streams into one LDS buffer while the passes of
+// tile t run in the other, so HBM latency hides behind the LDS passes at no
+// register cost.  Every pass runs from LDS.  The loads read whole TC-column
+// rows, which needs the k-space row pitch to be a multiple of TC (columns
+// past nx/2+1 are padding; their transforms are computed and not stored).
+typedef __attribute__((address_space(3))) char *lds_bytes_t;
+
+template <typename T, int N, int TC, int NT>
+__device__ __forceinline__ void stream_tile(const cpx<T> *base, size_t s_axis, cpx<T> *tile)
+{
+    constexpr int UPR = TC * (int)sizeof(cpx<T>) / 16; // 16-byte units per row
+    constexpr int UNITS = N * UPR;
+    static_assert(TC * sizeof(cpx<T>) % 16 == 0 && UNITS % NT == 0, "tile must split into whole wave loads");
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int i = 0; i < UNITS / NT; ++i) {
+        const int u0 = (i * (NT / 64) + wave) * 64; // first unit of this wave's load
+        const int u = u0 + lane;
+        const int row = u / UPR, part = u - row * UPR;
+        const char *g = reinterpret_cast<const char *>(base + (size_t)row * s_axis) + part * 16;
+        __builtin_amdgcn_global_load_lds(g, (lds_bytes_t)(tile) + u0 * 16, 16, 0, 0);
+    }
+}
+
+template <typename T, int N, int TC, int MODE, int MR = 16, int NT = TPB, bool FUSE = true>
+__global__ __launch_bounds__(NT) void k_col_stream(cpx<T> *__restrict__ Fk, const cpx<T> *__restrict__ tw, int nxh,
+                                                  int n_other, size_t s_axis, size_t s_other, size_t s_arr,
+                                                  int n_arr, int ny, double kfac, size_t ms_k)
+{
+    constexpr bool TWL = true;
+    LAT_TWIDDLES_TO_LDS(N, NT)
+    Fk += blockIdx.y * ms_k;
+    __shared__ __align__(16) cpx<T> bufs[2][TC * N];
+    constexpr int NP = num_passes(N, MR);
+    constexpr int SIGN0 = (MODE == 1) ? +1 : -1;
+    const int nct = (nxh + TC - 1) / TC;
+    int t0, t1, ts;
+    tile_range(nct * n_other * n_arr, t0, t1, ts);
+    auto tile_base = [&](int t, int &ncol, int &ct, int &o) -> cpx<T> * {
+        ct = t % nct;
+        const int rest = t / nct;
+        o = rest % n_other;
+        const int arr = rest / n_other;
+        ncol = min(TC, nxh - ct * TC);
+        return Fk + (size_t)arr * s_arr + (size_t)o * s_other + (size_t)ct * TC;
+    };
+    if (t0 < t1) {
+        int ncol, ct, o;
+        stream_tile<T, N, TC, NT>(tile_base(t0, ncol, ct, o), s_axis, bufs[0]);
+    }
+    int cur = 0;
+    for (int t = t0; t < t1; t += ts, cur ^= 1) {
+        int ncol, ct, o;
+        cpx<T> *base = tile_base(t, ncol, ct, o);
+        cpx<T> *buf = bufs[cur];
+        // this tile's loads (and the previous tile's stores) complete; every
+        // thread is done with the other buffer, which now takes the next tile
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (t + ts < t1) {
+            int nc2, ct2, o2;
+            stream_tile<T, N, TC, NT>(tile_base(t + ts, nc2, ct2, o2), s_axis, bufs[cur ^ 1]);
+        }
+        auto store = [&](int line, int pos, cpx<T> a) {
+            if (line < ncol) base[(size_t)pos * s_axis + line] = a;
+        };
+        if constexpr (MODE == 2 && FUSE && NP >= 2) {
+            lds_fft_range<T, N, -1, TC, NT, true, 1, 0, NP - 1, MR>(buf, 1, TC, twp);
+            const int jj = wmag(o, ny);
+            auto mul = [&](int c, int n) -> T {
+                const int ii = ct * TC + c;
+                const int kk = wmag(n, N);
+                const double rad2 = double(ii * ii) + double(jj * jj) + double(kk * kk);
+                return T(-rad2 * kfac);
+            };
+            fused_last_first<T, N, TC, NT, MR>(buf, twp, mul);
+            lds_fft_range<T, N, +1, TC, NT, true, 1, 1, NP - 1, MR, true>(buf, 1, TC, twp);
+            stockham_pass_out<T, N, NP - 1, +1, TC, NT, true, 1, MR, true>(buf, 1, TC, twp, store);
+        } else if constexpr (MODE == 2) {
+            lds_fft_range<T, N, -1, TC, NT, true, 1, 0, NP, MR>(buf, 1, TC, twp);
+            const int jj = wmag(o, ny);
+            for (int idx = threadIdx.x; idx < TC * N; idx += NT) {
+                const int c = idx % TC, n = idx / TC;
+                const int ii = ct * TC + c;
+                const int kk = wmag(n, N);
+                const double rad2 = double(ii * ii) + double(jj * jj) + double(kk * kk);
+                const T m = T(-rad2 * kfac);
+                cpx<T> a = buf[idx];
+                buf[idx] = {a.x * m, a.y * m};
+            }
+            __syncthreads();
+            lds_fft_range<T, N, +1, TC, NT, true, 1, 0, NP - 1, MR>(buf, 1, TC, twp);
+            stockham_pass_out<T, N, NP - 1, +1, TC, NT, true, 1, MR>(buf, 1, TC, twp, store);
+        } else {
+            lds_fft_range<T, N, SIGN0, TC, NT, true, 1, 0, NP - 1, MR>(buf, 1, TC, twp);
+            stockham_pass_out<T, N, NP - 1, SIGN0, TC, NT, true, 1, MR>(buf, 1, TC, twp, store);
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+}
+
 // ---------------------------------------------------------------- x C2R + epilogue
-template <typename T, int N, int L, int EPI, int MR = 4, bool PRE = true, bool TWL = false>
-__global__ __launch_bounds__(TPB) void k_x_c2r(const cpx<T> *__restrict__ Fk, const cpx<T> *__restrict__ tw,
+template <typename T, int N, int L, int EPI, int MR = 4, bool PRE = true, bool TWL = false, int MINW = 1>
+__global__ __launch_bounds__(TPB, MINW) void k_x_c2r(const cpx<T> *__restrict__ Fk, const cpx<T> *__restrict__ tw,
                                                int rows, int P, size_t nk, size_t nvol, int nf, T *__restrict__ out,
                                                const T *__restrict__ fld, T *__restrict__ fldp,
                                                const DevScalars *__restrict__ dsc, ModelDev md,
@@ -520,9 +625,10 @@ __global__ __launch_bounds__(TPB) void k_x_c2r(const cpx<T> *__restrict__ Fk, co
 // repository documentation) and scaled by the line length N for other sizes.
 constexpr int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-// rows per workgroup: x R2C 2048/(N/2) complex elements, x C2R 1024/(N/2)
-template <int N>
-constexpr int r2c_lines() { return clampi(2048 / (N / 2), 1, 512); }
+// rows per workgroup: x R2C 2048/(N/2) complex elements (FP32 at N = 1024:
+// 2 rows, one block per tile), x C2R 1024/(N/2)
+template <typename T, int N>
+constexpr int r2c_lines() { return (sizeof(T) == 4 && N == 1024) ? 2 : clampi(2048 / (N / 2), 1, 512); }
 template <int N>
 constexpr int c2r_lines() { return clampi(1024 / (N / 2), 1, 256); }
 template <typename T>
@@ -552,6 +658,10 @@ struct ColCfg {
     static constexpr int NT = (Z && F64) ? 256 : 512;
     static constexpr bool FUSE = Z;
     static constexpr int GRID = Z ? (F64 ? 1024 : 8192) : 65536;
+    // FP32 y passes at N = 1024: tiles streamed into LDS (k_col_stream),
+    // 1024 threads, one persistent block per CU
+    static constexpr bool STREAM = !Z && !F64 && N == 1024;
+    static constexpr int STREAM_NT = 1024, STREAM_GRID = 256;
 };
 
 #define LAT_FOR_SIZES(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(48) X(96) X(192)
@@ -565,10 +675,10 @@ template <typename T, int N>
 constexpr int col_tc(int mode, int v) { return v ? clampi((mode == 2 ? ColCfg<T, N, 2>::TC : ColCfg<T, N, 0>::TC) / 2, 4, 64)
                                               : (mode == 2 ? ColCfg<T, N, 2>::TC : ColCfg<T, N, 0>::TC); }
 
-// Twiddles in LDS (TWL) where the 1024^3 sweeps measured a gain: the x C2R
-// (both precisions), the FP32 x R2C, the y passes and the fused z pass.
+// Twiddles in LDS (TWL) in every sweep: the 1024^3 sweeps measured a gain
+// (or no loss) in each.
 template <typename T>
-constexpr bool r2c_twl() { return sizeof(T) == 4; }
+constexpr bool r2c_twl() { return true; }
 
 template <typename T, int N, int L, int MR>
 static void r2c_go(Ctx &c, const cpx<T> *tw, T *f, const T *pi, cpx<T> *Fk, int drift, int field)
@@ -587,7 +697,7 @@ static void r2c_go(Ctx &c, const cpx<T> *tw, T *f, const T *pi, cpx<T> *Fk, int 
 template <typename T, int N>
 static void r2c_size(Ctx &c, const cpx<T> *tw, T *f, const T *pi, cpx<T> *Fk, int drift, int field)
 {
-    constexpr int L = r2c_lines<N>();
+    constexpr int L = r2c_lines<T, N>();
     constexpr int MR = r2c_maxradix<T>();
     if constexpr (tunable<N>() && L > 1) {
         if (c.tune.r2c == 1) return r2c_go<T, N, L / 2, MR>(c, tw, f, pi, Fk, drift, field);
@@ -618,6 +728,17 @@ static void col_go(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int n_other, size_t s_a
                    double kfac)
 {
     using C = ColCfg<T, N, MODE == 2 ? 2 : 0>;
+    if constexpr (C::STREAM && TC == C::TC) {
+        // whole-row loads read TC columns from every tile start: the row pitch
+        // must cover the last tile
+        if (((c.nxh + TC - 1) / TC) * TC <= c.P) {
+            const int g = grid_for(((c.nxh + TC - 1) / TC) * n_other * n_arrays, C::STREAM_GRID);
+            hipLaunchKernelGGL((k_col_stream<T, N, TC, MODE, C::MR, C::STREAM_NT, false>), dim3(g, c.B),
+                               dim3(C::STREAM_NT), 0, c.stream, Fk, tw, c.nxh, n_other, s_axis, s_other, c.nk,
+                               n_arrays, c.ny, kfac, c.ms.k);
+            return;
+        }
+    }
     const int g = grid_for(((c.nxh + TC - 1) / TC) * n_other * n_arrays, C::GRID);
     hipLaunchKernelGGL((k_col<T, N, TC, MODE, C::MR, C::PF, 1, C::NT, true, C::FUSE>), dim3(g, c.B),
                        dim3(C::NT), 0, c.stream, Fk, tw, c.nxh, n_other, s_axis, s_other, c.nk, n_arrays, c.ny, kfac,

@@ -82,3 +82,30 @@ def test_mixed_state_roundtrip(gpu):
     for l in range(2):  # fluctuations ~1e-6 stored in FP32: ~1e-13 absolute
         assert np.abs(f[l] - fld[l]).max() <= 1e-12
         assert np.abs(p[l] - fldp[l]).max() <= 1e-12
+
+
+def test_mixed_tracks_fp64_at_1024(gpu):
+    """The 1024^3 launch shapes (fused z pass, LDS-streamed FP32 y passes,
+    one-block-per-tile x sweeps) against the FP64 form: energies and spectra
+    only (no host copy of the 8 GiB fields)."""
+    print("case mixed 1024", flush=True)
+    n, steps, dt = 1024, 4, (10.0 / 1024) / 40
+    out = {}
+    for prec in (gpu.F64, gpu.MIXED):
+        L, g2 = 10.0, 2.5e5
+        with gpu.Lattice(n, nfld=2, length=L, precision=prec, time_var=gpu.COSMIC, model=gpu.MODEL_TWOFIELD,
+                         model_params=(1.0, g2)) as lat:
+            lat.sample_field(0, 0, PHI0, -0.25, 1.0, 2e5, 5, 0)
+            lat.sample_field(0, 1, DPHI0, 0.25, 1.0, 2e5, 5, 1)
+            lat.sample_field(1, 0, 0.0, -0.25, g2 * PHI0**2, 2e5, 5, 2)
+            lat.sample_field(1, 1, 0.0, 0.25, g2 * PHI0**2, 2e5, 5, 3)
+            lat.set_scalars(1.0, 0.0)
+            lat.calc_metric()
+            lat.symp4(dt, steps)
+            out[prec] = (lat.energies(), [(lat.output_spectra(0), lat.output_spectra(1))])
+    (ed, sd), (em, sm) = out[gpu.F64], out[gpu.MIXED]
+    assert abs(em["rho"] - ed["rho"]) <= 1e-8 * abs(ed["rho"])
+    assert abs(em["a"] - ed["a"]) <= 1e-12
+    err3 = spec_err(sm, sd, 1e-3)
+    print("1024 mixed spectrum max rel err (floor 1e-3)", err3, flush=True)
+    assert err3 < 1e-5
