@@ -865,7 +865,8 @@ int lat_stencil_laplacian(lat_ctx *h, const double *f, double *lap)
 static int ensure_spec(Ctx &c)
 {
     const size_t cap = (size_t)c.spec_cap;
-    int rc = ensure(c, (void **)&c.spec_part, sizeof(double) * 1024 * 5 * cap, "hipMalloc spec_part");
+    int rc = ensure(c, (void **)&c.spec_part, sizeof(double) * (BIN_BLOCKS + BIN_BLOCKS / BIN_GROUP) * 5 * cap,
+                    "hipMalloc spec_part");
     rc |= ensure(c, (void **)&c.spec_out, sizeof(double) * 5 * cap, "hipMalloc spec_out");
     return rc;
 }

@@ -264,11 +264,18 @@ __global__ __launch_bounds__(NT, MINW) void k_col(cpx<T> *__restrict__ Fk, const
         if constexpr (MODE == 2 && FUSE && NP >= 2) {
             lds_fft_range<T, N, -1, TC, NT, true, 1, 1, NP - 1, MR>(buf, 1, TC, twp);
             const int jj = wmag(o, ny);
+            const float kf = float(kfac);
             auto mul = [&](int c, int n) -> T {
                 const int ii = ct * TC + c;
                 const int kk = wmag(n, N);
-                const double rad2 = double(ii * ii) + double(jj * jj) + double(kk * kk);
-                return T(-rad2 * kfac);
+                if constexpr (sizeof(T) == 4) {
+                    // |k|^2 in lattice units is an integer below 2^24 (N <= 1024):
+                    // exact in FP32, one rounding in the product
+                    return -float(ii * ii + jj * jj + kk * kk) * kf;
+                } else {
+                    const double rad2 = double(ii * ii) + double(jj * jj) + double(kk * kk);
+                    return T(-rad2 * kfac);
+                }
             };
             fused_last_first<T, N, TC, NT, MR>(buf, twp, mul);
             lds_fft_range<T, N, +1, TC, NT, true, 1, 1, NP - 1, MR, true>(buf, 1, TC, twp);
@@ -372,11 +379,18 @@ __global__ __launch_bounds__(NT) void k_col_stream(cpx<T> *__restrict__ Fk, cons
         if constexpr (MODE == 2 && FUSE && NP >= 2) {
             lds_fft_range<T, N, -1, TC, NT, true, 1, 0, NP - 1, MR>(buf, 1, TC, twp);
             const int jj = wmag(o, ny);
+            const float kf = float(kfac);
             auto mul = [&](int c, int n) -> T {
                 const int ii = ct * TC + c;
                 const int kk = wmag(n, N);
-                const double rad2 = double(ii * ii) + double(jj * jj) + double(kk * kk);
-                return T(-rad2 * kfac);
+                if constexpr (sizeof(T) == 4) {
+                    // |k|^2 in lattice units is an integer below 2^24 (N <= 1024):
+                    // exact in FP32, one rounding in the product
+                    return -float(ii * ii + jj * jj + kk * kk) * kf;
+                } else {
+                    const double rad2 = double(ii * ii) + double(jj * jj) + double(kk * kk);
+                    return T(-rad2 * kfac);
+                }
             };
             fused_last_first<T, N, TC, NT, MR>(buf, twp, mul);
             lds_fft_range<T, N, +1, TC, NT, true, 1, 1, NP - 1, MR, true>(buf, 1, TC, twp);

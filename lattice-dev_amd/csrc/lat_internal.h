@@ -181,6 +181,10 @@ int grid_for(int tiles, int cap = GRID_CAP);
 // mode 0: |F1|^2; 1: F1 conj(F2); 2: output channels; 3: sum_ij |Lambda U|^2 of
 // the six tensor components F1 + c*nk (TT projection).  nbins <= 0: c.ns.
 enum { BIN_COUNTS = 1, BIN_HALF = 2, BIN_NO_NYQ = 4 }; // launch_bin flags
+// spectrum binning: workgroups (one wave each) and the group size of the
+// first stage of the two-stage block reduction; spec_part holds
+// BIN_BLOCKS + BIN_BLOCKS / BIN_GROUP histograms
+constexpr int BIN_BLOCKS = 4096, BIN_GROUP = 64;
 template <typename T>
 int launch_bin(Ctx &c, const cpx<T> *F1, const cpx<T> *F2, int mode, int nbins = 0, int flags = 0);
 

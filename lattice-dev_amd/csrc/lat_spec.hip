@@ -7,12 +7,13 @@
 // all nx x-indices and re-reads Fk(nx-i) for i > nx/2; on the half-complex
 // array that is a multiplicity of 2 for 0 < kx < nx/2.
 //
-// Determinism: one wave per workgroup walks one (ky,kz) row at a time with
-// the 64 lanes on consecutive kx.  Along a row p is non-decreasing in kx, so
-// equal bins form contiguous lane segments; a segmented shuffle scan sums each
-// segment and only segment tails update the wave-private LDS histogram, in
-// program order (no atomics).  Workgroup histograms are reduced in block
-// order by k_bin_reduce.  Results are bitwise reproducible run to run.
+// Determinism: each wave walks one (ky,kz) row at a time with the 64 lanes
+// on consecutive kx (4 waves per workgroup share one LDS histogram).  Along a
+// row p is non-decreasing in kx, so equal bins form contiguous lane
+// segments; a segmented DPP scan sums each segment and only segment tails
+// update the histogram, one wave after the other in wave order (no atomics).
+// Workgroup histograms are reduced in block order by k_bin_reduce (groups of
+// 64 blocks, then the groups).  Results are bitwise reproducible run to run.
 #include "lat_internal.h"
 
 namespace lat {
@@ -77,122 +78,204 @@ __device__ __forceinline__ double tt_norm2(const double kv[3], const double ur[6
 //               `nofft` routines, :54-186);
 //   BIN_NO_NYQ  skip the x-Nyquist plane (spectrum_bin_nofft :69,
 //               crossspec_nofft :166).
+constexpr int BIN_ROWCHUNKS = 9; // 64-mode chunks per k-space row (nx <= 1024)
+constexpr int BIN_WAVES = 4;      // waves per workgroup, sharing one histogram
+
+// Cross-lane moves for the segmented scan with DPP (GFX9 data-parallel
+// primitives: row_shr:n within 16-lane rows, row_bcast:15 / row_bcast:31
+// across rows) instead of LDS-routed shuffles.  A lane whose source is out
+// of range, or whose row is masked off, receives the `old` value: key -1 (no
+// match) and 0.0.
+template <int CTRL, int RM>
+__device__ __forceinline__ int dpp_key(int v)
+{
+    return __builtin_amdgcn_update_dpp(-1, v, CTRL, RM, 0xF, false);
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ double dpp_val(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, RM, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, RM, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+// one step of the segmented inclusive scan: add the moved partial sums where
+// the moved key equals the lane's key (keys are sorted along the wave, so
+// equal keys are one contiguous segment)
+template <int CTRL, int RM, int NC>
+__device__ __forceinline__ void seg_scan_step(int key, double (&v1)[NC], double (&v2)[NC])
+{
+    const int ok = dpp_key<CTRL, RM>(key);
+    double o1[NC], o2[NC];
+#pragma unroll
+    for (int ch = 0; ch < NC; ++ch) {
+        o1[ch] = dpp_val<CTRL, RM>(v1[ch]);
+        o2[ch] = dpp_val<CTRL, RM>(v2[ch]);
+    }
+    if (ok == key) {
+#pragma unroll
+        for (int ch = 0; ch < NC; ++ch) {
+            v1[ch] += o1[ch];
+            v2[ch] += o2[ch];
+        }
+    }
+}
+template <int NC>
+__device__ __forceinline__ void seg_scan(int key, double (&v1)[NC], double (&v2)[NC])
+{
+    seg_scan_step<0x111, 0xF>(key, v1, v2); // row_shr:1
+    seg_scan_step<0x112, 0xF>(key, v1, v2); // row_shr:2
+    seg_scan_step<0x114, 0xF>(key, v1, v2); // row_shr:4
+    seg_scan_step<0x118, 0xF>(key, v1, v2); // row_shr:8
+    seg_scan_step<0x142, 0xA>(key, v1, v2); // row_bcast:15 into rows 1, 3
+    seg_scan_step<0x143, 0xC>(key, v1, v2); // row_bcast:31 into rows 2, 3
+}
+
 template <typename T, int NCH, int MODE>
-__global__ __launch_bounds__(64) void k_bin(const cpx<T> *__restrict__ F1, const cpx<T> *__restrict__ F2, int nx,
-                                            int ny, int nz, int nxh, int P, int ns, double *__restrict__ part,
-                                            int flags, size_t nk)
+__global__ __launch_bounds__(64 * BIN_WAVES) void k_bin(const cpx<T> *__restrict__ F1, const cpx<T> *__restrict__ F2,
+                                                        int nx, int ny, int nz, int nxh, int P, int ns,
+                                                        double *__restrict__ part, int flags, size_t nk)
 {
     extern __shared__ double hist[]; // (NCH+1) * ns
-    const int lane = threadIdx.x;
-    for (int q = lane; q < (NCH + 1) * ns; q += 64) hist[q] = 0.0;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int q = threadIdx.x; q < (NCH + 1) * ns; q += 64 * BIN_WAVES) hist[q] = 0.0;
     __syncthreads();
     const int rows = ny * nz;
-    for (int row = blockIdx.x; row < rows; row += gridDim.x) {
-        const int j = row % ny, k = row / ny;
-        const int jj = (j <= ny / 2) ? j : ny - j;
-        const int kk = (k <= nz / 2) ? k : nz - k;
-        for (int i0 = 0; i0 < nxh; i0 += 64) {
-            const int i = i0 + lane;
-            const bool counts = (flags & BIN_COUNTS) != 0;
-            const bool valid = i < nxh && !((flags & BIN_NO_NYQ) && 2 * i == nx);
-            double v1[NCH + 1], v2[NCH + 1];
-            int key = 0x7fffffff;
-            if (valid) {
-                const double p = sqrt(double(i * i + jj * jj + kk * kk));
-                const int l = (int)floor(p);
-                const double d1 = double(l) - p, d2 = double(l + 1) - p;
-                const double c1 = counts ? 1.0 : (1.0 - d1 * d1) * (1.0 - d1 * d1);
-                const double c2 = counts ? 0.0 : (1.0 - d2 * d2) * (1.0 - d2 * d2);
-                const double mult = ((flags & BIN_HALF) || i == 0 || (2 * i == nx)) ? 1.0 : 2.0;
+    const bool counts = (flags & BIN_COUNTS) != 0;
+    // one 64-lane chunk of a row: value(s) a, b of mode i = i0 + lane
+    // one 64-lane chunk of a row: segment sums of bins l (v1) and l + 1 (v2)
+    // at the segment tails (tail = true)
+    auto chunk = [&](bool has_row, int row, int j, int k, int jj, int kk, int i0, cpx<T> a, cpx<T> b, int &key,
+                     bool &tail, double (&v1)[NCH + 1], double (&v2)[NCH + 1]) {
+        const int i = i0 + lane;
+        const bool valid = has_row && i < nxh && !((flags & BIN_NO_NYQ) && 2 * i == nx);
+        key = 0x7fffffff;
+        if (valid) {
+            const double p = sqrt(double(i * i + jj * jj + kk * kk));
+            const int l = (int)floor(p);
+            const double d1 = double(l) - p, d2 = double(l + 1) - p;
+            const double c1 = counts ? 1.0 : (1.0 - d1 * d1) * (1.0 - d1 * d1);
+            const double c2 = counts ? 0.0 : (1.0 - d2 * d2) * (1.0 - d2 * d2);
+            const double mult = ((flags & BIN_HALF) || i == 0 || (2 * i == nx)) ? 1.0 : 2.0;
+            double val[NCH];
+            if constexpr (MODE == 3) {
                 const size_t id = (size_t)row * P + i;
-                double val[NCH];
-                if constexpr (MODE == 3) {
-                    const double kv[3] = {double(i), double(j <= ny / 2 ? j : j - ny), double(k <= nz / 2 ? k : k - nz)};
-                    double ur[6], ui[6];
+                const double kv[3] = {double(i), double(j <= ny / 2 ? j : j - ny), double(k <= nz / 2 ? k : k - nz)};
+                double ur[6], ui[6];
 #pragma unroll
-                    for (int q = 0; q < 6; ++q) {
-                        const cpx<T> u = F1[(size_t)q * nk + id];
-                        ur[q] = double(u.x);
-                        ui[q] = double(u.y);
-                    }
-                    val[0] = tt_norm2(kv, ur, ui);
-                } else if constexpr (MODE == 0) {
-                    const cpx<T> a = F1[id];
-                    val[0] = double(a.x) * a.x + double(a.y) * a.y;
-                } else {
-                    const cpx<T> a = F1[id];
-                    const cpx<T> b = F2[id];
-                    const double re = double(a.x) * b.x + double(a.y) * b.y;
-                    const double im = double(a.y) * b.x - double(a.x) * b.y;
-                    if constexpr (MODE == 1) {
-                        val[0] = re;
-                        val[1] = im;
-                    } else {
-                        val[0] = double(b.x) * b.x + double(b.y) * b.y;
-                        val[1] = double(a.x) * a.x + double(a.y) * a.y;
-                        val[2] = re;
-                        val[3] = im;
-                    }
+                for (int q = 0; q < 6; ++q) {
+                    const cpx<T> u = F1[(size_t)q * nk + id];
+                    ur[q] = double(u.x);
+                    ui[q] = double(u.y);
                 }
-#pragma unroll
-                for (int ch = 0; ch < NCH; ++ch) {
-                    v1[ch] = mult * c1 * val[ch];
-                    v2[ch] = mult * c2 * val[ch];
-                }
-                v1[NCH] = mult * c1;
-                v2[NCH] = mult * c2;
-                key = l;
+                val[0] = tt_norm2(kv, ur, ui);
+            } else if constexpr (MODE == 0) {
+                val[0] = double(a.x) * a.x + double(a.y) * a.y;
             } else {
-#pragma unroll
-                for (int ch = 0; ch <= NCH; ++ch) v1[ch] = v2[ch] = 0.0;
-            }
-            // segmented inclusive scan over lanes with equal key
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const int okey = __shfl_up(key, off, 64);
-#pragma unroll
-                for (int ch = 0; ch <= NCH; ++ch) {
-                    const double o1 = __shfl_up(v1[ch], off, 64);
-                    const double o2 = __shfl_up(v2[ch], off, 64);
-                    if (lane >= off && okey == key) {
-                        v1[ch] += o1;
-                        v2[ch] += o2;
-                    }
+                const double re = double(a.x) * b.x + double(a.y) * b.y;
+                const double im = double(a.y) * b.x - double(a.x) * b.y;
+                if constexpr (MODE == 1) {
+                    val[0] = re;
+                    val[1] = im;
+                } else {
+                    val[0] = double(b.x) * b.x + double(b.y) * b.y;
+                    val[1] = double(a.x) * a.x + double(a.y) * a.y;
+                    val[2] = re;
+                    val[3] = im;
                 }
             }
-            const int nkey = __shfl_down(key, 1, 64);
-            const bool tail = valid && (lane == 63 || nkey != key);
-            // bin l first, then bin l+1 (which may be another segment's bin l):
-            // the workgroup is one wave, the barriers order the two phases.
-            if (tail) {
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch) {
+                v1[ch] = mult * c1 * val[ch];
+                v2[ch] = mult * c2 * val[ch];
+            }
+            v1[NCH] = mult * c1;
+            v2[NCH] = mult * c2;
+            key = l;
+        } else {
+#pragma unroll
+            for (int ch = 0; ch <= NCH; ++ch) v1[ch] = v2[ch] = 0.0;
+        }
+        // segmented inclusive scan over lanes with equal key
+        seg_scan<NCH + 1>(key, v1, v2);
+        const int nkey = __shfl_down(key, 1, 64);
+        tail = valid && (lane == 63 || nkey != key);
+    };
+    // Histogram updates, one wave at a time in wave order (a barrier after
+    // each), so the order of the additions into every bin is fixed.  Within
+    // a wave the tails have distinct keys; bin l + 1 of one segment may be
+    // bin l of the next, so the l + 1 additions follow the l additions in
+    // program order.
+    auto update = [&](int key, bool tail, const double (&v1)[NCH + 1], const double (&v2)[NCH + 1]) {
+        for (int w = 0; w < BIN_WAVES; ++w) {
+            if (wave == w && tail) {
 #pragma unroll
                 for (int ch = 0; ch <= NCH; ++ch) hist[ch * ns + key] += v1[ch];
-            }
-            __syncthreads();
-            if (tail && key + 1 < ns) {
+                if (key + 1 < ns) {
 #pragma unroll
-                for (int ch = 0; ch <= NCH; ++ch) hist[ch * ns + key + 1] += v2[ch];
+                    for (int ch = 0; ch <= NCH; ++ch) hist[ch * ns + key + 1] += v2[ch];
+                }
             }
             __syncthreads();
+        }
+    };
+    // each wave walks its own row; the block's BIN_WAVES rows advance together
+    for (int base = blockIdx.x * BIN_WAVES; base < rows; base += gridDim.x * BIN_WAVES) {
+        const int row = base + wave;
+        const bool has_row = row < rows;
+        const int rr = has_row ? row : 0;
+        const int j = rr % ny, k = rr / ny;
+        const int jj = (j <= ny / 2) ? j : ny - j;
+        const int kk = (k <= nz / 2) ? k : nz - k;
+        int key;
+        bool tail;
+        double v1[NCH + 1], v2[NCH + 1];
+        if constexpr (MODE == 3) {
+            for (int i0 = 0; i0 < nxh; i0 += 64) {
+                chunk(has_row, rr, j, k, jj, kk, i0, cpx<T>{0, 0}, cpx<T>{0, 0}, key, tail, v1, v2);
+                update(key, tail, v1, v2);
+            }
+        } else {
+            // the whole row's loads are issued first (up to BIN_ROWCHUNKS
+            // chunks of 64 modes), so their latency is paid once per row
+            cpx<T> a[BIN_ROWCHUNKS], b[BIN_ROWCHUNKS];
+            const cpx<T> *r1 = F1 + (size_t)rr * P;
+            const cpx<T> *r2 = MODE != 0 ? F2 + (size_t)rr * P : nullptr;
+#pragma unroll
+            for (int c = 0; c < BIN_ROWCHUNKS; ++c) {
+                const int i = c * 64 + lane;
+                a[c] = b[c] = cpx<T>{0, 0};
+                if (has_row && i < nxh) {
+                    a[c] = r1[i];
+                    if constexpr (MODE != 0) b[c] = r2[i];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < BIN_ROWCHUNKS; ++c)
+                if (c * 64 < nxh) {
+                    chunk(has_row, rr, j, k, jj, kk, c * 64, a[c], b[c], key, tail, v1, v2);
+                    update(key, tail, v1, v2);
+                }
         }
     }
     __syncthreads();
     double *dst = part + (size_t)blockIdx.x * (NCH + 1) * ns;
-    for (int q = lane; q < (NCH + 1) * ns; q += 64) dst[q] = hist[q];
+    for (int q = threadIdx.x; q < (NCH + 1) * ns; q += 64 * BIN_WAVES) dst[q] = hist[q];
 }
 
-// out[q] = sum over blocks b (in order) of part[b][q]
-__global__ __launch_bounds__(256) void k_bin_reduce(const double *__restrict__ part, int nblk, int nq,
+// out[g][q] = sum over blocks b of group g (in order) of part[b][q]; with
+// group = nblk this is the final, single-group sum
+__global__ __launch_bounds__(256) void k_bin_reduce(const double *__restrict__ part, int nblk, int group, int nq,
                                                     double *__restrict__ out)
 {
     const int q = blockIdx.x * 256 + threadIdx.x;
+    const int g = blockIdx.y;
     if (q >= nq) return;
+    const int b0 = g * group, b1 = min(nblk, b0 + group);
     double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += part[(size_t)b * nq + q];
-    out[q] = s;
+    for (int b = b0; b < b1; ++b) s += part[(size_t)b * nq + q];
+    out[(size_t)g * nq + q] = s;
 }
-
-constexpr int BIN_BLOCKS = 1024;
 
 template <typename T>
 int launch_bin(Ctx &c, const cpx<T> *F1, const cpx<T> *F2, int mode, int nbins, int flags)
@@ -204,23 +287,34 @@ int launch_bin(Ctx &c, const cpx<T> *F1, const cpx<T> *F2, int mode, int nbins, 
     }
     const int nch = (mode == 0 || mode == 3) ? 1 : (mode == 1 ? 2 : 4);
     const int rows = c.ny * c.nz;
-    const int g = rows < BIN_BLOCKS ? rows : BIN_BLOCKS;
+    const int gr = (rows + BIN_WAVES - 1) / BIN_WAVES; // row groups
+    const int g = gr < BIN_BLOCKS ? gr : BIN_BLOCKS;
     const size_t lds = sizeof(double) * (nch + 1) * ns;
+    if (mode != 3 && c.nxh > BIN_ROWCHUNKS * 64) {
+        c.last_error = "spectrum rows longer than the binning kernel's row buffer";
+        return -1;
+    }
     if (mode == 0)
-        hipLaunchKernelGGL((k_bin<T, 1, 0>), dim3(g), dim3(64), lds, c.stream, F1, F2, c.nx, c.ny, c.nz, c.nxh, c.P,
+        hipLaunchKernelGGL((k_bin<T, 1, 0>), dim3(g), dim3(64 * BIN_WAVES), lds, c.stream, F1, F2, c.nx, c.ny, c.nz, c.nxh, c.P,
                            ns, c.spec_part, flags, c.nk);
     else if (mode == 1)
-        hipLaunchKernelGGL((k_bin<T, 2, 1>), dim3(g), dim3(64), lds, c.stream, F1, F2, c.nx, c.ny, c.nz, c.nxh, c.P,
+        hipLaunchKernelGGL((k_bin<T, 2, 1>), dim3(g), dim3(64 * BIN_WAVES), lds, c.stream, F1, F2, c.nx, c.ny, c.nz, c.nxh, c.P,
                            ns, c.spec_part, flags, c.nk);
     else if (mode == 2)
-        hipLaunchKernelGGL((k_bin<T, 4, 2>), dim3(g), dim3(64), lds, c.stream, F1, F2, c.nx, c.ny, c.nz, c.nxh, c.P,
+        hipLaunchKernelGGL((k_bin<T, 4, 2>), dim3(g), dim3(64 * BIN_WAVES), lds, c.stream, F1, F2, c.nx, c.ny, c.nz, c.nxh, c.P,
                            ns, c.spec_part, flags, c.nk);
     else
-        hipLaunchKernelGGL((k_bin<T, 1, 3>), dim3(g), dim3(64), lds, c.stream, F1, F2, c.nx, c.ny, c.nz, c.nxh, c.P,
+        hipLaunchKernelGGL((k_bin<T, 1, 3>), dim3(g), dim3(64 * BIN_WAVES), lds, c.stream, F1, F2, c.nx, c.ny, c.nz, c.nxh, c.P,
                            ns, c.spec_part, flags, c.nk);
     LAT_HIP(&c, hipGetLastError());
     const int nq = (nch + 1) * ns;
-    hipLaunchKernelGGL(k_bin_reduce, dim3((nq + 255) / 256), dim3(256), 0, c.stream, c.spec_part, g, nq, c.spec_out);
+    // two stages, both in fixed order: groups of BIN_GROUP blocks, then the groups
+    const int ngrp = (g + BIN_GROUP - 1) / BIN_GROUP;
+    double *grp = c.spec_part + (size_t)BIN_BLOCKS * nq;
+    hipLaunchKernelGGL(k_bin_reduce, dim3((nq + 255) / 256, ngrp), dim3(256), 0, c.stream, c.spec_part, g, BIN_GROUP,
+                       nq, grp);
+    hipLaunchKernelGGL(k_bin_reduce, dim3((nq + 255) / 256, 1), dim3(256), 0, c.stream, grp, ngrp, ngrp, nq,
+                       c.spec_out);
     LAT_HIP(&c, hipGetLastError());
     return nch;
 }
