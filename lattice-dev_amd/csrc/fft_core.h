@@ -370,4 +370,91 @@ __device__ __forceinline__ void fused_last_first(cpx<T> *buf, const cpx<T> *__re
     __syncthreads();
 }
 
+// ---------------------------------------------------------------- ping-pong passes
+// Out-of-place variants for column-major tiles (ESTR = L): a pass reads one
+// LDS buffer and writes the other, so each butterfly stores its outputs as
+// soon as they are computed (registers for one butterfly at a time, one
+// barrier per pass instead of two).  Twice the LDS of the in-place passes.
+template <typename T, int N, int R, int NS, int SIGN, int L, int TPB>
+__device__ __forceinline__ void stockham_pass_pp(const cpx<T> *__restrict__ src, cpx<T> *__restrict__ dst,
+                                                 const cpx<T> *__restrict__ tw)
+{
+    constexpr int NR = N / R;
+    constexpr int TOTAL = L * NR;
+    constexpr int BPT = cdiv(TOTAL, TPB);
+    constexpr int TSTEP = N / (NS * R);
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+        const int g = threadIdx.x + b * TPB;
+        if (TOTAL % TPB == 0 || g < TOTAL) {
+            const int line = g % L, j = g / L;
+            const int k = j % NS;
+            cpx<T> v[R];
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                cpx<T> a = src[(j + q * NR) * L + line];
+                if (q > 0 && NS > 1) {
+                    cpx<T> w = tw[(k * q * TSTEP) % N];
+                    if (SIGN > 0) w.y = -w.y;
+                    a = cmul(a, w);
+                }
+                v[q] = a;
+            }
+            dft_small<R, SIGN>(v);
+            const int o = (j / NS) * NS * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) dst[(o + q * NS) * L + line] = v[q];
+        }
+    }
+    __syncthreads();
+}
+
+// Passes [P, PEND) alternating between b0 and b1; PAR = buffer holding the
+// input of pass P.  After the range the data is in buffer PAR ^ ((PEND-P)&1).
+template <typename T, int N, int SIGN, int L, int TPB, int P, int PEND, int MR, bool REV, int PAR>
+__device__ __forceinline__ void lds_fft_range_pp(cpx<T> *b0, cpx<T> *b1, const cpx<T> *__restrict__ tw)
+{
+    if constexpr (P < PEND) {
+        stockham_pass_pp<T, N, sched_radix(N, P, MR, REV), sched_ns(N, P, MR, REV), SIGN, L, TPB>(
+            PAR ? b1 : b0, PAR ? b0 : b1, tw);
+        lds_fft_range_pp<T, N, SIGN, L, TPB, P + 1, PEND, MR, REV, PAR ^ 1>(b0, b1, tw);
+    }
+}
+
+// fused_last_first, out of place
+template <typename T, int N, int L, int TPB, int MR, typename Mul>
+__device__ __forceinline__ void fused_last_first_pp(const cpx<T> *__restrict__ src, cpx<T> *__restrict__ dst,
+                                                    const cpx<T> *__restrict__ tw, Mul mul)
+{
+    constexpr int NP = num_passes(N, MR);
+    constexpr int R = radix_at(N, NP - 1, MR), NS = ns_at(N, NP - 1, MR);
+    static_assert(NS * R == N, "last pass spans the line");
+    constexpr int TOTAL = L * NS;
+    constexpr int BPT = cdiv(TOTAL, TPB);
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+        const int g = threadIdx.x + b * TPB;
+        if (TOTAL % TPB == 0 || g < TOTAL) {
+            const int line = g % L, j = g / L;
+            cpx<T> v[R];
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                cpx<T> a = src[(j + q * NS) * L + line];
+                if (q > 0 && NS > 1) a = cmul(a, tw[(j * q) % N]);
+                v[q] = a;
+            }
+            dft_small<R, -1>(v);
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const T m = mul(line, j + q * NS);
+                v[q] = {v[q].x * m, v[q].y * m};
+            }
+            dft_small<R, +1>(v);
+#pragma unroll
+            for (int q = 0; q < R; ++q) dst[(j * R + q) * L + line] = v[q];
+        }
+    }
+    __syncthreads();
+}
+
 } // namespace lat

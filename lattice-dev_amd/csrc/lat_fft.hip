@@ -194,8 +194,9 @@ __global__ __launch_bounds__(TPB) void k_x_r2c(T *__restrict__ f, const T *__res
 // FUSE (MODE 2): the last forward pass, the -k^2 factor and the first pass of
 // the inverse (radices in reverse order) run in registers in one step, which
 // saves one LDS write + read of the tile and a barrier pair.
+// PP (with FUSE): the LDS passes run out of place between two tile buffers.
 template <typename T, int N, int TC, int MODE, int MR = 16, bool PF = true, int MINW = 1, int NT = TPB,
-          bool TWL = false, bool FUSE = false>
+          bool TWL = false, bool FUSE = false, bool PP = false>
 __global__ __launch_bounds__(NT, MINW) void k_col(cpx<T> *__restrict__ Fk, const cpx<T> *__restrict__ tw, int nxh,
                                              int n_other, size_t s_axis, size_t s_other, size_t s_arr,
                                              int n_arr, int ny, double kfac, size_t ms_k)
@@ -208,6 +209,7 @@ __global__ __launch_bounds__(NT, MINW) void k_col(cpx<T> *__restrict__ Fk, const
     //   same (now dead) registers so they are in flight during the remaining
     //   LDS passes; the last pass writes straight to HBM.
     __shared__ cpx<T> buf[TC * N];
+    __shared__ cpx<T> buf2[PP ? TC * N : 1];
     constexpr int NP = num_passes(N, MR);
     constexpr int SIGN0 = (MODE == 1) ? +1 : -1;
     constexpr int R0 = radix_at(N, 0, MR);
@@ -261,7 +263,28 @@ __global__ __launch_bounds__(NT, MINW) void k_col(cpx<T> *__restrict__ Fk, const
         auto store = [&](int line, int pos, cpx<T> a) {
             if (line < ncol) base[(size_t)pos * s_axis + line] = a;
         };
-        if constexpr (MODE == 2 && FUSE && NP >= 2) {
+        if constexpr (MODE == 2 && FUSE && PP && NP >= 2) {
+            // pass 0 left the tile in buf; forward passes 1 .. NP-2, fused,
+            // inverse passes 1 .. NP-2 alternate buffers; the data ends in buf2
+            lds_fft_range_pp<T, N, -1, TC, NT, 1, NP - 1, MR, false, 0>(buf, buf2, twp);
+            constexpr int FB = (NP - 2) & 1; // buffer holding the forward result
+            const int jj = wmag(o, ny);
+            const float kf = float(kfac);
+            auto mul = [&](int c, int n) -> T {
+                const int ii = ct * TC + c;
+                const int kk = wmag(n, N);
+                if constexpr (sizeof(T) == 4) {
+                    return -float(ii * ii + jj * jj + kk * kk) * kf;
+                } else {
+                    const double rad2 = double(ii * ii) + double(jj * jj) + double(kk * kk);
+                    return T(-rad2 * kfac);
+                }
+            };
+            fused_last_first_pp<T, N, TC, NT, MR>(FB ? buf2 : buf, FB ? buf : buf2, twp, mul);
+            lds_fft_range_pp<T, N, +1, TC, NT, 1, NP - 1, MR, true, FB ^ 1>(buf, buf2, twp);
+            static_assert(((FB ^ 1) ^ ((NP - 2) & 1)) == 1, "inverse result in buf2");
+            stockham_pass_out<T, N, NP - 1, +1, TC, NT, true, 1, MR, true>(buf2, 1, TC, twp, store);
+        } else if constexpr (MODE == 2 && FUSE && NP >= 2) {
             lds_fft_range<T, N, -1, TC, NT, true, 1, 1, NP - 1, MR>(buf, 1, TC, twp);
             const int jj = wmag(o, ny);
             const float kf = float(kfac);
@@ -659,18 +682,20 @@ static_assert(c2r_grid<float>() <= MAX_GRID && c2r_grid<double>() <= MAX_GRID, "
 
 // y/z column passes: TC adjacent kx columns per tile.  y passes: 8 columns
 // at N = 1024 (128 B rows in FP64), radix <= 16, 512 threads.  Fused z pass:
-// radix <= 16 with the last forward / first inverse pass in registers
-// (FUSE); FP32 8 columns, 512 threads; FP64 4 columns, 256 threads, with the
-// next tile's first-pass loads in flight during the LDS passes (PF).
+// last forward / first inverse pass in registers (FUSE), LDS passes out of
+// place between two tile buffers (PP), 512 threads; FP32 8 columns, radix
+// <= 16; FP64 4 columns, radix <= 8, with the next tile's first-pass loads
+// in flight during the LDS passes (PF).
 template <typename T, int N, int MODE>
 struct ColCfg {
     static constexpr bool Z = MODE == 2;
     static constexpr bool F64 = sizeof(T) == 8;
     static constexpr int TC = (Z && F64) ? clampi(4096 / N, 4, 64) : clampi(8192 / N, 8, 64);
-    static constexpr int MR = 16;
+    static constexpr int MR = (Z && F64) ? 8 : 16;
     static constexpr bool PF = Z && F64;
-    static constexpr int NT = (Z && F64) ? 256 : 512;
+    static constexpr int NT = 512;
     static constexpr bool FUSE = Z;
+    static constexpr bool PP = Z;
     static constexpr int GRID = Z ? (F64 ? 1024 : 8192) : 65536;
     // FP32 y passes at N = 1024: tiles streamed into LDS (k_col_stream),
     // 1024 threads, one persistent block per CU
@@ -754,7 +779,7 @@ static void col_go(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int n_other, size_t s_a
         }
     }
     const int g = grid_for(((c.nxh + TC - 1) / TC) * n_other * n_arrays, C::GRID);
-    hipLaunchKernelGGL((k_col<T, N, TC, MODE, C::MR, C::PF, 1, C::NT, true, C::FUSE>), dim3(g, c.B),
+    hipLaunchKernelGGL((k_col<T, N, TC, MODE, C::MR, C::PF, 1, C::NT, true, C::FUSE, C::PP>), dim3(g, c.B),
                        dim3(C::NT), 0, c.stream, Fk, tw, c.nxh, n_other, s_axis, s_other, c.nk, n_arrays, c.ny, kfac,
                        c.ms.k);
 }
