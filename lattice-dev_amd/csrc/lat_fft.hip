@@ -683,16 +683,16 @@ static_assert(c2r_grid<float>() <= MAX_GRID && c2r_grid<double>() <= MAX_GRID, "
 // y/z column passes: TC adjacent kx columns per tile.  y passes: 8 columns
 // at N = 1024 (128 B rows in FP64), radix <= 16, 512 threads.  Fused z pass:
 // last forward / first inverse pass in registers (FUSE), LDS passes out of
-// place between two tile buffers (PP), 512 threads; FP32 8 columns, radix
-// <= 16; FP64 4 columns, radix <= 8, with the next tile's first-pass loads
-// in flight during the LDS passes (PF).
+// place between two tile buffers (PP), the next tile's first-pass loads in
+// flight during the LDS passes (PF), 512 threads; FP32 8 columns, radix
+// <= 16; FP64 4 columns, radix <= 8.
 template <typename T, int N, int MODE>
 struct ColCfg {
     static constexpr bool Z = MODE == 2;
     static constexpr bool F64 = sizeof(T) == 8;
     static constexpr int TC = (Z && F64) ? clampi(4096 / N, 4, 64) : clampi(8192 / N, 8, 64);
     static constexpr int MR = (Z && F64) ? 8 : 16;
-    static constexpr bool PF = Z && F64;
+    static constexpr bool PF = Z;
     static constexpr int NT = 512;
     static constexpr bool FUSE = Z;
     static constexpr bool PP = Z;
