@@ -34,6 +34,19 @@ __device__ __forceinline__ cpx<T> csub(cpx<T> a, cpx<T> b) { return {a.x - b.x, 
 template <typename T>
 __device__ __forceinline__ cpx<T> cconj(cpx<T> a) { return {a.x, -a.y}; }
 
+// ---------------------------------------------------------------- LDS barrier
+// Workgroup barrier for LDS data: the wave's LDS operations complete and
+// every wave arrives, with the fences restricted to the LDS address space.
+// Unlike __syncthreads(), global memory operations (prefetched loads,
+// stores of results) stay in flight across it.  Data that reaches LDS
+// through the vector memory path (LDS-direct loads) needs its own vmcnt wait.
+__device__ __forceinline__ void lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // ---------------------------------------------------------------- schedule
 // Largest radix first (fewest LDS round trips), capped at MR: 16, 8, 4, 2,
 // then 3, 5.
@@ -246,7 +259,7 @@ __device__ __forceinline__ void stockham_pass(cpx<T> *buf, int lstr, int estr, c
             oidx[b] = line * lstr + ((j / NS) * NS * R + k) * estr;
         }
     }
-    __syncthreads();
+    lds_sync();
 #pragma unroll
     for (int b = 0; b < BPT; ++b) {
         const int g = tid + b * TPB;
@@ -255,7 +268,7 @@ __device__ __forceinline__ void stockham_pass(cpx<T> *buf, int lstr, int estr, c
             for (int q = 0; q < R; ++q) buf[oidx[b] + q * NS * estr] = v[b][q];
         }
     }
-    __syncthreads();
+    lds_sync();
 }
 
 template <typename T, int N, int SIGN, int L, int TPB, bool COLMAJOR, int TWS, int P = 0>
@@ -358,7 +371,7 @@ __device__ __forceinline__ void fused_last_first(cpx<T> *buf, const cpx<T> *__re
             pos[b] = j * R * L + line;
         }
     }
-    __syncthreads();
+    lds_sync();
 #pragma unroll
     for (int b = 0; b < BPT; ++b) {
         const int g = threadIdx.x + b * TPB;
@@ -367,7 +380,7 @@ __device__ __forceinline__ void fused_last_first(cpx<T> *buf, const cpx<T> *__re
             for (int q = 0; q < R; ++q) buf[pos[b] + q * L] = v[b][q];
         }
     }
-    __syncthreads();
+    lds_sync();
 }
 
 // ---------------------------------------------------------------- ping-pong passes
@@ -406,7 +419,7 @@ __device__ __forceinline__ void stockham_pass_pp(const cpx<T> *__restrict__ src,
             for (int q = 0; q < R; ++q) dst[(o + q * NS) * L + line] = v[q];
         }
     }
-    __syncthreads();
+    lds_sync();
 }
 
 // Passes [P, PEND) alternating between b0 and b1; PAR = buffer holding the
@@ -454,7 +467,7 @@ __device__ __forceinline__ void fused_last_first_pp(const cpx<T> *__restrict__ s
             for (int q = 0; q < R; ++q) dst[(j * R + q) * L + line] = v[q];
         }
     }
-    __syncthreads();
+    lds_sync();
 }
 
 } // namespace lat

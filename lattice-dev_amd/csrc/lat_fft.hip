@@ -66,7 +66,7 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NCH], double *out
         for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
         if (lane == 0) red[c][wid] = x;
     }
-    __syncthreads();
+    lds_sync();
     if (threadIdx.x < NCH) {
         double s = 0.0;
 #pragma unroll
@@ -86,7 +86,7 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NCH], double *out
     const cpx<T> *twp = tw;                                                       \
     if constexpr (TWL) {                                                          \
         for (int i_ = threadIdx.x; i_ < (NTW); i_ += (NTHREADS)) twl_[i_] = tw[i_]; \
-        __syncthreads();                                                          \
+        lds_sync();                                                          \
         twp = twl_;                                                               \
     }
 
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(TPB) void k_x_r2c(T *__restrict__ f, const T *__res
 #pragma unroll
             for (int e = 0; e < VW; e += 2) buf[line * LS + (i0 + e) / 2] = cpx<T>{v.e[e], v.e[e + 1]};
         }
-        __syncthreads();
+        lds_sync();
         lds_fft_range<T, M, -1, L, TPB, false, 2, 0, num_passes(M, MR), MR>(buf, LS, 1, twp);
         // real-FFT split: X[k] = (Z[k] + Z*[M-k])/2 - i W^k (Z[k] - Z*[M-k])/2
         for (int idx = threadIdx.x; idx < L * (M / 2 + 1); idx += TPB) {
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(TPB) void k_x_r2c(T *__restrict__ f, const T *__res
                 z[k2] = {s.x + wd.y, s.y - wd.x};
             }
         }
-        __syncthreads();
+        lds_sync();
         if constexpr (sizeof(T) == 4) {
             // pairs of complex per lane (16 bytes); the odd last element k = M alone
             for (int idx = threadIdx.x; idx < L * (M / 2 + 1); idx += TPB) {
@@ -185,7 +185,7 @@ __global__ __launch_bounds__(TPB) void k_x_r2c(T *__restrict__ f, const T *__res
                 if (row < rows) Fk[(size_t)row * P + k] = buf[line * LS + k];
             }
         }
-        __syncthreads();
+        lds_sync();
     }
 }
 
@@ -259,7 +259,7 @@ __global__ __launch_bounds__(NT, MINW) void k_col(cpx<T> *__restrict__ Fk, const
             }
         }
         if (PF && t + ts < t1) load(t + ts);
-        __syncthreads();
+        lds_sync();
         auto store = [&](int line, int pos, cpx<T> a) {
             if (line < ncol) base[(size_t)pos * s_axis + line] = a;
         };
@@ -315,7 +315,7 @@ __global__ __launch_bounds__(NT, MINW) void k_col(cpx<T> *__restrict__ Fk, const
                 cpx<T> a = buf[idx];
                 buf[idx] = {a.x * m, a.y * m};
             }
-            __syncthreads();
+            lds_sync();
             lds_fft_range<T, N, +1, TC, NT, true, 1, 0, NP - 1, MR>(buf, 1, TC, twp);
             stockham_pass_out<T, N, NP - 1, +1, TC, NT, true, 1, MR>(buf, 1, TC, twp, store);
         } else if constexpr (NP == 1) {
@@ -325,7 +325,7 @@ __global__ __launch_bounds__(NT, MINW) void k_col(cpx<T> *__restrict__ Fk, const
             lds_fft_range<T, N, SIGN0, TC, NT, true, 1, 1, NP - 1, MR>(buf, 1, TC, twp);
             stockham_pass_out<T, N, NP - 1, SIGN0, TC, NT, true, 1, MR>(buf, 1, TC, twp, store);
         }
-        __syncthreads();
+        lds_sync();
     }
 }
 
@@ -391,7 +391,7 @@ __global__ __launch_bounds__(NT) void k_col_stream(cpx<T> *__restrict__ Fk, cons
         // this tile's loads (and the previous tile's stores) complete; every
         // thread is done with the other buffer, which now takes the next tile
         __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
+        lds_sync();
         if (t + ts < t1) {
             int nc2, ct2, o2;
             stream_tile<T, N, TC, NT>(tile_base(t + ts, nc2, ct2, o2), s_axis, bufs[cur ^ 1]);
@@ -430,7 +430,7 @@ __global__ __launch_bounds__(NT) void k_col_stream(cpx<T> *__restrict__ Fk, cons
                 cpx<T> a = buf[idx];
                 buf[idx] = {a.x * m, a.y * m};
             }
-            __syncthreads();
+            lds_sync();
             lds_fft_range<T, N, +1, TC, NT, true, 1, 0, NP - 1, MR>(buf, 1, TC, twp);
             stockham_pass_out<T, N, NP - 1, +1, TC, NT, true, 1, MR>(buf, 1, TC, twp, store);
         } else {
@@ -548,7 +548,7 @@ __global__ __launch_bounds__(TPB, MINW) void k_x_c2r(const cpx<T> *__restrict__ 
                 }
             }
         }
-        __syncthreads();
+        lds_sync();
         // Hermitian pre-split: Z[k] = (X[k] + X*[M-k]) + i e^{+2 pi i k/N} (X[k] - X*[M-k]),
         // with Im X[0] = Im X[M] = 0 (imaginary parts of the x-DC and x-Nyquist
         // coefficients are discarded, as numpy irfftn / FFTW c2r rows do).
@@ -575,7 +575,7 @@ __global__ __launch_bounds__(TPB, MINW) void k_x_c2r(const cpx<T> *__restrict__ 
                 z[k2] = {s.x - wd.y, s.y + wd.x};
             }
         }
-        __syncthreads();
+        lds_sync();
         if (nf == 2) lds_fft_range<T, M, +1, 2 * L, TPB, false, 2, 0, num_passes(M, MR), MR>(buf, LS, 1, twp);
         else lds_fft_range<T, M, +1, L, TPB, false, 2, 0, num_passes(M, MR), MR>(buf, LS, 1, twp);
         // 4. epilogue: VW consecutive sites per thread, every global access a
@@ -650,7 +650,7 @@ __global__ __launch_bounds__(TPB, MINW) void k_x_c2r(const cpx<T> *__restrict__ 
                 }
             }
         }
-        __syncthreads();
+        lds_sync();
     }
     if constexpr (EPI == EPI_KICK || EPI == EPI_REDUCE)
         block_reduce_store<NRED>(acc, partials + (size_t)blockIdx.x * NRED);
