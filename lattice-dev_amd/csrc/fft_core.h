@@ -48,11 +48,12 @@ __device__ __forceinline__ void lds_sync()
 }
 
 // ---------------------------------------------------------------- schedule
-// Largest radix first (fewest LDS round trips), capped at MR: 16, 8, 4, 2,
-// then 3, 5.
+// Largest radix first (fewest LDS round trips), capped at MR: 32, 16, 8, 4,
+// 2, then 3, 5.
 constexpr int pick_radix(int rem, int mr = 16)
 {
-    return (mr >= 16 && rem % 16 == 0) ? 16
+    return (mr >= 32 && rem % 32 == 0) ? 32
+           : (mr >= 16 && rem % 16 == 0) ? 16
            : (mr >= 8 && rem % 8 == 0) ? 8
            : (rem % 4 == 0) ? 4
            : (rem % 2 == 0) ? 2
@@ -152,6 +153,49 @@ __device__ __forceinline__ void dft16(cpx<T> *v)
     }
 }
 
+// multiply by e^{SIGN 2 pi i m / 32} for a compile-time m (even m reduce to
+// the 16-point rotations)
+template <int M, int SIGN, typename T>
+__device__ __forceinline__ cpx<T> rot32(cpx<T> a)
+{
+    constexpr int m = ((M % 32) + 32) % 32;
+    if constexpr (m % 2 == 0) {
+        return rot16<m / 2, SIGN>(a);
+    } else {
+        // cos and sin of 2 pi m / 32 for odd m, from the first-octant values
+        // cos(pi/16), sin(pi/16), cos(3 pi/16), sin(3 pi/16)
+        constexpr double A = 0.98078528040323044912618223613424, B = 0.19509032201612826784828486847702;
+        constexpr double C = 0.83146961230254523707878837761791, D = 0.55557023301960222474283081394853;
+        constexpr double cs[16] = {A, C, D, B, -B, -D, -C, -A, -A, -C, -D, -B, B, D, C, A};
+        constexpr double sn[16] = {B, D, C, A, A, C, D, B, -B, -D, -C, -A, -A, -C, -D, -B};
+        const T c = T(cs[m / 2]), s = T(SIGN < 0 ? -sn[m / 2] : sn[m / 2]);
+        return {a.x * c - a.y * s, a.x * s + a.y * c};
+    }
+}
+
+// 32-point DFT as 2 x 16: n = 2 n1 + n2, k = k1 + 16 k2.
+template <int SIGN, typename T>
+__device__ __forceinline__ void dft32(cpx<T> *v)
+{
+    cpx<T> a[16], b[16];
+#pragma unroll
+    for (int n1 = 0; n1 < 16; ++n1) {
+        a[n1] = v[2 * n1];
+        b[n1] = v[2 * n1 + 1];
+    }
+    dft16<SIGN>(a);
+    dft16<SIGN>(b);
+#define LAT_R32(K) b[K] = rot32<K, SIGN>(b[K]);
+    LAT_R32(1) LAT_R32(2) LAT_R32(3) LAT_R32(4) LAT_R32(5) LAT_R32(6) LAT_R32(7) LAT_R32(8)
+    LAT_R32(9) LAT_R32(10) LAT_R32(11) LAT_R32(12) LAT_R32(13) LAT_R32(14) LAT_R32(15)
+#undef LAT_R32
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) {
+        v[k1] = cadd(a[k1], b[k1]);
+        v[k1 + 16] = csub(a[k1], b[k1]);
+    }
+}
+
 // 8-point DFT as 2 x 4: n = 2 n1 + n2, k = k1 + 4 k2.
 template <int SIGN, typename T>
 __device__ __forceinline__ void dft8(cpx<T> *v)
@@ -173,7 +217,9 @@ __device__ __forceinline__ void dft8(cpx<T> *v)
 template <int R, int SIGN, typename T>
 __device__ __forceinline__ void dft_small(cpx<T> *v)
 {
-    if constexpr (R == 16) {
+    if constexpr (R == 32) {
+        dft32<SIGN>(v);
+    } else if constexpr (R == 16) {
         dft16<SIGN>(v);
     } else if constexpr (R == 8) {
         dft8<SIGN>(v);

@@ -329,118 +329,6 @@ __global__ __launch_bounds__(NT, MINW) void k_col(cpx<T> *__restrict__ Fk, const
     }
 }
 
-// ---------------------------------------------------------------- y / z columns, streamed tiles
-// Same transforms as k_col, with the tiles streamed into LDS by the
-// LDS-direct loads of gfx950 (global_load_lds_dwordx4: each lane's 16 bytes
-// land at consecutive LDS addresses of its wave, no registers involved) and
-// double-buffered: tile t+1 streams into one LDS buffer while the passes of
-// tile t run in the other, so HBM latency hides behind the LDS passes at no
-// register cost.  Every pass runs from LDS.  The loads read whole TC-column
-// rows, which needs the k-space row pitch to be a multiple of TC (columns
-// past nx/2+1 are padding; their transforms are computed and not stored).
-typedef __attribute__((address_space(3))) char *lds_bytes_t;
-
-template <typename T, int N, int TC, int NT>
-__device__ __forceinline__ void stream_tile(const cpx<T> *base, size_t s_axis, cpx<T> *tile)
-{
-    constexpr int UPR = TC * (int)sizeof(cpx<T>) / 16; // 16-byte units per row
-    constexpr int UNITS = N * UPR;
-    static_assert(TC * sizeof(cpx<T>) % 16 == 0 && UNITS % NT == 0, "tile must split into whole wave loads");
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-#pragma unroll
-    for (int i = 0; i < UNITS / NT; ++i) {
-        const int u0 = (i * (NT / 64) + wave) * 64; // first unit of this wave's load
-        const int u = u0 + lane;
-        const int row = u / UPR, part = u - row * UPR;
-        const char *g = reinterpret_cast<const char *>(base + (size_t)row * s_axis) + part * 16;
-        __builtin_amdgcn_global_load_lds(g, (lds_bytes_t)(tile) + u0 * 16, 16, 0, 0);
-    }
-}
-
-template <typename T, int N, int TC, int MODE, int MR = 16, int NT = TPB, bool FUSE = true>
-__global__ __launch_bounds__(NT) void k_col_stream(cpx<T> *__restrict__ Fk, const cpx<T> *__restrict__ tw, int nxh,
-                                                  int n_other, size_t s_axis, size_t s_other, size_t s_arr,
-                                                  int n_arr, int ny, double kfac, size_t ms_k)
-{
-    constexpr bool TWL = true;
-    LAT_TWIDDLES_TO_LDS(N, NT)
-    Fk += blockIdx.y * ms_k;
-    __shared__ __align__(16) cpx<T> bufs[2][TC * N];
-    constexpr int NP = num_passes(N, MR);
-    constexpr int SIGN0 = (MODE == 1) ? +1 : -1;
-    const int nct = (nxh + TC - 1) / TC;
-    int t0, t1, ts;
-    tile_range(nct * n_other * n_arr, t0, t1, ts);
-    auto tile_base = [&](int t, int &ncol, int &ct, int &o) -> cpx<T> * {
-        ct = t % nct;
-        const int rest = t / nct;
-        o = rest % n_other;
-        const int arr = rest / n_other;
-        ncol = min(TC, nxh - ct * TC);
-        return Fk + (size_t)arr * s_arr + (size_t)o * s_other + (size_t)ct * TC;
-    };
-    if (t0 < t1) {
-        int ncol, ct, o;
-        stream_tile<T, N, TC, NT>(tile_base(t0, ncol, ct, o), s_axis, bufs[0]);
-    }
-    int cur = 0;
-    for (int t = t0; t < t1; t += ts, cur ^= 1) {
-        int ncol, ct, o;
-        cpx<T> *base = tile_base(t, ncol, ct, o);
-        cpx<T> *buf = bufs[cur];
-        // this tile's loads (and the previous tile's stores) complete; every
-        // thread is done with the other buffer, which now takes the next tile
-        __builtin_amdgcn_s_waitcnt(0);
-        lds_sync();
-        if (t + ts < t1) {
-            int nc2, ct2, o2;
-            stream_tile<T, N, TC, NT>(tile_base(t + ts, nc2, ct2, o2), s_axis, bufs[cur ^ 1]);
-        }
-        auto store = [&](int line, int pos, cpx<T> a) {
-            if (line < ncol) base[(size_t)pos * s_axis + line] = a;
-        };
-        if constexpr (MODE == 2 && FUSE && NP >= 2) {
-            lds_fft_range<T, N, -1, TC, NT, true, 1, 0, NP - 1, MR>(buf, 1, TC, twp);
-            const int jj = wmag(o, ny);
-            const float kf = float(kfac);
-            auto mul = [&](int c, int n) -> T {
-                const int ii = ct * TC + c;
-                const int kk = wmag(n, N);
-                if constexpr (sizeof(T) == 4) {
-                    // |k|^2 in lattice units is an integer below 2^24 (N <= 1024):
-                    // exact in FP32, one rounding in the product
-                    return -float(ii * ii + jj * jj + kk * kk) * kf;
-                } else {
-                    const double rad2 = double(ii * ii) + double(jj * jj) + double(kk * kk);
-                    return T(-rad2 * kfac);
-                }
-            };
-            fused_last_first<T, N, TC, NT, MR>(buf, twp, mul);
-            lds_fft_range<T, N, +1, TC, NT, true, 1, 1, NP - 1, MR, true>(buf, 1, TC, twp);
-            stockham_pass_out<T, N, NP - 1, +1, TC, NT, true, 1, MR, true>(buf, 1, TC, twp, store);
-        } else if constexpr (MODE == 2) {
-            lds_fft_range<T, N, -1, TC, NT, true, 1, 0, NP, MR>(buf, 1, TC, twp);
-            const int jj = wmag(o, ny);
-            for (int idx = threadIdx.x; idx < TC * N; idx += NT) {
-                const int c = idx % TC, n = idx / TC;
-                const int ii = ct * TC + c;
-                const int kk = wmag(n, N);
-                const double rad2 = double(ii * ii) + double(jj * jj) + double(kk * kk);
-                const T m = T(-rad2 * kfac);
-                cpx<T> a = buf[idx];
-                buf[idx] = {a.x * m, a.y * m};
-            }
-            lds_sync();
-            lds_fft_range<T, N, +1, TC, NT, true, 1, 0, NP - 1, MR>(buf, 1, TC, twp);
-            stockham_pass_out<T, N, NP - 1, +1, TC, NT, true, 1, MR>(buf, 1, TC, twp, store);
-        } else {
-            lds_fft_range<T, N, SIGN0, TC, NT, true, 1, 0, NP - 1, MR>(buf, 1, TC, twp);
-            stockham_pass_out<T, N, NP - 1, SIGN0, TC, NT, true, 1, MR>(buf, 1, TC, twp, store);
-        }
-    }
-    __builtin_amdgcn_s_waitcnt(0);
-}
-
 // ---------------------------------------------------------------- x C2R + epilogue
 template <typename T, int N, int L, int EPI, int MR = 4, bool PRE = true, bool TWL = false, int MINW = 1>
 __global__ __launch_bounds__(TPB, MINW) void k_x_c2r(const cpx<T> *__restrict__ Fk, const cpx<T> *__restrict__ tw,
@@ -690,17 +578,18 @@ template <typename T, int N, int MODE>
 struct ColCfg {
     static constexpr bool Z = MODE == 2;
     static constexpr bool F64 = sizeof(T) == 8;
-    static constexpr int TC = (Z && F64) ? clampi(4096 / N, 4, 64) : clampi(8192 / N, 8, 64);
-    static constexpr int MR = (Z && F64) ? 8 : 16;
-    static constexpr bool PF = Z;
-    static constexpr int NT = 512;
+    // radix-32 passes (1024 = 32 x 32: two passes, one LDS round trip per
+    // transform) where the 1024^3 sweeps measured a gain: every y pass and
+    // the FP64 z pass, 16 FP32 / 8 FP64 columns per tile
+    static constexpr bool R32 = N == 1024 && !(Z && !F64);
+    static constexpr int TC = R32 ? (F64 ? 8 : 16)
+                                  : (Z && F64) ? clampi(4096 / N, 4, 64) : clampi(8192 / N, 8, 64);
+    static constexpr int MR = R32 ? 32 : (Z && F64) ? 8 : 16;
+    static constexpr bool PF = Z && !R32;
+    static constexpr int NT = (R32 && F64) ? 256 : 512;
     static constexpr bool FUSE = Z;
-    static constexpr bool PP = Z;
-    static constexpr int GRID = Z ? (F64 ? 1024 : 8192) : 65536;
-    // FP32 y passes at N = 1024: tiles streamed into LDS (k_col_stream),
-    // 1024 threads, one persistent block per CU
-    static constexpr bool STREAM = !Z && !F64 && N == 1024;
-    static constexpr int STREAM_NT = 1024, STREAM_GRID = 256;
+    static constexpr bool PP = Z && !R32;
+    static constexpr int GRID = Z ? ((F64 && !R32) ? 1024 : 8192) : 65536;
 };
 
 #define LAT_FOR_SIZES(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(48) X(96) X(192)
@@ -767,17 +656,6 @@ static void col_go(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int n_other, size_t s_a
                    double kfac)
 {
     using C = ColCfg<T, N, MODE == 2 ? 2 : 0>;
-    if constexpr (C::STREAM && TC == C::TC) {
-        // whole-row loads read TC columns from every tile start: the row pitch
-        // must cover the last tile
-        if (((c.nxh + TC - 1) / TC) * TC <= c.P) {
-            const int g = grid_for(((c.nxh + TC - 1) / TC) * n_other * n_arrays, C::STREAM_GRID);
-            hipLaunchKernelGGL((k_col_stream<T, N, TC, MODE, C::MR, C::STREAM_NT, false>), dim3(g, c.B),
-                               dim3(C::STREAM_NT), 0, c.stream, Fk, tw, c.nxh, n_other, s_axis, s_other, c.nk,
-                               n_arrays, c.ny, kfac, c.ms.k);
-            return;
-        }
-    }
     const int g = grid_for(((c.nxh + TC - 1) / TC) * n_other * n_arrays, C::GRID);
     hipLaunchKernelGGL((k_col<T, N, TC, MODE, C::MR, C::PF, 1, C::NT, true, C::FUSE, C::PP>), dim3(g, c.B),
                        dim3(C::NT), 0, c.stream, Fk, tw, c.nxh, n_other, s_axis, s_other, c.nk, n_arrays, c.ny, kfac,
