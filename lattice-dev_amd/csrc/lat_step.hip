@@ -427,19 +427,22 @@ struct ScalarEnv {
 
 // One workgroup: ordered FP64 reduction of `npart` partials (Neumaier
 // compensated per channel), then the split operators' scalar parts in order.
-__global__ __launch_bounds__(TPB) void k_scalar(DevScalars *__restrict__ sc, const double *__restrict__ partials,
-                                                int npart, ScalarOps ops, ScalarEnv env)
+constexpr int SCALAR_TPB = 1024; // 8 reduction channels x 128 lanes
+__global__ __launch_bounds__(SCALAR_TPB) void k_scalar(DevScalars *__restrict__ sc,
+                                                       const double *__restrict__ partials, int npart, ScalarOps ops,
+                                                       ScalarEnv env)
 {
     // one workgroup per member of a batch
     sc += blockIdx.x;
     partials += (size_t)blockIdx.x * MAX_GRID * NRED;
     env.md.p0 = sc->mp[0];
     env.md.p1 = sc->mp[1];
-    __shared__ double sh[NRED][32];
-    const int ch = threadIdx.x >> 5, lane = threadIdx.x & 31; // 8 channels x 32 lanes
+    constexpr int LANES = SCALAR_TPB / NRED;
+    __shared__ double sh[NRED][LANES];
+    const int ch = threadIdx.x / LANES, lane = threadIdx.x % LANES;
     if (npart > 0) {
         double s = 0.0, comp = 0.0;
-        for (int g = lane; g < npart; g += 32) {
+        for (int g = lane; g < npart; g += LANES) {
             const double x = partials[(size_t)g * NRED + ch];
             const double t = s + x;
             comp += (fabs(s) >= fabs(x)) ? (s - t) + x : (x - t) + s;
@@ -449,7 +452,7 @@ __global__ __launch_bounds__(TPB) void k_scalar(DevScalars *__restrict__ sc, con
         __syncthreads();
         if (lane == 0) {
             double tot = 0.0, c2 = 0.0;
-            for (int q = 0; q < 32; ++q) {
+            for (int q = 0; q < LANES; ++q) {
                 const double x = sh[ch][q];
                 const double t = tot + x;
                 c2 += (fabs(tot) >= fabs(x)) ? (tot - t) + x : (x - t) + tot;
@@ -570,7 +573,7 @@ int launch_scalar(Ctx &c, const ScalarOps &ops, int npart)
     ScalarEnv env{c.cfg.time_var, (double)c.nvol,
                   ModelDev{c.cfg.model, c.nfld, c.cfg.model_params[0], c.cfg.model_params[1]}};
     const int pi = prof_begin(c, LAT_ST_SCALAR);
-    hipLaunchKernelGGL(k_scalar, dim3(c.B), dim3(TPB), 0, c.stream, c.dsc, c.partials, npart, ops, env);
+    hipLaunchKernelGGL(k_scalar, dim3(c.B), dim3(SCALAR_TPB), 0, c.stream, c.dsc, c.partials, npart, ops, env);
     prof_end(c, pi);
     LAT_HIP(&c, hipGetLastError());
     return 0;

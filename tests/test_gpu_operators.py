@@ -8,7 +8,7 @@ import oracle as O
 pytestmark = pytest.mark.gpu
 
 SIZES = [(8, 8, 8), (16, 16, 16), (32, 32, 32), (64, 64, 64), (16, 32, 8), (48, 48, 48), (96, 32, 16),
-         (192, 16, 8), (128, 16, 16), (256, 8, 8), (1024, 8, 8), (8, 512, 8), (8, 8, 1024)]
+         (192, 16, 8), (128, 16, 16), (256, 8, 8), (1024, 8, 8), (8, 512, 8), (8, 1024, 8), (8, 8, 1024)]
 
 
 def _lat(la, shape, **kw):
