@@ -235,11 +235,11 @@ def ensemble_leg(la, args):
     """Batched ensemble: 96 realisations of the two-field model, 32 x 64^3 +
     48 x 128^3 + 16 x 192^3, fixed L = 10/m (dt = dx/40 per size class),
     g^2/m^2 log-uniform in [1e4, 1e6] (seed 11), realisation r seeded 3000 + r.
-    One batched launch per sweep per size class; the Yoshida-4 call is
+    One launch per FFT sweep for all size classes; the Yoshida-4 call is
     captured into a hipGraph on its first use and replayed.  Reported:
-    aggregate site-steps/s with the graph and eagerly, and the spectra of one
-    realisation per class against a single-context run of the same
-    realisation."""
+    aggregate site-steps/s with the graph, eagerly, and with one launch per
+    sweep per class (graph), and the spectra of one realisation per class
+    against a single-context run of the same realisation."""
     classes = [(64, 32, 10.0), (128, 48, 10.0), (192, 16, 10.0)]
     rng = np.random.default_rng(11)
     g2s = np.exp(rng.uniform(np.log(1e4), np.log(1e6), sum(c for _, c, _ in classes)))
@@ -254,8 +254,12 @@ def ensemble_leg(la, args):
            "classes": [{"n": n, "count": c, "L": L} for n, c, L in classes], "form": args.ens_form,
            "steps": k, "sites_per_field": sites}
     got = {}
-    for graph in (True, False):
+    # single: one launch per FFT sweep for all classes; per_class: one per class
+    for name, single, graph in (("graph", True, True), ("eager", True, False), ("graph_per_class", False, True),
+                                ("eager_per_class", False, False)):
         with la.Ensemble(classes, precision=prec, members=params) as ens:
+            out.setdefault("autotuner_picked_single_launch", ens.single_launch(None))
+            assert ens.single_launch(single) == single
             ens.use_graph(graph)
             views = [ens.member(i) for i in range(ens.size)]
             for i, v in enumerate(views):
@@ -266,8 +270,8 @@ def ensemble_leg(la, args):
             ens.symp(4, dts, k)
             ens.synchronize()
             t = time.perf_counter() - t0
-            out["graph" if graph else "eager"] = {"seconds": t, "site_steps_per_s": sites * k / t}
-            if graph:
+            out[name] = {"seconds": t, "site_steps_per_s": sites * k / t}
+            if name == "graph":
                 got = {i: views[i].output_spectra(0) for i in samples}
             for v in views:
                 v.close()
@@ -284,7 +288,10 @@ def ensemble_leg(la, args):
     # bins >= 1e-6 of the peak; the ensemble's autotuned launch shapes may
     # differ from the single context's, which changes the rounding only
     out["spectrum_rel_err_vs_single_runs"] = errs
-    out["value"] = out["graph"]["site_steps_per_s"]
+    # the mode the ensemble's autotuner keeps
+    best = "graph" if out["autotuner_picked_single_launch"] else "graph_per_class"
+    out["value"] = out[best]["site_steps_per_s"]
+    out["value_mode"] = best
     print(json.dumps(out))
 
 

@@ -110,7 +110,13 @@ int  lat_set_tuning(lat_ctx *ctx, const int *in);   /* in[4] of 0/1, as lat_tuni
  * class-major order (nullable).  All classes share one HIP stream.
  * lat_ensemble_symp advances every member of class k by nsteps steps of
  * dt[k]: the first call with given (order, nsteps, dt) is captured into a
- * hipGraph (one launch per sweep per class), later calls replay it.  Members
+ * hipGraph, later calls replay it.  Two launch modes: every FFT sweep of a
+ * force evaluation as ONE launch covering all classes in lockstep (up to 4
+ * classes of spectral, cubic lattices of 16, 32, 48, 64, 96, 128, 192 or 256
+ * sites per side), or one launch per sweep per class.  At creation the
+ * autotuner times both and keeps the faster.  lat_ensemble_single_launch(ens,
+ * 1 or 0) forces a mode, (ens, -1) only queries; it returns 1 when the single
+ * launch is in use (0 if not, or if the classes do not allow it).  Members
  * are addressed class-major through lat_ensemble_member views. */
 typedef struct lat_ensemble lat_ensemble;
 int  lat_ensemble_create(const lat_config *base, int nclass, const int *n, const int *count,
@@ -120,6 +126,10 @@ const char *lat_ensemble_last_error(const lat_ensemble *ens);
 int  lat_ensemble_size(const lat_ensemble *ens);
 int  lat_ensemble_member(lat_ensemble *ens, int index, lat_ctx **view);
 int  lat_ensemble_use_graph(lat_ensemble *ens, int enable);   /* default 1 */
+int  lat_ensemble_single_launch(lat_ensemble *ens, int enable); /* -1: query */
+/* launch shapes of size class k, as lat_set_tuning / lat_tuning */
+int  lat_ensemble_set_tuning(lat_ensemble *ens, int k, const int *in);
+int  lat_ensemble_tuning(const lat_ensemble *ens, int k, int *out);
 int  lat_ensemble_symp(lat_ensemble *ens, int order, const double *dt, int nsteps);
 int  lat_ensemble_synchronize(lat_ensemble *ens);
 const char *lat_last_error(const lat_ctx *ctx);

@@ -1803,14 +1803,17 @@ struct lat_ensemble {
     hipStream_t stream = nullptr;
     int device = 0;
     bool use_graph = true;
+    bool single_launch = false; // one launch per sweep for all classes (set by ens_tune_launch)
     struct Entry {
         int order, nsteps;
+        bool single;
         std::vector<double> dt;
         hipGraph_t graph;
         hipGraphExec_t exec;
     };
     std::vector<Entry> graphs;
     std::string last_error;
+    float tune_ms[2] = {0.f, 0.f}; // sweep timing per class / single launch
 };
 
 static int ens_fail(lat_ensemble *e, hipError_t err, const char *where)
@@ -1819,10 +1822,245 @@ static int ens_fail(lat_ensemble *e, hipError_t err, const char *where)
     return -3;
 }
 
+// Lockstep form of run_ops over the size classes of an ensemble.  The op
+// sequences of the classes have the same types (same integrator, order and
+// step count; only the step sizes differ), so every sweep of a force
+// evaluation is one launch for all classes (lat_multi.hip); the scalar,
+// reduce and drift kernels stay one launch per class (one workgroup per
+// member, or an element-wise sweep).  Per member the arithmetic is that of
+// run_ops on the class alone.
+template <typename T>
+static int force_multi(std::vector<Ctx *> &cs, bool drift, std::vector<int> &npart)
+{
+    const int K = (int)cs.size(), nf = cs[0]->nfld;
+    int rc;
+    for (int l = 0; l < nf; ++l)
+        if ((rc = launch_x_r2c_multi<T>(cs.data(), K, l, drift ? 1 : 0))) return rc;
+    if ((rc = launch_col_multi<T>(cs.data(), K, 1, 0, nf))) return rc;
+    if ((rc = launch_col_multi<T>(cs.data(), K, 2, 2, nf))) return rc;
+    if ((rc = launch_col_multi<T>(cs.data(), K, 1, 1, nf))) return rc;
+    return launch_x_c2r_multi<T>(cs.data(), K, nf, EPI_KICK, npart.data());
+}
+
+static int run_ops_multi(std::vector<Ctx *> &cs, const std::vector<std::vector<Op>> &ops)
+{
+    const size_t K = cs.size();
+    const bool f64 = cs[0]->prec == LAT_F64;
+    std::vector<int> npart(K, 0);
+    for (size_t k = 0; k < K; ++k)
+        if (!get_ke_valid(*cs[k])) {
+            npart[k] = f64 ? launch_reduce_state<double>(*cs[k]) : launch_reduce_state<float>(*cs[k]);
+            if (npart[k] < 0) return npart[k];
+        }
+    std::vector<ScalarOps> so(K);
+    int ndrift = 0;
+    bool kick_pending = false;
+    auto scalars = [&]() -> int {
+        for (size_t k = 0; k < K; ++k) {
+            const int rc = launch_scalar(*cs[k], so[k], npart[k]);
+            if (rc) return rc;
+        }
+        return 0;
+    };
+    auto drifts = [&]() -> int {
+        for (size_t k = 0; k < K; ++k) {
+            const int rc = f64 ? launch_drift<double>(*cs[k]) : launch_drift<float>(*cs[k]);
+            if (rc) return rc;
+        }
+        return 0;
+    };
+    auto begin_seg = [&]() {
+        for (size_t k = 0; k < K; ++k) {
+            so[k] = ScalarOps{};
+            if (kick_pending) {
+                so[k].type[so[k].n] = OP_FINISH_KICK;
+                so[k].h[so[k].n++] = 0.0;
+            }
+        }
+    };
+    begin_seg();
+    for (size_t i = 0; i < ops[0].size(); ++i) {
+        const int type = ops[0][i].type;
+        if (type == OP_KICK) {
+            for (size_t k = 0; k < K; ++k) {
+                so[k].type[so[k].n] = OP_PREP_KICK;
+                so[k].h[so[k].n++] = ops[k][i].h;
+            }
+            int rc = scalars();
+            if (rc) return rc;
+            rc = f64 ? force_multi<double>(cs, ndrift > 0, npart) : force_multi<float>(cs, ndrift > 0, npart);
+            if (rc) return rc;
+            ndrift = 0;
+            kick_pending = true;
+            begin_seg();
+        } else {
+            if (type == OP_DRIFT && ndrift == 2) {
+                // more drifts than the fused sweep applies: settle them now
+                int rc = scalars();
+                if (rc) return rc;
+                std::fill(npart.begin(), npart.end(), 0);
+                kick_pending = false;
+                if ((rc = drifts())) return rc;
+                ndrift = 0;
+                begin_seg();
+            }
+            if (so[0].n >= MAX_OPS - 1) {
+                cs[0]->last_error = "op segment overflow";
+                return -1;
+            }
+            for (size_t k = 0; k < K; ++k) {
+                so[k].type[so[k].n] = type;
+                so[k].h[so[k].n++] = ops[k][i].h;
+            }
+            if (type == OP_DRIFT) ++ndrift;
+        }
+    }
+    for (size_t k = 0; k < K; ++k)
+        if (so[k].n > 0 || npart[k] > 0) {
+            const int rc = launch_scalar(*cs[k], so[k], npart[k]);
+            if (rc) return rc;
+        }
+    if (ndrift > 0) {
+        const int rc = drifts();
+        if (rc) return rc;
+    }
+    for (Ctx *c : cs) set_ke_valid(*c, true);
+    return 0;
+}
+
+// Classes that one launch per sweep can carry: spectral Laplacian, no tensor
+// modes, cubic lattices of the sizes lat_multi.hip is built for.
+static bool ens_single_ok(const lat_ensemble *e)
+{
+    if (e->cls.empty() || (int)e->cls.size() > MULTI_MAXC) return false;
+    const Ctx &c0 = e->cls[0]->c;
+    for (const lat_ctx *h : e->cls) {
+        const Ctx &c = h->c;
+        if (c.cfg.laplacian != LAT_LAP_SPECTRAL || c.gw || c.prec != c0.prec || c.nfld != c0.nfld) return false;
+        if (!multi_supported(c.nx) || c.ny != c.nx || c.nz != c.nx) return false;
+    }
+    return true;
+}
+
+static bool ens_single(const lat_ensemble *e) { return e->single_launch && ens_single_ok(e); }
+
+// The sweeps of one force evaluation over all classes, in the non-mutating
+// modes the per-class autotuner times (x R2C without drift and the y/z
+// passes write only the k-space scratch, the x C2R only reduces).
+template <typename T>
+static int ens_sweeps(lat_ensemble *e, bool single)
+{
+    std::vector<Ctx *> cs;
+    for (lat_ctx *h : e->cls) cs.push_back(&h->c);
+    const int K = (int)cs.size(), nf = cs[0]->nfld;
+    int rc;
+    if (single) {
+        for (int l = 0; l < nf; ++l)
+            if ((rc = launch_x_r2c_multi<T>(cs.data(), K, l, 0))) return rc;
+        if ((rc = launch_col_multi<T>(cs.data(), K, 1, 0, nf))) return rc;
+        if ((rc = launch_col_multi<T>(cs.data(), K, 2, 2, nf))) return rc;
+        if ((rc = launch_col_multi<T>(cs.data(), K, 1, 1, nf))) return rc;
+        std::vector<int> np(K);
+        return launch_x_c2r_multi<T>(cs.data(), K, nf, EPI_REDUCE, np.data());
+    }
+    for (Ctx *cp : cs) {
+        Ctx &c = *cp;
+        const cpx<T> *twx = (const cpx<T> *)(sizeof(T) == 8 ? (void *)c.twx64 : (void *)c.twx32);
+        const cpx<T> *twy = (const cpx<T> *)(sizeof(T) == 8 ? (void *)c.twy64 : (void *)c.twy32);
+        const cpx<T> *twz = (const cpx<T> *)(sizeof(T) == 8 ? (void *)c.twz64 : (void *)c.twz32);
+        cpx<T> *Fk = (cpx<T> *)c.Fk;
+        for (int l = 0; l < nf; ++l)
+            if ((rc = launch_x_r2c<T>(c, twx, (T *)c.fld + (size_t)l * c.nvol, nullptr, Fk + (size_t)l * c.nk, 0, l)))
+                return rc;
+        if ((rc = launch_col<T>(c, twy, Fk, 1, 0, nf))) return rc;
+        if ((rc = launch_col<T>(c, twz, Fk, 2, 2, nf))) return rc;
+        if ((rc = launch_col<T>(c, twy, Fk, 1, 1, nf))) return rc;
+        const int g = launch_x_c2r<T>(c, twx, Fk, nf, EPI_REDUCE, nullptr, (const T *)c.fld, (T *)c.fldp);
+        if (g < 0) return g;
+    }
+    return 0;
+}
+
+// Autotuner, ensemble level: times those sweeps with one launch per sweep
+// for all classes and with one launch per sweep per class (shapes as tuned
+// per class), keeps the faster.  One kernel has one register and LDS
+// footprint, that of its largest shape, so the single launch can lose
+// occupancy on the smaller classes; which way it goes depends on the sizes.
+static int ens_tune_launch(lat_ensemble *e, int reps)
+{
+    e->single_launch = false;
+    if (!ens_single_ok(e)) return 0;
+    const bool f64 = e->cls[0]->c.prec == LAT_F64;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return -3;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return -3;
+    }
+    float t[2] = {0.f, 0.f};
+    int rc = 0;
+    for (int m = 0; m < 2 && !rc; ++m) {
+        auto run = [&]() { return f64 ? ens_sweeps<double>(e, m == 1) : ens_sweeps<float>(e, m == 1); };
+        if ((rc = run())) break;
+        if (hipEventRecord(e0, e->stream) != hipSuccess) {
+            rc = -3;
+            break;
+        }
+        for (int r = 0; r < reps && !rc; ++r) rc = run();
+        if (rc || hipEventRecord(e1, e->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+            rc = rc ? rc : -3;
+            break;
+        }
+        (void)hipEventElapsedTime(&t[m], e0, e1);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    for (lat_ctx *h : e->cls) set_ke_valid(h->c, false);
+    if (rc) {
+        e->last_error = "launch-mode timing failed";
+        return rc;
+    }
+    e->single_launch = t[1] < t[0];
+    e->tune_ms[0] = t[0];
+    e->tune_ms[1] = t[1];
+    return 0;
+}
+
 // One Yoshida call on every class; each starts from fresh reductions so that
 // the captured work is the same on every replay.
 static int ens_record(lat_ensemble *e, int order, const double *dt, int nsteps)
 {
+    if (ens_single(e)) {
+        std::vector<Ctx *> cs;
+        std::vector<std::vector<Op>> ops(e->cls.size());
+        for (size_t k = 0; k < e->cls.size(); ++k) {
+            Ctx &c = e->cls[k]->c;
+            if (build_symp(c, ops[k], order, dt[k], nsteps)) {
+                e->last_error = "symp order must be 2, 4, 6 or 8";
+                return -1;
+            }
+            if (ops[k].size() != ops[0].size()) {
+                e->last_error = "classes with different operator sequences";
+                return -1;
+            }
+            for (size_t i = 0; i < ops[k].size(); ++i)
+                if (ops[k][i].type != ops[0][i].type) {
+                    e->last_error = "classes with different operator sequences";
+                    return -1;
+                }
+            set_ke_valid(c, false);
+            cs.push_back(&c);
+        }
+        std::vector<bool> prof(cs.size());
+        for (size_t k = 0; k < cs.size(); ++k) {
+            prof[k] = cs[k]->prof.on;
+            cs[k]->prof.on = false;
+        }
+        const int rc = run_ops_multi(cs, ops);
+        for (size_t k = 0; k < cs.size(); ++k) cs[k]->prof.on = prof[k];
+        if (rc) e->last_error = cs[0]->last_error.empty() ? "multi-size launch failed" : cs[0]->last_error;
+        return rc;
+    }
     for (size_t k = 0; k < e->cls.size(); ++k) {
         Ctx &c = e->cls[k]->c;
         std::vector<Op> ops;
@@ -1883,6 +2121,10 @@ int lat_ensemble_create(const lat_config *base, int nclass, const int *n, const 
         e->first.push_back(total);
         total += count[k];
     }
+    if (ens_tune_launch(e, 3)) {
+        lat_ensemble_destroy(e);
+        return -3;
+    }
     *out = e;
     return 0;
 }
@@ -1933,8 +2175,9 @@ int lat_ensemble_symp(lat_ensemble *e, int order, const double *dt, int nsteps)
     if (nsteps <= 0) return 0;
     if (!e->use_graph) return ens_record(e, order, dt, nsteps);
     const std::vector<double> key(dt, dt + e->cls.size());
+    const bool single = ens_single(e);
     for (auto &g : e->graphs)
-        if (g.order == order && g.nsteps == nsteps && g.dt == key) {
+        if (g.order == order && g.nsteps == nsteps && g.dt == key && g.single == single) {
             hipError_t err = hipGraphLaunch(g.exec, e->stream);
             return err == hipSuccess ? 0 : ens_fail(e, err, "hipGraphLaunch");
         }
@@ -1954,9 +2197,38 @@ int lat_ensemble_symp(lat_ensemble *e, int order, const double *dt, int nsteps)
         (void)hipGraphDestroy(graph);
         return ens_fail(e, err, "hipGraphInstantiate");
     }
-    e->graphs.push_back({order, nsteps, key, graph, exec});
+    e->graphs.push_back({order, nsteps, single, key, graph, exec});
     err = hipGraphLaunch(exec, e->stream);
     return err == hipSuccess ? 0 : ens_fail(e, err, "hipGraphLaunch");
+}
+
+// Launch shapes of class k (as lat_set_tuning / lat_tuning); changing them
+// drops the captured graphs, which hold the old shapes.
+int lat_ensemble_set_tuning(lat_ensemble *e, int k, const int *in)
+{
+    if (!e || k < 0 || k >= (int)e->cls.size()) return -1;
+    const int rc = lat_set_tuning(e->cls[k], in);
+    if (rc) return rc;
+    (void)hipStreamSynchronize(e->stream);
+    for (auto &g : e->graphs) {
+        (void)hipGraphExecDestroy(g.exec);
+        (void)hipGraphDestroy(g.graph);
+    }
+    e->graphs.clear();
+    return 0;
+}
+
+int lat_ensemble_tuning(const lat_ensemble *e, int k, int *out)
+{
+    if (!e || k < 0 || k >= (int)e->cls.size()) return -1;
+    return lat_tuning(e->cls[k], out);
+}
+
+int lat_ensemble_single_launch(lat_ensemble *e, int enable)
+{
+    if (!e) return -1;
+    if (enable >= 0) e->single_launch = enable != 0;
+    return ens_single(e) ? 1 : 0;
 }
 
 int lat_ensemble_synchronize(lat_ensemble *e)

@@ -163,6 +163,20 @@ enum { EPI_STORE = 0, EPI_KICK = 1, EPI_REDUCE = 2, EPI_GWKICK = 3 };
 template <typename T>
 int launch_x_c2r(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int nf, int epi, T *out, const T *fld, T *fldp);
 
+// ---- one launch per sweep over the size classes of an ensemble (lat_multi.hip) ----
+// cs[0..ncls) are batches of cubic lattices of the sizes multi_supported()
+// accepts, sharing cs[0]'s stream; shapes follow each class's tuning.
+// Returns 0, or <0 (unsupported class set / HIP error).
+constexpr int MULTI_MAXC = 4;
+bool multi_supported(int n);
+template <typename T>
+int launch_x_r2c_multi(Ctx *const *cs, int ncls, int field, int drift);
+template <typename T>
+int launch_col_multi(Ctx *const *cs, int ncls, int axis, int mode, int n_arrays);
+// EPI_KICK or EPI_REDUCE; npart[k] = reduction partials per member of class k
+template <typename T>
+int launch_x_c2r_multi(Ctx *const *cs, int ncls, int nf, int epi, int *npart);
+
 // ---- step kernels (lat_step.hip) ----
 template <typename T>
 int launch_drift(Ctx &c);              // apply pending drifts of dsc to all fields (and u_ij)

@@ -63,7 +63,8 @@ EXPORTS = [
     "lat_spectrum_discrete_k", "lat_crossspec_discrete_k",
     "lat_create_batch", "lat_num_members", "lat_member",
     "lat_ensemble_create", "lat_ensemble_destroy", "lat_ensemble_last_error", "lat_ensemble_size",
-    "lat_ensemble_member", "lat_ensemble_use_graph", "lat_ensemble_symp", "lat_ensemble_synchronize",
+    "lat_ensemble_member", "lat_ensemble_use_graph", "lat_ensemble_single_launch", "lat_ensemble_symp",
+    "lat_ensemble_synchronize", "lat_ensemble_set_tuning", "lat_ensemble_tuning",
     "lat_autotune", "lat_tuning", "lat_set_tuning", "lat_copy_bandwidth", "lat_make_output",
     "lat_checkpoint", "lat_restore",
     "lat_corrector_create", "lat_corrector_destroy", "lat_corrector_last_error", "lat_corrector_set_layer",
@@ -546,6 +547,25 @@ class Ensemble:
 
     def use_graph(self, enable):
         self._check(lib().lat_ensemble_use_graph(self._h, int(bool(enable))), "use_graph")
+
+    def set_tuning(self, k, shape):
+        """Launch shapes (x R2C, y, z, x C2R; 0/1 each) of size class k."""
+        arr = (ctypes.c_int * 4)(*[int(v) for v in shape])
+        self._check(lib().lat_ensemble_set_tuning(self._h, k, arr), "set_tuning")
+
+    def tuning(self, k):
+        arr = (ctypes.c_int * 4)()
+        self._check(lib().lat_ensemble_tuning(self._h, k, arr), "tuning")
+        return tuple(arr)
+
+    def single_launch(self, enable=None):
+        """Force one launch per FFT sweep for all size classes (True) or one
+        per class (False); None only queries the mode the autotuner picked.
+        Returns whether the single launch is in use."""
+        rc = lib().lat_ensemble_single_launch(self._h, -1 if enable is None else int(bool(enable)))
+        if rc < 0:
+            self._check(rc, "single_launch")
+        return rc == 1
 
     def symp(self, order, dt, nsteps):
         """dt: one time step per class."""

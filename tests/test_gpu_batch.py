@@ -145,6 +145,50 @@ def test_ensemble_graph_replays_match_eager_and_single_runs(gpu):
         assert abs(graph[i][2] - ref[2]) < 1e-14, i
 
 
+@pytest.mark.parametrize("precision", ["f64", "mixed"])
+def test_ensemble_single_launch_matches_per_class_launches(gpu, precision):
+    """One launch per sweep for three size classes (a radix-3 size among them)
+    against one launch per sweep per class, with the same launch shapes: each
+    member keeps its tile loop, radix schedule and reduction slots, so states
+    and background scalars agree bitwise, eagerly and through the graph."""
+    print("case ensemble single launch", precision, flush=True)
+    classes = [(16, 2, 10.0), (48, 1, 10.0), (32, 2, 10.0)]
+    g2s = [1e4, 2e5, 5e4, 1e5, 2.5e5]
+    params = [(1.0, g) for g in g2s]
+    sizes = [16, 16, 48, 32, 32]
+    ics = [member_ic(n, 10.0, 400 + i) for i, n in enumerate(sizes)]
+    dts = [1e-3, 3e-4, 5e-4]
+    shapes = [(1, 1, 1, 1), (0, 0, 0, 0), (0, 1, 0, 1)]
+    prec = gpu.F64 if precision == "f64" else gpu.MIXED
+    out = {}
+    for single, graph in ((True, True), (True, False), (False, False)):
+        with gpu.Ensemble(classes, precision=prec, members=params) as ens:
+            assert ens.single_launch(None) in (True, False)  # the autotuner's pick
+            assert ens.single_launch(single) is single  # these classes allow both
+            for k, sh in enumerate(shapes):
+                ens.set_tuning(k, sh)
+                assert ens.tuning(k) == sh
+            ens.use_graph(graph)
+            views = [ens.member(i) for i in range(ens.size)]
+            for v, ic in zip(views, ics):
+                v.set_state(*ic, 1.0, 0.0)
+                v.calc_metric()
+            for nsteps in (2, 2, 1):  # the second call replays the first's graph
+                ens.symp(4, dts, nsteps)
+            ens.synchronize()
+            out[(single, graph)] = [v.get_state() for v in views]
+            for v in views:
+                v.close()
+    ref = out[(False, False)]
+    for key in ((True, True), (True, False)):
+        for i in range(len(sizes)):
+            for a, b in zip(out[key][i], ref[i]):
+                assert np.array_equal(np.asarray(a), np.asarray(b)), (key, i)
+    # classes the single launch does not cover run one launch per class
+    with gpu.Ensemble([(8, 1, 5.0), (16, 1, 5.0)]) as ens:
+        assert ens.single_launch(True) is False
+
+
 @pytest.mark.parametrize("n", [32, 64, 192])
 @pytest.mark.parametrize("precision", ["f64", "mixed"])
 def test_tuned_launch_shapes_keep_results(gpu, n, precision):
