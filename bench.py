@@ -64,11 +64,19 @@ def algorithmic_bytes(stage, n, nf, w):
         "z_lap": nf * nvol * 2 * wc,
         "y_inv": nf * nvol * 2 * wc,
         "x_c2r": nvol * nf * (wc + 3 * w),   # read F, phi; read + write pi
+        # separable form: x (one field): read phi, pi; write phi, lap.  y: read
+        # phi, read + write lap.  z: read phi, lap, the other field's phi, pi;
+        # write pi (all fields)
+        "sep_x": nvol * 4 * w,
+        "sep_y": nf * nvol * 3 * w,
+        "sep_z": nf * nvol * (4 + (1 if nf == 2 else 0)) * w,
     }.get(stage)
 
 
-def run_form(la, n, precision, steps, warmup, out_every, dt):
+def run_form(la, n, precision, steps, warmup, out_every, dt, sep=False):
     lat = make_lattice(la, n, precision)
+    if sep:
+        lat.set_laplacian_form(True)
     t0 = time.time()
     init_state(lat)
     lat.synchronize()
@@ -303,6 +311,8 @@ def main():
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--output-every", type=int, default=20)
     ap.add_argument("--forms", default="f64,mixed")
+    ap.add_argument("--lap-form", default="3d", choices=["3d", "sep"],
+                    help="mixed form: five-sweep 3-D spectral Laplacian or the separable three-sweep form")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=12)
@@ -361,7 +371,8 @@ def main():
     for form in forms:
         prec = la.F64 if form == "f64" else la.MIXED
         print(f"[bench] {form} {n}^3: warmup {args.warmup}, timing {args.steps} steps", file=sys.stderr, flush=True)
-        res[form] = run_form(la, n, prec, args.steps, args.warmup, args.output_every, dt)
+        res[form] = run_form(la, n, prec, args.steps, args.warmup, args.output_every, dt,
+                             sep=(form == "mixed" and args.lap_form == "sep"))
         r = res[form]
         print(f"[bench] {form}: {r['elapsed']:.3f} s, {n ** 3 * args.steps / r['elapsed']:.4e} site-steps/s "
               f"(init {r['t_init']:.1f} s)", file=sys.stderr, flush=True)
@@ -384,7 +395,9 @@ def main():
         "data": "synthetic: seeded vacuum fluctuations (device Philox sampler) on the homogeneous two-field background",
         "config": {"workload": f"{n}^3 two-field preheating g^2/m^2=2.5e5, cosmic FRW, spectral Laplacian, "
                                f"Yoshida-4, L=10/m, dt=dx/40, spectra every {args.output_every} steps",
-                   "form": head},
+                   "form": head,
+                   "laplacian_form": ("separable, 3 sweeps" if args.lap_form == "sep" and head == "mixed"
+                                      else "3-D, 5 sweeps")},
         "roofline": roofline(r["prof"], n, 2, w),
         "hbm_copy_GBs": round(copy_gbs, 1),
     }

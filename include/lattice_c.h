@@ -100,6 +100,14 @@ int  lat_copy_bandwidth(lat_ctx *ctx, size_t bytes, int reps, double *gbps);
  * and keeps the faster; fields are not modified.  Ensembles tune every size
  * class at creation.  lat_tuning reports the choices (0 = default shape):
  * out[4] = {x R2C, y, z, x C2R}. */
+/* Form of the spectral Laplacian in the field kicks (no reference
+ * counterpart; the reference has the 3-D form only): 0 = five HBM sweeps
+ * (x R2C, y, z * -k^2, y, x C2R + kick), 1 = the same operator as the sum of
+ * one-axis second derivatives in three sweeps (x: drift + D_xx; y: + D_yy;
+ * z: + D_zz, kick, reductions).  1 is for the mixed form at cubic 128, 256,
+ * 512 or 1024 (error otherwise); default 0. */
+int  lat_set_laplacian_form(lat_ctx *ctx, int form);
+int  lat_laplacian_form(lat_ctx *ctx);
 int  lat_autotune(lat_ctx *ctx);
 int  lat_tuning(lat_ctx *ctx, int *out);
 int  lat_set_tuning(lat_ctx *ctx, const int *in);   /* in[4] of 0/1, as lat_tuning reports */
@@ -327,6 +335,9 @@ enum lat_stage {
     LAT_ST_STENCIL,   /* FD stencil + kick                         */
     LAT_ST_REDUCE,    /* state reductions                          */
     LAT_ST_GW_SRC,    /* u_ij source S_ij = sum d_i phi d_j phi     */
+    LAT_ST_SEP_X,     /* separable Laplacian: drift + D_xx          */
+    LAT_ST_SEP_Y,     /* separable Laplacian: lap += D_yy           */
+    LAT_ST_SEP_Z,     /* separable Laplacian: D_zz + kick + reductions */
     LAT_NSTAGE
 };
 /* enable = 1 starts per-launch HIP-event timing on the context stream (and

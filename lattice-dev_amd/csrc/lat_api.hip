@@ -215,6 +215,15 @@ template <typename T>
 static int state_force(Ctx &c, bool drift, int epi)
 {
     const bool gw = c.gw && epi == EPI_KICK;
+    if constexpr (sizeof(T) == 4) {
+        if (c.sep && epi == EPI_KICK && !c.gw && c.cfg.laplacian == LAT_LAP_SPECTRAL && sep_supported(c)) {
+            int rc = ensure(c, &c.lap, sizeof(float) * c.ms.f * c.B, "hipMalloc lap");
+            if (rc) return rc;
+            int g = 0;
+            rc = launch_sep_kick<float>(c, drift ? 1 : 0, &g);
+            return rc ? rc : g;
+        }
+    }
     if (c.cfg.laplacian == LAT_LAP_STENCIL) {
         if (drift) {
             int rc = launch_drift<T>(c);
@@ -741,7 +750,7 @@ void lat_destroy(lat_ctx *h)
     if (c.stream) (void)hipStreamSynchronize(c.stream);
     void *ptrs[] = {c.fld, c.fldp, c.Fk, c.Fk2, c.io64, (c.Fk64 != c.Fk) ? c.Fk64 : nullptr, c.dsc, c.partials,
                     c.spec_part, c.spec_out, c.twx64, c.twy64, c.twz64, c.twx32, c.twy32, c.twz32,
-                    c.gwu, c.gwp, c.gwsrc, c.gwk};
+                    c.gwu, c.gwp, c.gwsrc, c.gwk, c.lap};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : c.prof.pool) (void)hipEventDestroy(e);
@@ -1764,6 +1773,31 @@ int lat_autotune(lat_ctx *h)
         return -2;
     }
     return autotune(c, 3);
+}
+
+// 0: the five-sweep 3-D pipeline; 1: the separable three-sweep Laplacian
+// for the field kicks of the mixed form (lat_sep.hip; other forms, the GW
+// kick and the reductions keep the 3-D pipeline)
+int lat_set_laplacian_form(lat_ctx *h, int form)
+{
+    CTX_OR_FAIL(h);
+    if (form != 0 && form != 1) return -1;
+    if (c.view) {
+        c.last_error = "set the form on the batch, not a member view";
+        return -2;
+    }
+    if (form == 1 && (c.prec != LAT_MIXED || !sep_supported(c))) {
+        c.last_error = "separable Laplacian: mixed form, cubic 128, 256, 512 or 1024 only";
+        return -2;
+    }
+    c.sep = form == 1;
+    return 0;
+}
+
+int lat_laplacian_form(lat_ctx *h)
+{
+    CTX_OR_FAIL(h);
+    return c.sep ? 1 : 0;
 }
 
 int lat_set_tuning(lat_ctx *h, const int *in)

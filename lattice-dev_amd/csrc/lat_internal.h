@@ -122,6 +122,10 @@ struct Ctx {
     void *gwu, *gwp;        // 6 * nvol of T
     void *gwsrc;            // 2 * nvol of T: S_ij of the component pair being kicked
     void *gwk;              // 6 * nk of cpx<T>: transforms for the GW spectra
+    // separable form of the spectral Laplacian (lat_sep.hip): lap holds the
+    // running sum D_xx + D_yy of every field between the x and z sweeps
+    bool sep = false;
+    void *lap = nullptr;    // nfld * nvol of T per member
     DevScalars *dsc;
     double *partials;       // MAX_GRID * NRED
     double *spec_part;      // spectrum block partials
@@ -162,6 +166,14 @@ int launch_col(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int axis, int mode, int n_a
 enum { EPI_STORE = 0, EPI_KICK = 1, EPI_REDUCE = 2, EPI_GWKICK = 3 };
 template <typename T>
 int launch_x_c2r(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int nf, int epi, T *out, const T *fld, T *fldp);
+
+// ---- separable spectral Laplacian (lat_sep.hip), mixed form ----
+// Kick of all fields with the three-sweep Laplacian (x: drift + D_xx; y:
+// += D_yy; z: + D_zz, kick, reductions).  *npart = reduction partials per
+// member.  c.lap must hold nfld * nvol floats per member.
+bool sep_supported(const Ctx &c);
+template <typename T>
+int launch_sep_kick(Ctx &c, int drift, int *npart);
 
 // ---- one launch per sweep over the size classes of an ensemble (lat_multi.hip) ----
 // cs[0..ncls) are batches of cubic lattices of the sizes multi_supported()

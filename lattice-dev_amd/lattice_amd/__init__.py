@@ -66,12 +66,14 @@ EXPORTS = [
     "lat_ensemble_member", "lat_ensemble_use_graph", "lat_ensemble_single_launch", "lat_ensemble_symp",
     "lat_ensemble_synchronize", "lat_ensemble_set_tuning", "lat_ensemble_tuning",
     "lat_autotune", "lat_tuning", "lat_set_tuning", "lat_copy_bandwidth", "lat_make_output",
+    "lat_set_laplacian_form", "lat_laplacian_form",
     "lat_checkpoint", "lat_restore",
     "lat_corrector_create", "lat_corrector_destroy", "lat_corrector_last_error", "lat_corrector_set_layer",
     "lat_corrector_get_layer", "lat_corrector_set_norm", "lat_corrector_get_norm", "lat_corrector_infer",
     "lat_corrector_gradients", "lat_corrector_train", "lat_corrector_step",
 ]
-STAGES = ["x_r2c", "y_fwd", "z_lap", "y_inv", "x_c2r", "scalar", "drift", "stencil", "reduce", "gw_src"]
+STAGES = ["x_r2c", "y_fwd", "z_lap", "y_inv", "x_c2r", "scalar", "drift", "stencil", "reduce", "gw_src",
+          "sep_x", "sep_y", "sep_z"]
 
 
 def lib():
@@ -161,6 +163,14 @@ class Lattice:
         self._check(lib().lat_copy_bandwidth(self._h, ctypes.c_size_t(nbytes), reps, ctypes.byref(out)),
                     "lat_copy_bandwidth")
         return out.value
+
+    def set_laplacian_form(self, separable):
+        """False: five-sweep 3-D spectral Laplacian; True: the same operator
+        as a sum of one-axis second derivatives in three sweeps (mixed form)."""
+        self._check(lib().lat_set_laplacian_form(self._h, int(bool(separable))), "lat_set_laplacian_form")
+
+    def laplacian_form(self):
+        return lib().lat_laplacian_form(self._h)
 
     def set_tuning(self, x_r2c=0, y=0, z=0, x_c2r=0):
         vals = (ctypes.c_int * 4)(x_r2c, y, z, x_c2r)
