@@ -30,8 +30,9 @@ __global__ __launch_bounds__(TPB) void k_x_r2c(T *__restrict__ f, const T *__res
                                                const cpx<T> *__restrict__ tw, int rows, int P,
                                                const DevScalars *__restrict__ dsc, int field, MemberStride ms)
 {
-    __shared__ __attribute__((aligned(16))) cpx<T> lds[r2c_lds<T, N, L, TWL>()];
-    x_r2c_body<T, N, L, DRIFT, MR, TWL>(f, pi, Fk, tw, rows, P, dsc, field, ms, hw_blk(), lds);
+    __shared__ __attribute__((aligned(16))) cpx<T> buf[r2c_buf<T, N, L>()];
+    __shared__ cpx<T> twl[tw_lds<T, N, TWL>()];
+    x_r2c_body<T, N, L, DRIFT, MR, TWL>(f, pi, Fk, tw, rows, P, dsc, field, ms, hw_blk(), buf, twl);
 }
 
 template <typename T, int N, int TC, int MODE, int MR = 16, bool PF = true, int MINW = 1, int NT = TPB,
@@ -40,9 +41,11 @@ __global__ __launch_bounds__(NT, MINW) void k_col(cpx<T> *__restrict__ Fk, const
                                              int n_other, size_t s_axis, size_t s_other, size_t s_arr,
                                              int n_arr, int ny, double kfac, size_t ms_k)
 {
-    __shared__ __attribute__((aligned(16))) cpx<T> lds[col_lds<T, N, TC, PP, TWL>()];
+    __shared__ cpx<T> twl[tw_lds<T, N, TWL>()];
+    __shared__ __attribute__((aligned(16))) cpx<T> buf[col_buf<T, N, TC>()];
+    __shared__ __attribute__((aligned(16))) cpx<T> buf2[PP ? col_buf<T, N, TC>() : 1];
     col_body<T, N, TC, MODE, MR, PF, MINW, NT, TWL, FUSE, PP>(Fk, tw, nxh, n_other, s_axis, s_other, s_arr, n_arr,
-                                                              ny, kfac, ms_k, hw_blk(), lds);
+                                                              ny, kfac, ms_k, hw_blk(), buf, buf2, twl);
 }
 
 template <typename T, int N, int L, int EPI, int MR = 4, bool PRE = true, bool TWL = false, int MINW = 1>
@@ -52,9 +55,10 @@ __global__ __launch_bounds__(TPB, MINW) void k_x_c2r(const cpx<T> *__restrict__ 
                                                const DevScalars *__restrict__ dsc, ModelDev md,
                                                double *__restrict__ partials, MemberStride ms)
 {
-    __shared__ __attribute__((aligned(16))) cpx<T> lds[c2r_lds<T, N, L, TWL>()];
+    __shared__ __attribute__((aligned(16))) cpx<T> buf[c2r_buf<T, N, L>()];
+    __shared__ cpx<T> twl[tw_lds<T, N, TWL>()];
     x_c2r_body<T, N, L, EPI, MR, PRE, TWL, MINW>(Fk, tw, rows, P, nk, nvol, nf, out, fld, fldp, dsc, md, partials,
-                                                 ms, hw_blk(), lds);
+                                                 ms, hw_blk(), buf, twl);
 }
 
 template <typename T, int N, int L, int MR>

@@ -113,8 +113,9 @@ struct R2CArgs {
 template <typename T, int N, int V, bool DRIFT>
 __device__ __forceinline__ void r2c_case(const R2CCls<T> &c, int field, const Blk &bk, cpx<T> *lds)
 {
-    x_r2c_body<T, N, r2c_L<T, N, V>(), DRIFT, r2c_maxradix<T>(), r2c_twl<T>()>(c.f, c.pi, c.Fk, c.tw, c.rows, c.P,
-                                                                              c.dsc, field, c.ms, bk, lds);
+    constexpr int L = r2c_L<T, N, V>();
+    x_r2c_body<T, N, L, DRIFT, r2c_maxradix<T>(), r2c_twl<T>()>(c.f, c.pi, c.Fk, c.tw, c.rows, c.P, c.dsc, field,
+                                                               c.ms, bk, lds, lds + r2c_buf<T, N, L>());
 }
 
 template <typename T, bool DRIFT>
@@ -159,8 +160,11 @@ __device__ __forceinline__ void col_case(const ColCls<T> &c, const Blk &bk, cpx<
 {
     using C = ColCfg<T, N, MODE == 2 ? 2 : 0>;
     static_assert(C::NT == COL_MULTI_NT, "one block size for every class of a launch");
-    col_body<T, N, col_TC<T, N, MODE, V>(), MODE, C::MR, C::PF, 1, C::NT, true, C::FUSE, C::PP>(
-        c.Fk, c.tw, c.nxh, c.n_other, c.s_axis, c.s_other, c.s_arr, c.n_arr, c.ny, c.kfac, c.ms_k, bk, lds);
+    constexpr int TC = col_TC<T, N, MODE, V>();
+    constexpr int B = col_buf<T, N, TC>();
+    col_body<T, N, TC, MODE, C::MR, C::PF, 1, C::NT, true, C::FUSE, C::PP>(
+        c.Fk, c.tw, c.nxh, c.n_other, c.s_axis, c.s_other, c.s_arr, c.n_arr, c.ny, c.kfac, c.ms_k, bk, lds, lds + B,
+        lds + (C::PP ? 2 * B : B));
 }
 
 template <typename T, int MODE>
@@ -207,8 +211,9 @@ struct C2RArgs {
 template <typename T, int N, int V, int EPI>
 __device__ __forceinline__ void c2r_case(const C2RCls<T> &c, int nf, const Blk &bk, cpx<T> *lds)
 {
-    x_c2r_body<T, N, c2r_L<N, V>(), EPI, 4, true, true, 1>(c.Fk, c.tw, c.rows, c.P, c.nk, c.nvol, nf, nullptr, c.fld,
-                                                          c.fldp, c.dsc, c.md, c.partials, c.ms, bk, lds);
+    constexpr int L = c2r_L<N, V>();
+    x_c2r_body<T, N, L, EPI, 4, true, true, 1>(c.Fk, c.tw, c.rows, c.P, c.nk, c.nvol, nf, nullptr, c.fld, c.fldp,
+                                               c.dsc, c.md, c.partials, c.ms, bk, lds, lds + c2r_buf<T, N, L>());
 }
 
 template <typename T, int EPI>

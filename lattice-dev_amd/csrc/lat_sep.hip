@@ -29,8 +29,9 @@ __global__ __launch_bounds__(TPB) void k_sep_x(T *__restrict__ f, const T *__res
     constexpr int M = N / 2;
     constexpr int LS = M + 2; // even pitch: rows start 16-byte aligned
     constexpr bool TWL = true;
-    __shared__ __attribute__((aligned(16))) cpx<T> buf[L * LS + N];
-    LAT_TWIDDLES_TO_LDS(N, TPB, buf + L * LS)
+    __shared__ __attribute__((aligned(16))) cpx<T> buf[L * LS];
+    __shared__ cpx<T> twl[N];
+    LAT_TWIDDLES_TO_LDS(N, TPB, twl)
     f += blockIdx.y * ms.f;
     if (pi) pi += blockIdx.y * ms.f;
     lap += blockIdx.y * ms.f;
@@ -176,14 +177,14 @@ __global__ __launch_bounds__(NT) void k_sep_col(T *__restrict__ fld, T *__restri
                                                double *__restrict__ partials, MemberStride ms)
 {
     constexpr bool TWL = true;
-    __shared__ __attribute__((aligned(16))) cpx<T> lds[2 * TC * N + N];
-    LAT_TWIDDLES_TO_LDS(N, NT, lds + 2 * TC * N)
+    __shared__ cpx<T> twl[N];
+    __shared__ cpx<T> buf[TC * N], buf2[TC * N];
+    LAT_TWIDDLES_TO_LDS(N, NT, twl)
     cpx<T> *F = reinterpret_cast<cpx<T> *>(fld + blockIdx.y * ms.f);
     cpx<T> *Lc = reinterpret_cast<cpx<T> *>(lap + blockIdx.y * ms.f);
     cpx<T> *Pc = reinterpret_cast<cpx<T> *>(fldp + blockIdx.y * ms.f);
     dsc += blockIdx.y;
     const Blk bk = hw_blk();
-    cpx<T> *buf = lds, *buf2 = lds + TC * N;
     constexpr int NP = num_passes(N, MR);
     static_assert(NP >= 2, "fused passes need two passes");
     constexpr int R0 = radix_at(N, 0, MR);

@@ -19,6 +19,8 @@ struct Blk {
     int x, gx, y;
 };
 __device__ __forceinline__ Blk hw_blk() { return Blk{(int)blockIdx.x, (int)gridDim.x, (int)blockIdx.y}; }
+// member index of a body: non-negative, below the 65536 grid limit
+__device__ __forceinline__ void assume_blk(const Blk &bk) { __builtin_assume(bk.y >= 0 && bk.y < 65536); }
 
 // ---------------------------------------------------------------- tiling
 // Grid-stride loop over T tiles with an XCD-aware assignment: hardware hands
@@ -28,6 +30,11 @@ __device__ __forceinline__ Blk hw_blk() { return Blk{(int)blockIdx.x, (int)gridD
 __device__ __forceinline__ void tile_range(int T, const Blk &bk, int &start, int &end, int &step)
 {
     const int G = bk.gx, b = bk.x;
+    // block coordinates are non-negative (as blockIdx / gridDim are): lets
+    // the compiler keep unsigned index arithmetic downstream
+    __builtin_assume(G > 0);
+    __builtin_assume(b >= 0 && b < G);
+    __builtin_assume(T >= 0);
     if ((G & 7) == 0 && T >= G) {
         const int xcd = b & 7, slot = b >> 3, G8 = G >> 3;
         const int C = (T + 7) >> 3;
@@ -87,25 +94,34 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NCH], double *out
         twp = twl_;                                                               \
     }
 
-// LDS of each body in complex elements: the transform buffer(s), then the
-// twiddle table when TWL.
+// LDS of each body in complex elements: the transform buffer(s) and the
+// twiddle table (when TWL), passed to the bodies as separate restrict
+// pointers so that buffer stores and table loads are known not to alias.
+template <typename T, int N, int L>
+constexpr int r2c_buf() { return L * (N / 2 + 1); }
+template <typename T, int N, int TC>
+constexpr int col_buf() { return TC * N; }
+template <typename T, int N, int L>
+constexpr int c2r_buf() { return 2 * L * (N / 2 + 2); }
+template <typename T, int N, bool TWL>
+constexpr int tw_lds() { return TWL ? N : 1; }
 template <typename T, int N, int L, bool TWL>
-constexpr int r2c_lds() { return L * (N / 2 + 1) + (TWL ? N : 0); }
+constexpr int r2c_lds() { return r2c_buf<T, N, L>() + (TWL ? N : 0); }
 template <typename T, int N, int TC, bool PP, bool TWL>
-constexpr int col_lds() { return TC * N * (PP ? 2 : 1) + (TWL ? N : 0); }
+constexpr int col_lds() { return col_buf<T, N, TC>() * (PP ? 2 : 1) + (TWL ? N : 0); }
 template <typename T, int N, int L, bool TWL>
-constexpr int c2r_lds() { return 2 * L * (N / 2 + 2) + (TWL ? N : 0); }
+constexpr int c2r_lds() { return c2r_buf<T, N, L>() + (TWL ? N : 0); }
 
 template <typename T, int N, int L, bool DRIFT, int MR = 4, bool TWL = false>
 __device__ __forceinline__ void x_r2c_body(T *__restrict__ f, const T *__restrict__ pi, cpx<T> *__restrict__ Fk,
                                            const cpx<T> *__restrict__ tw, int rows, int P,
                                            const DevScalars *__restrict__ dsc, int field, MemberStride ms,
-                                           const Blk bk, cpx<T> *lds)
+                                           const Blk bk, cpx<T> *__restrict__ buf, cpx<T> *__restrict__ twl)
 {
     constexpr int M = N / 2;
     constexpr int LS = M + 1;
-    cpx<T> *buf = lds;
-    LAT_TWIDDLES_TO_LDS(N, TPB, lds + L * LS)
+    LAT_TWIDDLES_TO_LDS(N, TPB, twl)
+    assume_blk(bk);
     f += bk.y * ms.f;
     if (pi) pi += bk.y * ms.f;
     Fk += bk.y * ms.k;
@@ -205,17 +221,17 @@ template <typename T, int N, int TC, int MODE, int MR = 16, bool PF = true, int 
           bool TWL = false, bool FUSE = false, bool PP = false>
 __device__ __forceinline__ void col_body(cpx<T> *__restrict__ Fk, const cpx<T> *__restrict__ tw, int nxh, int n_other,
                                          size_t s_axis, size_t s_other, size_t s_arr, int n_arr, int ny, double kfac,
-                                         size_t ms_k, const Blk bk, cpx<T> *lds)
+                                         size_t ms_k, const Blk bk, cpx<T> *__restrict__ buf,
+                                         cpx<T> *__restrict__ buf2, cpx<T> *__restrict__ twl)
 {
-    LAT_TWIDDLES_TO_LDS(N, NT, lds + TC * N * (PP ? 2 : 1))
+    LAT_TWIDDLES_TO_LDS(N, NT, twl)
+    assume_blk(bk);
     Fk += bk.y * ms_k;
     // Software pipeline per tile of TC adjacent columns:
     //   pass 0 straight from HBM into registers (radix R0, no twiddles),
     //   results to LDS, then the next tile's pass-0 loads are issued into the
     //   same (now dead) registers so they are in flight during the remaining
     //   LDS passes; the last pass writes straight to HBM.
-    cpx<T> *buf = lds;
-    cpx<T> *buf2 = lds + TC * N; // PP only
     constexpr int NP = num_passes(N, MR);
     constexpr int SIGN0 = (MODE == 1) ? +1 : -1;
     constexpr int R0 = radix_at(N, 0, MR);
@@ -342,13 +358,13 @@ __device__ __forceinline__ void x_c2r_body(const cpx<T> *__restrict__ Fk, const 
                                            const T *__restrict__ fld, T *__restrict__ fldp,
                                            const DevScalars *__restrict__ dsc, ModelDev md,
                                            double *__restrict__ partials, MemberStride ms, const Blk bk,
-                                           cpx<T> *lds)
+                                           cpx<T> *__restrict__ buf, cpx<T> *__restrict__ twl)
 {
     constexpr int M = N / 2;
-    constexpr int LS = M + 2; // even pitch: every row starts 16-byte aligned (lds 16-byte aligned)
+    constexpr int LS = M + 2; // even pitch: every row starts 16-byte aligned (buf 16-byte aligned)
     constexpr int MAXF = 2;
-    cpx<T> *buf = lds;
-    LAT_TWIDDLES_TO_LDS(N, TPB, lds + MAXF * L * LS)
+    LAT_TWIDDLES_TO_LDS(N, TPB, twl)
+    assume_blk(bk);
     Fk += bk.y * ms.k;
     if (out) out += bk.y * ms.f;
     if (fld) fld += bk.y * ms.f;
