@@ -165,10 +165,11 @@ def corrector_leg(la, args):
     pairs the plain integrator generates, then run at --corr-n with coarse
     steps of --corr-cf times dt (leapfrog + predicted residual).  Reported against the
     plain Yoshida-4 run over the same time: the field error of leapfrog
-    alone, of the always-accepted corrected run, and of the guarded run, in
+    alone, of the always-accepted corrected run, and of two guarded runs, in
     which a step whose Friedmann-residual change exceeds --corr-tol times
-    the plain run's average change per coarse interval is replaced by 8
-    Yoshida-4 steps; cost in force evaluations (coarse step 1, Yoshida-4 3).
+    the plain run's average change per coarse interval, or the fixed bound
+    --corr-bound, is replaced by --corr-cf Yoshida-4 steps; cost in force
+    evaluations (coarse step 1, Yoshida-4 3).
     Errors are relative to the fluctuation norm of the plain run's fields
     and momenta; non-finite entries mean the coarse leapfrog step is beyond
     its stability limit at that coupling."""
@@ -228,13 +229,22 @@ def corrector_leg(la, args):
                 lat.set_state(*start)
                 st = corr.step(lat, dt, cf, nsteps=args.corr_steps, threshold=thr)
                 guarded = lat.get_state()
+                # the same guard at a fixed bound on the relative Friedmann
+                # residual (change per coarse step)
+                lat.set_state(*start)
+                st_b = corr.step(lat, dt, cf, nsteps=args.corr_steps, threshold=args.corr_bound)
+                bounded = lat.get_state()
+                d_bound = lat.energies()["resid"] - e0
             plain = 3.0 * cf * args.corr_steps
             run["eval"] = {"n": n, "coarse_steps": args.corr_steps, "plain_resid_change": d_plain,
                            "leapfrog_resid_change": d_leap, "corrected_resid_change": d_always,
                            "threshold": thr, "err_leapfrog": fluct_err(leap, ref),
                            "err_corrected": fluct_err(always, ref), "err_guarded": fluct_err(guarded, ref),
                            "fallbacks": st["fallbacks"], "force_evals": st["force_evals"],
-                           "plain_force_evals": plain, "saving": plain / st["force_evals"]}
+                           "plain_force_evals": plain, "saving": plain / st["force_evals"],
+                           "bound": args.corr_bound, "bound_fallbacks": st_b["fallbacks"],
+                           "bound_force_evals": st_b["force_evals"], "bound_saving": plain / st_b["force_evals"],
+                           "bound_resid_change": d_bound, "err_bound": fluct_err(bounded, ref)}
         out["runs"].append(run)
     print(json.dumps(out))
 
@@ -329,6 +339,9 @@ def main():
     ap.add_argument("--ensemble", action="store_true", help="only run the batched-ensemble leg")
     ap.add_argument("--ens-steps", type=int, default=20)
     ap.add_argument("--ens-form", default="mixed", choices=["mixed", "f64"])
+    ap.add_argument("--corr-bound", type=float, default=1e-6,
+                    help="second guarded run: fall back when a coarse step moves the relative Friedmann residual "
+                         "by more than this")
     ap.add_argument("--corr-tol", type=float, default=10.0,
                     help="accept a corrected step whose constraint change is within this multiple of the plain "
                          "run's average change per coarse interval")
