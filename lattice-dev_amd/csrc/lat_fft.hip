@@ -109,10 +109,10 @@ static void col_go(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int n_other, size_t s_a
                    double kfac)
 {
     using C = ColCfg<T, N, MODE == 2 ? 2 : 0>;
+    constexpr int NT = col_nt<T, N, MODE, TC>();
     const int g = grid_for(((c.nxh + TC - 1) / TC) * n_other * n_arrays, C::GRID);
-    hipLaunchKernelGGL((k_col<T, N, TC, MODE, C::MR, C::PF, 1, C::NT, true, C::FUSE, C::PP>), dim3(g, c.B),
-                       dim3(C::NT), 0, c.stream, Fk, tw, c.nxh, n_other, s_axis, s_other, c.nk, n_arrays, c.ny, kfac,
-                       c.ms.k);
+    hipLaunchKernelGGL((k_col<T, N, TC, MODE, C::MR, C::PF, 1, NT, true, C::FUSE, C::PP>), dim3(g, c.B), dim3(NT),
+                       0, c.stream, Fk, tw, c.nxh, n_other, s_axis, s_other, c.nk, n_arrays, c.ny, kfac, c.ms.k);
 }
 
 template <typename T, int N, int MODE>

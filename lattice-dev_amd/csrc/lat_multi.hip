@@ -158,11 +158,13 @@ constexpr int COL_MULTI_NT = 512;
 template <typename T, int N, int V, int MODE>
 __device__ __forceinline__ void col_case(const ColCls<T> &c, const Blk &bk, cpx<T> *lds)
 {
+    // one block size for every class of the launch: the default 512 threads
+    // (the bodies loop or idle lanes where a shape has more or fewer
+    // first-pass butterflies)
     using C = ColCfg<T, N, MODE == 2 ? 2 : 0>;
-    static_assert(C::NT == COL_MULTI_NT, "one block size for every class of a launch");
     constexpr int TC = col_TC<T, N, MODE, V>();
     constexpr int B = col_buf<T, N, TC>();
-    col_body<T, N, TC, MODE, C::MR, C::PF, 1, C::NT, true, C::FUSE, C::PP>(
+    col_body<T, N, TC, MODE, C::MR, C::PF, 1, COL_MULTI_NT, true, C::FUSE, C::PP>(
         c.Fk, c.tw, c.nxh, c.n_other, c.s_axis, c.s_other, c.s_arr, c.n_arr, c.ny, c.kfac, c.ms_k, bk, lds, lds + B,
         lds + (C::PP ? 2 * B : B));
 }

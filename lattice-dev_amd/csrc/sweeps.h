@@ -618,13 +618,33 @@ struct ColCfg {
 #define LAT_FOR_SIZES(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(48) X(96) X(192)
 
 // Sizes up to 256 carry a second launch shape per sweep that the autotuner
-// (lat_autotune) may pick: half the rows per workgroup for the x sweeps, half
-// the columns per tile for the y/z sweeps.
+// (lat_autotune) may pick: half the rows per workgroup for the x sweeps; for
+// the y/z sweeps the largest power-of-two column count up to half the
+// default (whole 8-byte-multiple row segments: 192^3 takes 16, not 21), run
+// by exactly the threads of its first pass, rounded to whole waves (the
+// default shape keeps its 512 threads).  Measured on MI355X: 128^3 y pass
+// 0.418 -> 0.357 ms and fused z pass 0.546 -> 0.402 ms (32 columns, 256
+// threads), 192^3 y pass 0.462 -> 0.398 ms (16 columns, 192 threads).
 template <int N>
 constexpr bool tunable() { return N <= 256; }
+constexpr int pow2_floor(int v) { return v >= 2 ? 2 * pow2_floor(v / 2) : 1; }
 template <typename T, int N>
-constexpr int col_tc(int mode, int v) { return v ? clampi((mode == 2 ? ColCfg<T, N, 2>::TC : ColCfg<T, N, 0>::TC) / 2, 4, 64)
-                                              : (mode == 2 ? ColCfg<T, N, 2>::TC : ColCfg<T, N, 0>::TC); }
+constexpr int col_tc(int mode, int v)
+{
+    return v ? clampi(pow2_floor((mode == 2 ? ColCfg<T, N, 2>::TC : ColCfg<T, N, 0>::TC) / 2), 4, 64)
+             : (mode == 2 ? ColCfg<T, N, 2>::TC : ColCfg<T, N, 0>::TC);
+}
+template <typename T, int N, int MODE, int TC>
+constexpr int col_nt()
+{
+    using C = ColCfg<T, N, MODE == 2 ? 2 : 0>;
+    if constexpr (!tunable<N>() || TC == col_tc<T, N>(MODE == 2 ? 2 : 0, 0)) {
+        return C::NT;
+    } else {
+        constexpr int t = TC * (N / radix_at(N, 0, C::MR));
+        return clampi((t + 63) / 64 * 64, 64, 512);
+    }
+}
 
 // Twiddles in LDS (TWL) in every sweep: the 1024^3 sweeps measured a gain
 // (or no loss) in each.
