@@ -313,6 +313,57 @@ def ensemble_leg(la, args):
     print(json.dumps(out))
 
 
+def gw_leg(la, args):
+    """The headline physics with the six metric perturbations u_ij (spectral,
+    sourced by the field gradients, TT projection in the GW spectrum),
+    u = pi_u = 0 at the start, both forms from identical IC: site-steps/s of
+    each form and the mixed form's errors against FP64 after the run (field
+    spectra and GW spectra over bins >= 1e-6 of the peak, energy)."""
+    n = args.gw_n
+    dt = (10.0 / n) / 40.0
+    res = {}
+    for form, prec in (("f64", la.F64), ("mixed", la.MIXED)):
+        with la.Lattice(n, nfld=2, length=10.0, precision=prec, laplacian=la.LAP_SPECTRAL, time_var=la.COSMIC,
+                        model=la.MODEL_TWOFIELD, model_params=(1.0, G2), gw=True, gw_mpl=1.0) as lat:
+            init_state(lat)
+            lat.symp4(dt, args.warmup)
+            lat.synchronize()
+            t0 = time.perf_counter()
+            lat.symp4(dt, args.gw_steps)
+            lat.synchronize()
+            el = time.perf_counter() - t0
+            res[form] = {"site_steps_per_s": float(n) ** 3 * args.gw_steps / el,
+                         "ms_per_step": 1e3 * el / args.gw_steps,
+                         "spectra": [lat.output_spectra(0), lat.output_spectra(1)],
+                         "gw": lat.gw_spectrum(), "e": lat.energies()}
+        print(f"[bench] gw {form} {n}^3: {res[form]['site_steps_per_s']:.4e} site-steps/s", file=sys.stderr,
+              flush=True)
+
+    def maxrel(a, b, floor=1e-6):
+        a, b = np.asarray(a)[1:], np.asarray(b)[1:]
+        m = np.abs(b) >= floor * np.abs(b).max()
+        return float(np.max(np.abs(a[m] - b[m]) / np.abs(b[m])))
+
+    d, m = res["f64"], res["mixed"]
+    w = 4
+    out = {"metric": f"site-steps/s at {n}^3 with the six GW components, mixed vs plain FP64",
+           "steps": args.gw_steps, "warmup": args.warmup,
+           "mixed": {k: m[k] for k in ("site_steps_per_s", "ms_per_step")},
+           "f64": {k: d[k] for k in ("site_steps_per_s", "ms_per_step")},
+           "mixed_over_fp64": m["site_steps_per_s"] / d["site_steps_per_s"],
+           # model bytes per site-step with GW (two fields): 3 [2 (7w + 8wc) + 6 (6w + 8wc) + 2w]
+           "hbm_frac_mixed_vs_8TBs": m["site_steps_per_s"] * 3 * (2 * (7 * w + 8 * w) + 6 * (6 * w + 8 * w)
+                                                                 + 2 * w) / 8e12,
+           "mixed_vs_f64": {"field_spectra": max(maxrel(m["spectra"][l][q], d["spectra"][l][q])
+                                                 for l in (0, 1) for q in (0, 1)),
+                            "gw_spectrum_pi": maxrel(m["gw"][0], d["gw"][0]),
+                            "gw_spectrum_u": maxrel(m["gw"][1], d["gw"][1]),
+                            "energy": abs(m["e"]["rho"] - d["e"]["rho"]) / abs(d["e"]["rho"])},
+           "after_steps": args.warmup + args.gw_steps}
+    out["value"] = out["mixed"]["site_steps_per_s"]
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -337,6 +388,9 @@ def main():
     ap.add_argument("--corr-lr", type=float, default=1e-3)
     ap.add_argument("--corr-cf", type=int, default=4, help="coarse step = this many fine steps")
     ap.add_argument("--ensemble", action="store_true", help="only run the batched-ensemble leg")
+    ap.add_argument("--gw", action="store_true", help="only run the leg with the six GW components")
+    ap.add_argument("--gw-n", type=int, default=1024)
+    ap.add_argument("--gw-steps", type=int, default=5)
     ap.add_argument("--ens-steps", type=int, default=20)
     ap.add_argument("--ens-form", default="mixed", choices=["mixed", "f64"])
     ap.add_argument("--corr-bound", type=float, default=1e-6,
@@ -356,6 +410,11 @@ def main():
         import lattice_amd as la
 
         ensemble_leg(la, args)
+        return
+    if args.gw:
+        import lattice_amd as la
+
+        gw_leg(la, args)
         return
     if args.copy_calibration:
         # the library's plain copy kernel (k_copy16) over 2 GiB, 1 + 3 launches:
