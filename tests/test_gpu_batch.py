@@ -189,7 +189,7 @@ def test_ensemble_single_launch_matches_per_class_launches(gpu, precision):
         assert ens.single_launch(True) is False
 
 
-@pytest.mark.parametrize("n", [32, 64, 192])
+@pytest.mark.parametrize("n", [32, 64, 128, 192])
 @pytest.mark.parametrize("precision", ["f64", "mixed"])
 def test_tuned_launch_shapes_keep_results(gpu, n, precision):
     """The alternative launch shapes (fewer rows per x workgroup, fewer
