@@ -38,8 +38,9 @@ MPL = 2.0e5
 G2 = 2.5e5
 
 
-def make_lattice(la, n, precision):
-    return la.Lattice(n, nfld=2, length=10.0, precision=precision, laplacian=la.LAP_SPECTRAL,
+def make_lattice(la, n, precision, laplacian=None):
+    lap = la.LAP_SPECTRAL if laplacian is None else laplacian
+    return la.Lattice(n, nfld=2, length=10.0, precision=precision, laplacian=lap, stencil=la.STENCIL_7PT,
                       time_var=la.COSMIC, model=la.MODEL_TWOFIELD, model_params=(1.0, G2))
 
 
@@ -73,8 +74,8 @@ def algorithmic_bytes(stage, n, nf, w):
     }.get(stage)
 
 
-def run_form(la, n, precision, steps, warmup, out_every, dt, sep=False):
-    lat = make_lattice(la, n, precision)
+def run_form(la, n, precision, steps, warmup, out_every, dt, sep=False, laplacian=None):
+    lat = make_lattice(la, n, precision, laplacian)
     if sep:
         lat.set_laplacian_form(True)
     t0 = time.time()
@@ -313,6 +314,31 @@ def ensemble_leg(la, args):
     print(json.dumps(out))
 
 
+def config1_leg(la, args):
+    """The 256^3 drop-in configuration: two-field preheating (g^2/m^2 =
+    2.5e5, cosmic FRW, L = 10/m, dt = dx/40, Yoshida-4, spectra of both
+    fields every 20 steps) with the spectral and the 7-point FD Laplacian,
+    FP64 and mixed.  site-steps/s, and the HBM rate of the model bytes per
+    site-step (spectral 3 nf (7w + 8w_c), FD 18 nf w) against 8 TB/s."""
+    n = args.c1_n
+    dt = (10.0 / n) / 40.0
+    out = {"metric": f"site-steps/s at {n}^3, spectral and 7-point FD Laplacian", "steps": args.c1_steps,
+           "warmup": args.warmup, "spectra_every": 20, "runs": []}
+    for lap_name, lap in (("spectral", la.LAP_SPECTRAL), ("fd7", la.LAP_STENCIL)):
+        for form, prec in (("f64", la.F64), ("mixed", la.MIXED)):
+            r = run_form(la, n, prec, args.c1_steps, args.warmup, 20, dt, laplacian=lap)
+            w = 8 if form == "f64" else 4
+            wc = w * (1.0 + 2.0 / n)
+            model = 3 * 2 * (7 * w + 8 * wc) if lap_name == "spectral" else 18 * 2 * w
+            v = float(n) ** 3 * args.c1_steps / r["elapsed"]
+            out["runs"].append({"laplacian": lap_name, "form": form, "site_steps_per_s": v,
+                                "ms_per_step": 1e3 * r["elapsed"] / args.c1_steps,
+                                "model_bytes_per_site_step": model, "hbm_frac_vs_8TBs": v * model / 8e12,
+                                "stages_ms": {k: round(x[0], 3) for k, x in r["prof"].items() if x[1]}})
+            print(f"[bench] {n}^3 {lap_name} {form}: {v:.4e} site-steps/s", file=sys.stderr, flush=True)
+    print(json.dumps(out))
+
+
 def gw_leg(la, args):
     """The headline physics with the six metric perturbations u_ij (spectral,
     sourced by the field gradients, TT projection in the GW spectrum),
@@ -389,6 +415,10 @@ def main():
     ap.add_argument("--corr-cf", type=int, default=4, help="coarse step = this many fine steps")
     ap.add_argument("--ensemble", action="store_true", help="only run the batched-ensemble leg")
     ap.add_argument("--gw", action="store_true", help="only run the leg with the six GW components")
+    ap.add_argument("--config1", action="store_true",
+                    help="only run the 256^3 leg with the spectral and the 7-point FD Laplacian")
+    ap.add_argument("--c1-n", type=int, default=256)
+    ap.add_argument("--c1-steps", type=int, default=200)
     ap.add_argument("--gw-n", type=int, default=1024)
     ap.add_argument("--gw-steps", type=int, default=5)
     ap.add_argument("--ens-steps", type=int, default=20)
@@ -415,6 +445,11 @@ def main():
         import lattice_amd as la
 
         gw_leg(la, args)
+        return
+    if args.config1:
+        import lattice_amd as la
+
+        config1_leg(la, args)
         return
     if args.copy_calibration:
         # the library's plain copy kernel (k_copy16) over 2 GiB, 1 + 3 launches:
