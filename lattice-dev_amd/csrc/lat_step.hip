@@ -313,6 +313,100 @@ __global__ __launch_bounds__(TPB) void k_stencil(const T *__restrict__ fld, T *_
     if constexpr (EPI != EPI_STORE) block_reduce_store2<NRED>(acc, partials + (size_t)blockIdx.x * NRED);
 }
 
+// 7-point stencil (c2 = c3 = 0) + kick / reduce, marching along z: a
+// 64 x 4 block owns an (x, y) tile and a chunk of ST7_KC planes; each thread
+// keeps its column's values at k-1, k, k+1 in registers, so per site it loads
+// the new k+1 value and the four in-plane neighbours (which the tile's other
+// lanes load too) instead of seven scattered values.  The neighbour sum is
+// formed in stencil_sum's order, so the Laplacian is the same as there.
+constexpr int ST7_TX = 64, ST7_TY = 4, ST7_KC = 32;
+
+template <typename T, int EPI>
+__global__ __launch_bounds__(TPB) void k_stencil7(const T *__restrict__ fld, T *__restrict__ fldp, int nx, int ny,
+                                                  int nz, int nf, StencilC sc, ModelDev md,
+                                                  const DevScalars *__restrict__ dsc, double *__restrict__ partials,
+                                                  size_t ms)
+{
+    static_assert(ST7_TX * ST7_TY == TPB, "one site column per thread");
+    fld += blockIdx.y * ms;
+    fldp += blockIdx.y * ms;
+    dsc += blockIdx.y;
+    partials += (size_t)blockIdx.y * MAX_GRID * NRED;
+    md.p0 = dsc->mp[0];
+    md.p1 = dsc->mp[1];
+    const size_t nvol = (size_t)nx * ny * nz, plane = (size_t)nx * ny;
+    const T c0 = T(sc.c0), c1 = T(sc.c1), el = T(sc.elap);
+    T hA = 0, hB = 0;
+    if constexpr (EPI == EPI_KICK) {
+        hA = T(dsc->hA);
+        hB = T(dsc->hB);
+    }
+    float pb32[2] = {0, 0};
+    double pb64[2] = {0, 0};
+    T mu[2] = {0, 0};
+    if constexpr (sizeof(T) == 4) {
+        for (int l = 0; l < 2; ++l) {
+            pb64[l] = dsc->mean_phi[l];
+            pb32[l] = float(pb64[l]);
+            if constexpr (EPI == EPI_KICK) mu[l] = T(dsc->mu[l]);
+        }
+    }
+    double acc[NRED] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int ntx = (nx + ST7_TX - 1) / ST7_TX, nty = (ny + ST7_TY - 1) / ST7_TY, nkc = (nz + ST7_KC - 1) / ST7_KC;
+    const int items = ntx * nty * nkc;
+    const int tx = threadIdx.x % ST7_TX, ty = threadIdx.x / ST7_TX;
+    for (int it = blockIdx.x; it < items; it += gridDim.x) {
+        const int xt = it % ntx, rest = it / ntx, yt = rest % nty, kc = rest / nty;
+        const int i = xt * ST7_TX + tx, j = yt * ST7_TY + ty;
+        if (i >= nx || j >= ny) continue;
+        const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
+        const int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
+        const int k0 = kc * ST7_KC, k1 = min(k0 + ST7_KC, nz);
+        const size_t col = (size_t)i + (size_t)nx * j;
+        const size_t oxm = (size_t)im + (size_t)nx * j, oxp = (size_t)ip + (size_t)nx * j;
+        const size_t oym = (size_t)i + (size_t)nx * jm, oyp = (size_t)i + (size_t)nx * jp;
+        T prev[2] = {0, 0}, cur[2] = {0, 0};
+        for (int l = 0; l < nf; ++l) {
+            const T *f = fld + (size_t)l * nvol;
+            prev[l] = f[col + plane * (size_t)(k0 == 0 ? nz - 1 : k0 - 1)];
+            cur[l] = f[col + plane * (size_t)k0];
+        }
+        for (int k = k0; k < k1; ++k) {
+            const size_t pk = plane * (size_t)k, pn = plane * (size_t)(k == nz - 1 ? 0 : k + 1);
+            T lap[2] = {0, 0}, phi[2] = {0, 0}, next[2] = {0, 0};
+            for (int l = 0; l < nf; ++l) {
+                const T *f = fld + (size_t)l * nvol;
+                next[l] = f[col + pn];
+                const T r1 = f[oxm + pk] + f[oxp + pk] + f[oym + pk] + f[oyp + pk] + prev[l] + next[l];
+                lap[l] = (c0 * cur[l] + c1 * r1) * el;
+                phi[l] = cur[l];
+            }
+            const size_t s = col + pk;
+            T dv[2] = {0, 0};
+            if constexpr (EPI == EPI_KICK) site_dv(md, pb32, phi, dv);
+            acc[RC_PE] += site_v(md, pb64, phi);
+            for (int l = 0; l < nf; ++l) {
+                T p = fldp[(size_t)l * nvol + s];
+                if constexpr (EPI == EPI_KICK) {
+                    p = p + (hA * lap[l] - hB * dv[l]);
+                    if constexpr (sizeof(T) == 4) p = p - mu[l];
+                    fldp[(size_t)l * nvol + s] = p;
+                }
+                acc[RC_GE] -= double(phi[l]) * double(lap[l]);
+                acc[RC_KE] += double(p) * double(p);
+                acc[RC_S1 + l] += double(p);
+            }
+            acc[RC_M1] += double(phi[0]);
+            acc[RC_M2] += double(phi[1]);
+            for (int l = 0; l < nf; ++l) {
+                prev[l] = cur[l];
+                cur[l] = next[l];
+            }
+        }
+    }
+    block_reduce_store2<NRED>(acc, partials + (size_t)blockIdx.x * NRED);
+}
+
 template <typename T>
 int launch_stencil(Ctx &c, int epi)
 {
@@ -322,6 +416,19 @@ int launch_stencil(Ctx &c, int epi)
     size_t blocks = (c.nvol + TPB - 1) / TPB;
     int g = grid_for(blocks > (size_t)MAX_GRID ? MAX_GRID : (int)blocks);
     T *fld = (T *)c.fld, *fldp = (T *)c.fldp;
+    if (cs[2] == 0.0 && cs[3] == 0.0 && (epi == EPI_KICK || epi == EPI_REDUCE)) {
+        const int items = ((c.nx + ST7_TX - 1) / ST7_TX) * ((c.ny + ST7_TY - 1) / ST7_TY) *
+                          ((c.nz + ST7_KC - 1) / ST7_KC);
+        g = grid_for(items > MAX_GRID ? MAX_GRID : items);
+        if (epi == EPI_KICK)
+            hipLaunchKernelGGL((k_stencil7<T, EPI_KICK>), dim3(g, c.B), dim3(TPB), 0, c.stream, fld, fldp, c.nx,
+                               c.ny, c.nz, c.nfld, sc, md, c.dsc, c.partials, c.ms.f);
+        else
+            hipLaunchKernelGGL((k_stencil7<T, EPI_REDUCE>), dim3(g, c.B), dim3(TPB), 0, c.stream, fld, fldp, c.nx,
+                               c.ny, c.nz, c.nfld, sc, md, c.dsc, c.partials, c.ms.f);
+        LAT_HIP(&c, hipGetLastError());
+        return g;
+    }
     if (epi == EPI_KICK)
         hipLaunchKernelGGL((k_stencil<T, EPI_KICK>), dim3(g, c.B), dim3(TPB), 0, c.stream, fld, fldp, (T *)nullptr,
                            c.nx, c.ny, c.nz, c.nfld, sc, md, c.dsc, c.partials, c.ms.f);
