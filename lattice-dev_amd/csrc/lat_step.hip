@@ -549,12 +549,22 @@ __global__ __launch_bounds__(SCALAR_TPB) void k_scalar(DevScalars *__restrict__ 
     const int ch = threadIdx.x / LANES, lane = threadIdx.x % LANES;
     if (npart > 0) {
         double s = 0.0, comp = 0.0;
-        for (int g = lane; g < npart; g += LANES) {
-            const double x = partials[(size_t)g * NRED + ch];
+        auto add = [&](double x) {
             const double t = s + x;
             comp += (fabs(s) >= fabs(x)) ? (s - t) + x : (x - t) + s;
             s = t;
+        };
+        // the loads of 8 partials are issued before their (ordered) sums, so
+        // the chain waits on memory once per 8 terms, not per term
+        int g = lane;
+        for (; g + 7 * LANES < npart; g += 8 * LANES) {
+            double xs[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) xs[u] = partials[(size_t)(g + u * LANES) * NRED + ch];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) add(xs[u]);
         }
+        for (; g < npart; g += LANES) add(partials[(size_t)g * NRED + ch]);
         sh[ch][lane] = s + comp;
         __syncthreads();
         if (lane == 0) {
