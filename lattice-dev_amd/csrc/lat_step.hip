@@ -321,13 +321,13 @@ __global__ __launch_bounds__(TPB) void k_stencil(const T *__restrict__ fld, T *_
 // formed in stencil_sum's order, so the Laplacian is the same as there.
 constexpr int ST7_TX = 64, ST7_TY = 4, ST7_KC = 32;
 
-template <typename T, int EPI>
+template <typename T, int EPI, int TXV = ST7_TX, int TYV = ST7_TY, int KCV = ST7_KC>
 __global__ __launch_bounds__(TPB) void k_stencil7(const T *__restrict__ fld, T *__restrict__ fldp, int nx, int ny,
                                                   int nz, int nf, StencilC sc, ModelDev md,
                                                   const DevScalars *__restrict__ dsc, double *__restrict__ partials,
                                                   size_t ms)
 {
-    static_assert(ST7_TX * ST7_TY == TPB, "one site column per thread");
+    static_assert(TXV * TYV == TPB, "one site column per thread");
     fld += blockIdx.y * ms;
     fldp += blockIdx.y * ms;
     dsc += blockIdx.y;
@@ -352,32 +352,53 @@ __global__ __launch_bounds__(TPB) void k_stencil7(const T *__restrict__ fld, T *
         }
     }
     double acc[NRED] = {0, 0, 0, 0, 0, 0, 0, 0};
-    const int ntx = (nx + ST7_TX - 1) / ST7_TX, nty = (ny + ST7_TY - 1) / ST7_TY, nkc = (nz + ST7_KC - 1) / ST7_KC;
+    const int ntx = (nx + TXV - 1) / TXV, nty = (ny + TYV - 1) / TYV, nkc = (nz + KCV - 1) / KCV;
     const int items = ntx * nty * nkc;
-    const int tx = threadIdx.x % ST7_TX, ty = threadIdx.x / ST7_TX;
+    const int tx = threadIdx.x % TXV, ty = threadIdx.x / TXV;
     for (int it = blockIdx.x; it < items; it += gridDim.x) {
         const int xt = it % ntx, rest = it / ntx, yt = rest % nty, kc = rest / nty;
-        const int i = xt * ST7_TX + tx, j = yt * ST7_TY + ty;
+        const int i = xt * TXV + tx, j = yt * TYV + ty;
         if (i >= nx || j >= ny) continue;
         const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
         const int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
-        const int k0 = kc * ST7_KC, k1 = min(k0 + ST7_KC, nz);
+        const int k0 = kc * KCV, k1 = min(k0 + KCV, nz);
         const size_t col = (size_t)i + (size_t)nx * j;
         const size_t oxm = (size_t)im + (size_t)nx * j, oxp = (size_t)ip + (size_t)nx * j;
         const size_t oym = (size_t)i + (size_t)nx * jm, oyp = (size_t)i + (size_t)nx * jp;
-        T prev[2] = {0, 0}, cur[2] = {0, 0};
+        // Software pipeline: the loads of plane k+1 (its four in-plane
+        // neighbours, the column value at k+2, pi at k+1) are issued before
+        // plane k's arithmetic, so a column waits on memory once per plane
+        // with the next plane's loads in flight.
+        T prev[2] = {0, 0}, cur[2] = {0, 0}, nxt[2] = {0, 0}, nb[2][4] = {}, pp[2] = {0, 0};
+        auto load_plane = [&](int k, T (&n)[2][4], T (&x)[2], T (&q)[2]) {
+            const size_t pk = plane * (size_t)k, pn = plane * (size_t)(k == nz - 1 ? 0 : k + 1);
+#pragma unroll
+            for (int l = 0; l < 2; ++l) {
+                if (l >= nf) break;
+                const T *f = fld + (size_t)l * nvol;
+                n[l][0] = f[oxm + pk];
+                n[l][1] = f[oxp + pk];
+                n[l][2] = f[oym + pk];
+                n[l][3] = f[oyp + pk];
+                x[l] = f[col + pn];
+                q[l] = fldp[(size_t)l * nvol + col + pk];
+            }
+        };
         for (int l = 0; l < nf; ++l) {
             const T *f = fld + (size_t)l * nvol;
             prev[l] = f[col + plane * (size_t)(k0 == 0 ? nz - 1 : k0 - 1)];
             cur[l] = f[col + plane * (size_t)k0];
         }
+        load_plane(k0, nb, nxt, pp);
         for (int k = k0; k < k1; ++k) {
-            const size_t pk = plane * (size_t)k, pn = plane * (size_t)(k == nz - 1 ? 0 : k + 1);
-            T lap[2] = {0, 0}, phi[2] = {0, 0}, next[2] = {0, 0};
-            for (int l = 0; l < nf; ++l) {
-                const T *f = fld + (size_t)l * nvol;
-                next[l] = f[col + pn];
-                const T r1 = f[oxm + pk] + f[oxp + pk] + f[oym + pk] + f[oyp + pk] + prev[l] + next[l];
+            const size_t pk = plane * (size_t)k;
+            T nb2[2][4] = {}, nxt2[2] = {0, 0}, pp2[2] = {0, 0};
+            if (k + 1 < k1) load_plane(k + 1, nb2, nxt2, pp2);
+            T lap[2] = {0, 0}, phi[2] = {0, 0};
+#pragma unroll
+            for (int l = 0; l < 2; ++l) {
+                if (l >= nf) break;
+                const T r1 = nb[l][0] + nb[l][1] + nb[l][2] + nb[l][3] + prev[l] + nxt[l];
                 lap[l] = (c0 * cur[l] + c1 * r1) * el;
                 phi[l] = cur[l];
             }
@@ -386,7 +407,7 @@ __global__ __launch_bounds__(TPB) void k_stencil7(const T *__restrict__ fld, T *
             if constexpr (EPI == EPI_KICK) site_dv(md, pb32, phi, dv);
             acc[RC_PE] += site_v(md, pb64, phi);
             for (int l = 0; l < nf; ++l) {
-                T p = fldp[(size_t)l * nvol + s];
+                T p = pp[l];
                 if constexpr (EPI == EPI_KICK) {
                     p = p + (hA * lap[l] - hB * dv[l]);
                     if constexpr (sizeof(T) == 4) p = p - mu[l];
@@ -398,9 +419,14 @@ __global__ __launch_bounds__(TPB) void k_stencil7(const T *__restrict__ fld, T *
             }
             acc[RC_M1] += double(phi[0]);
             acc[RC_M2] += double(phi[1]);
-            for (int l = 0; l < nf; ++l) {
+#pragma unroll
+            for (int l = 0; l < 2; ++l) {
                 prev[l] = cur[l];
-                cur[l] = next[l];
+                cur[l] = nxt[l];
+                nxt[l] = nxt2[l];
+                pp[l] = pp2[l];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) nb[l][e] = nb2[l][e];
             }
         }
     }
