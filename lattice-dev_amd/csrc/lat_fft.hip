@@ -157,7 +157,7 @@ static int c2r_go(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int nf, int epi, T *out,
                   const ModelDev &md)
 {
     const int rows = c.ny * c.nz;
-    const int g = grid_for((rows + L - 1) / L, c2r_grid<T>());
+    const int g = grid_for((rows + L - 1) / L, c2r_grid_n<T, N>());
     if (epi == EPI_STORE)
         hipLaunchKernelGGL((k_x_c2r<T, N, L, EPI_STORE, 4, true, true>), dim3(g, c.B), dim3(TPB), 0, c.stream, Fk, tw,
                            rows, c.P, c.nk, c.nvol, nf, out, fld, fldp, c.dsc, md, c.partials, c.ms);

@@ -263,7 +263,7 @@ template <typename T, int N, int V>
 static Geo c2r_geo(const Ctx &c)
 {
     constexpr int L = c2r_L<N, V>();
-    return {grid_for((c.ny * c.nz + L - 1) / L, c2r_grid<T>()), sizeof(cpx<T>) * c2r_lds<T, N, L, true>()};
+    return {grid_for((c.ny * c.nz + L - 1) / L, c2r_grid_n<T, N>()), sizeof(cpx<T>) * c2r_lds<T, N, L, true>()};
 }
 
 // Fills the block map; returns the total number of blocks.

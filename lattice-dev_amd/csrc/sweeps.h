@@ -590,6 +590,11 @@ constexpr int R2C_GRID = 1 << 30;
 template <typename T>
 constexpr int c2r_grid() { return sizeof(T) == 8 ? 1024 : 8192; }
 static_assert(c2r_grid<float>() <= MAX_GRID && c2r_grid<double>() <= MAX_GRID, "one partial per block");
+// Lattices up to 256 per side: at most 2048 blocks, so the scalar kernel
+// that sums one reduction partial per block has a quarter of them to read
+// (at 256^3 that kernel was 10 % of the mixed-form step with 8192).
+template <typename T, int N>
+constexpr int c2r_grid_n() { return (N <= 256 && c2r_grid<T>() > 2048) ? 2048 : c2r_grid<T>(); }
 
 // y/z column passes: TC adjacent kx columns per tile.  y passes: 8 columns
 // at N = 1024 (128 B rows in FP64), radix <= 16, 512 threads.  Fused z pass:
