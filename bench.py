@@ -74,10 +74,12 @@ def algorithmic_bytes(stage, n, nf, w):
     }.get(stage)
 
 
-def run_form(la, n, precision, steps, warmup, out_every, dt, sep=False, laplacian=None):
+def run_form(la, n, precision, steps, warmup, out_every, dt, sep=False, laplacian=None, tune=False):
     lat = make_lattice(la, n, precision, laplacian)
     if sep:
         lat.set_laplacian_form(True)
+    if tune:
+        lat.autotune()  # launch shapes for lattices up to 256 per side
     t0 = time.time()
     init_state(lat)
     lat.synchronize()
@@ -318,15 +320,16 @@ def config1_leg(la, args):
     """The 256^3 drop-in configuration: two-field preheating (g^2/m^2 =
     2.5e5, cosmic FRW, L = 10/m, dt = dx/40, Yoshida-4, spectra of both
     fields every 20 steps) with the spectral and the 7-point FD Laplacian,
-    FP64 and mixed.  site-steps/s, and the HBM rate of the model bytes per
-    site-step (spectral 3 nf (7w + 8w_c), FD 18 nf w) against 8 TB/s."""
+    FP64 and mixed, launch shapes autotuned.  site-steps/s, and the HBM rate
+    of the model bytes per site-step (spectral 3 nf (7w + 8w_c), FD 18 nf w)
+    against 8 TB/s."""
     n = args.c1_n
     dt = (10.0 / n) / 40.0
     out = {"metric": f"site-steps/s at {n}^3, spectral and 7-point FD Laplacian", "steps": args.c1_steps,
            "warmup": args.warmup, "spectra_every": 20, "runs": []}
     for lap_name, lap in (("spectral", la.LAP_SPECTRAL), ("fd7", la.LAP_STENCIL)):
         for form, prec in (("f64", la.F64), ("mixed", la.MIXED)):
-            r = run_form(la, n, prec, args.c1_steps, args.warmup, 20, dt, laplacian=lap)
+            r = run_form(la, n, prec, args.c1_steps, args.warmup, 20, dt, laplacian=lap, tune=True)
             w = 8 if form == "f64" else 4
             wc = w * (1.0 + 2.0 / n)
             model = 3 * 2 * (7 * w + 8 * wc) if lap_name == "spectral" else 18 * 2 * w
