@@ -314,11 +314,12 @@ __global__ __launch_bounds__(TPB) void k_stencil(const T *__restrict__ fld, T *_
 }
 
 // 7-point stencil (c2 = c3 = 0) + kick / reduce, marching along z: a
-// 64 x 4 block owns an (x, y) tile and a chunk of ST7_KC planes; each thread
-// keeps its column's values at k-1, k, k+1 in registers, so per site it loads
-// the new k+1 value and the four in-plane neighbours (which the tile's other
-// lanes load too) instead of seven scattered values.  The neighbour sum is
-// formed in stencil_sum's order, so the Laplacian is the same as there.
+// TXV x TYV block (64 x 4) owns an (x, y) tile and a chunk of KCV (32)
+// planes; each thread keeps its column's values at k-1, k, k+1 in registers,
+// so per site it loads the new k+1 value and the four in-plane neighbours
+// (which the tile's other lanes load too) instead of seven scattered values.
+// The neighbour sum is formed in stencil_sum's order, so the Laplacian is the
+// same as there.
 constexpr int ST7_TX = 64, ST7_TY = 4, ST7_KC = 32;
 
 template <typename T, int EPI, int TXV = ST7_TX, int TYV = ST7_TY, int KCV = ST7_KC>

@@ -2,7 +2,8 @@
 // column passes, x C2R + epilogue) and their launch shapes.
 //
 // Each body takes its block coordinates (Blk: block index and grid size along
-// the tile loop, member index) and its LDS as arguments, so the same code
+// the tile loop, member index) and its LDS (transform buffers and twiddle
+// table as separate restrict pointers) as arguments, so the same code
 // runs as a kernel of its own (lat_fft.hip: one lattice size per launch,
 // coordinates from blockIdx, static LDS) or inside the multi-size kernels of
 // lat_multi.hip (one launch for every size class of an ensemble, coordinates
