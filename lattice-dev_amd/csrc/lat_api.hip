@@ -876,7 +876,7 @@ static int ensure_spec(Ctx &c)
     const size_t cap = (size_t)c.spec_cap;
     int rc = ensure(c, (void **)&c.spec_part, sizeof(double) * (BIN_BLOCKS + BIN_BLOCKS / BIN_GROUP) * 5 * cap,
                     "hipMalloc spec_part");
-    rc |= ensure(c, (void **)&c.spec_out, sizeof(double) * 5 * cap, "hipMalloc spec_out");
+    rc |= ensure(c, (void **)&c.spec_out, sizeof(double) * (5 + Ctx::WCACHE) * cap, "hipMalloc spec_out");
     return rc;
 }
 
@@ -1039,19 +1039,21 @@ static int output_spectra_t(Ctx &c, int field, double *out)
     if ((rc = launch_bin<T>(c, Fpi, Fphi, 2)) < 0) return rc;
     std::vector<double> raw(5 * c.ns);
     LAT_HIP(&c, hipMemcpyAsync(raw.data(), c.spec_out, sizeof(double) * 5 * c.ns, hipMemcpyDeviceToHost, c.stream));
+    cpx<T> f0[2];
+    if (sizeof(T) == 4) {
+        LAT_HIP(&c, hipMemcpyAsync(&f0[0], Fphi, sizeof(cpx<T>), hipMemcpyDeviceToHost, c.stream));
+        LAT_HIP(&c, hipMemcpyAsync(&f0[1], Fpi, sizeof(cpx<T>), hipMemcpyDeviceToHost, c.stream));
+        if ((rc = pull_scalars(c))) return rc;
+    }
     if ((rc = sync(c))) return rc;
     if (sizeof(T) == 4) {
         // Mixed form: the FP32 transform holds the fluctuations only.  Bin 0 is
-        // the k = 0 mode alone (weight 1), so it is set from the FP64 means.
-        const int g = launch_reduce_state<T>(c);
-        if (g < 0) return g;
-        ScalarOps none{};
-        if ((rc = launch_scalar(c, none, g))) return rc;
-        if ((rc = pull_scalars(c))) return rc;
-        set_ke_valid(c, true);
+        // the k = 0 mode alone (weight 1): the FP64 mean times the volume plus
+        // the fluctuation sum, which is the transform's own k = 0 coefficient
+        // (its rounding is that of every other mode of the same transform).
         const double nv = (double)c.nvol;
-        const double fphi = nv * c.host_sc.mean_phi[field] + c.host_sc.red[RC_M1 + field];
-        const double fpi = nv * c.host_sc.mean_pi[field] + c.host_sc.red[RC_S1 + field];
+        const double fphi = nv * c.host_sc.mean_phi[field] + (double)f0[0].x;
+        const double fpi = nv * c.host_sc.mean_pi[field] + (double)f0[1].x;
         raw[0] = fphi * fphi;
         raw[(size_t)c.ns] = fpi * fpi;
         raw[2 * (size_t)c.ns] = fpi * fphi;

@@ -133,6 +133,14 @@ struct Ctx {
     int ns;
     int spec_cap;           // bins the histogram buffers hold
     int spec_blocks;
+    // The weight channel W(b) of the binning depends on the lattice geometry,
+    // the bin count and the weighting flags only, so launch_bin computes it
+    // once per (bin count, flags) and keeps it in one of WCACHE slots of
+    // spec_cap doubles after the 5 * spec_cap of spec_out.  A view shares its
+    // batch's spec_out and therefore uses its batch's keys.
+    static constexpr int WCACHE = 4;
+    long long wkey[WCACHE] = {-1, -1, -1, -1};
+    int wnext = 0;
 
     // twiddle tables w[m] = exp(-2 pi i m / n) for n = nx, ny, nz
     cpx<double> *twx64, *twy64, *twz64;

@@ -70,7 +70,10 @@ __device__ __forceinline__ double tt_norm2(const double kv[3], const double ur[6
 
 // channel sets: mode 0 -> |F1|^2 ; mode 1 -> Re, Im of F1 conj(F2) ;
 // mode 2 -> |F2|^2, |F1|^2, Re, Im of F1 conj(F2) ; mode 3 -> sum_ab |Lambda U|^2
-// of the six arrays F1 + c*nk.  W is always the last channel.
+// of the six arrays F1 + c*nk ; mode 4 -> no value channel (W only).  With
+// WCH the weight W is binned too, as the last channel; it depends on the
+// geometry and the flags only, so launch_bin bins it once (mode 4) and runs
+// the value modes without it.
 // flags (Discrete/Entropy/analysis.f90 variants):
 //   BIN_COUNTS  integer-count bins floor(p) (spectrum_bin, :22-52) instead of
 //               the two-bin weights;
@@ -131,14 +134,15 @@ __device__ __forceinline__ void seg_scan(int key, double (&v1)[NC], double (&v2)
     seg_scan_step<0x143, 0xC>(key, v1, v2); // row_bcast:31 into rows 2, 3
 }
 
-template <typename T, int NCH, int MODE>
+template <typename T, int NCH, int MODE, bool WCH>
 __global__ __launch_bounds__(64 * BIN_WAVES) void k_bin(const cpx<T> *__restrict__ F1, const cpx<T> *__restrict__ F2,
                                                         int nx, int ny, int nz, int nxh, int P, int ns,
                                                         double *__restrict__ part, int flags, size_t nk)
 {
-    extern __shared__ double hist[]; // (NCH+1) * ns
+    constexpr int NC = NCH + (WCH ? 1 : 0); // binned channels
+    extern __shared__ double hist[];       // NC * ns
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int q = threadIdx.x; q < (NCH + 1) * ns; q += 64 * BIN_WAVES) hist[q] = 0.0;
+    for (int q = threadIdx.x; q < NC * ns; q += 64 * BIN_WAVES) hist[q] = 0.0;
     __syncthreads();
     const int rows = ny * nz;
     const bool counts = (flags & BIN_COUNTS) != 0;
@@ -146,7 +150,7 @@ __global__ __launch_bounds__(64 * BIN_WAVES) void k_bin(const cpx<T> *__restrict
     // one 64-lane chunk of a row: segment sums of bins l (v1) and l + 1 (v2)
     // at the segment tails (tail = true)
     auto chunk = [&](bool has_row, int row, int j, int k, int jj, int kk, int i0, cpx<T> a, cpx<T> b, int &key,
-                     bool &tail, double (&v1)[NCH + 1], double (&v2)[NCH + 1]) {
+                     bool &tail, double (&v1)[NC], double (&v2)[NC]) {
         const int i = i0 + lane;
         const bool valid = has_row && i < nxh && !((flags & BIN_NO_NYQ) && 2 * i == nx);
         key = 0x7fffffff;
@@ -157,8 +161,10 @@ __global__ __launch_bounds__(64 * BIN_WAVES) void k_bin(const cpx<T> *__restrict
             const double c1 = counts ? 1.0 : (1.0 - d1 * d1) * (1.0 - d1 * d1);
             const double c2 = counts ? 0.0 : (1.0 - d2 * d2) * (1.0 - d2 * d2);
             const double mult = ((flags & BIN_HALF) || i == 0 || (2 * i == nx)) ? 1.0 : 2.0;
-            double val[NCH];
-            if constexpr (MODE == 3) {
+            double val[NCH > 0 ? NCH : 1];
+            if constexpr (MODE == 4) {
+                val[0] = 0.0;
+            } else if constexpr (MODE == 3) {
                 const size_t id = (size_t)row * P + i;
                 const double kv[3] = {double(i), double(j <= ny / 2 ? j : j - ny), double(k <= nz / 2 ? k : k - nz)};
                 double ur[6], ui[6];
@@ -189,15 +195,17 @@ __global__ __launch_bounds__(64 * BIN_WAVES) void k_bin(const cpx<T> *__restrict
                 v1[ch] = mult * c1 * val[ch];
                 v2[ch] = mult * c2 * val[ch];
             }
-            v1[NCH] = mult * c1;
-            v2[NCH] = mult * c2;
+            if constexpr (WCH) {
+                v1[NCH] = mult * c1;
+                v2[NCH] = mult * c2;
+            }
             key = l;
         } else {
 #pragma unroll
-            for (int ch = 0; ch <= NCH; ++ch) v1[ch] = v2[ch] = 0.0;
+            for (int ch = 0; ch < NC; ++ch) v1[ch] = v2[ch] = 0.0;
         }
         // segmented inclusive scan over lanes with equal key
-        seg_scan<NCH + 1>(key, v1, v2);
+        seg_scan<NC>(key, v1, v2);
         const int nkey = __shfl_down(key, 1, 64);
         tail = valid && (lane == 63 || nkey != key);
     };
@@ -206,14 +214,14 @@ __global__ __launch_bounds__(64 * BIN_WAVES) void k_bin(const cpx<T> *__restrict
     // a wave the tails have distinct keys; bin l + 1 of one segment may be
     // bin l of the next, so the l + 1 additions follow the l additions in
     // program order.
-    auto update = [&](int key, bool tail, const double (&v1)[NCH + 1], const double (&v2)[NCH + 1]) {
+    auto update = [&](int key, bool tail, const double (&v1)[NC], const double (&v2)[NC]) {
         for (int w = 0; w < BIN_WAVES; ++w) {
             if (wave == w && tail) {
 #pragma unroll
-                for (int ch = 0; ch <= NCH; ++ch) hist[ch * ns + key] += v1[ch];
+                for (int ch = 0; ch < NC; ++ch) hist[ch * ns + key] += v1[ch];
                 if (key + 1 < ns) {
 #pragma unroll
-                    for (int ch = 0; ch <= NCH; ++ch) hist[ch * ns + key + 1] += v2[ch];
+                    for (int ch = 0; ch < NC; ++ch) hist[ch * ns + key + 1] += v2[ch];
                 }
             }
             __syncthreads();
@@ -229,8 +237,8 @@ __global__ __launch_bounds__(64 * BIN_WAVES) void k_bin(const cpx<T> *__restrict
         const int kk = (k <= nz / 2) ? k : nz - k;
         int key;
         bool tail;
-        double v1[NCH + 1], v2[NCH + 1];
-        if constexpr (MODE == 3) {
+        double v1[NC], v2[NC];
+        if constexpr (MODE == 3 || MODE == 4) {
             for (int i0 = 0; i0 < nxh; i0 += 64) {
                 chunk(has_row, rr, j, k, jj, kk, i0, cpx<T>{0, 0}, cpx<T>{0, 0}, key, tail, v1, v2);
                 update(key, tail, v1, v2);
@@ -259,8 +267,8 @@ __global__ __launch_bounds__(64 * BIN_WAVES) void k_bin(const cpx<T> *__restrict
         }
     }
     __syncthreads();
-    double *dst = part + (size_t)blockIdx.x * (NCH + 1) * ns;
-    for (int q = threadIdx.x; q < (NCH + 1) * ns; q += 64 * BIN_WAVES) dst[q] = hist[q];
+    double *dst = part + (size_t)blockIdx.x * NC * ns;
+    for (int q = threadIdx.x; q < NC * ns; q += 64 * BIN_WAVES) dst[q] = hist[q];
 }
 
 // out[g][q] = sum over blocks b of group g (in order) of part[b][q]; with
@@ -277,6 +285,17 @@ __global__ __launch_bounds__(256) void k_bin_reduce(const double *__restrict__ p
     out[(size_t)g * nq + q] = s;
 }
 
+// The two fixed-order reduction stages over the block histograms of nq
+// doubles each: groups of BIN_GROUP blocks, then the groups, into out.
+static void bin_reduce(Ctx &c, int g, int nq, double *out)
+{
+    const int ngrp = (g + BIN_GROUP - 1) / BIN_GROUP;
+    double *grp = c.spec_part + (size_t)BIN_BLOCKS * nq;
+    hipLaunchKernelGGL(k_bin_reduce, dim3((nq + 255) / 256, ngrp), dim3(256), 0, c.stream, c.spec_part, g, BIN_GROUP,
+                       nq, grp);
+    hipLaunchKernelGGL(k_bin_reduce, dim3((nq + 255) / 256, 1), dim3(256), 0, c.stream, grp, ngrp, ngrp, nq, out);
+}
+
 template <typename T>
 int launch_bin(Ctx &c, const cpx<T> *F1, const cpx<T> *F2, int mode, int nbins, int flags)
 {
@@ -289,33 +308,47 @@ int launch_bin(Ctx &c, const cpx<T> *F1, const cpx<T> *F2, int mode, int nbins, 
     const int rows = c.ny * c.nz;
     const int gr = (rows + BIN_WAVES - 1) / BIN_WAVES; // row groups
     const int g = gr < BIN_BLOCKS ? gr : BIN_BLOCKS;
-    const size_t lds = sizeof(double) * (nch + 1) * ns;
     if (mode != 3 && c.nxh > BIN_ROWCHUNKS * 64) {
         c.last_error = "spectrum rows longer than the binning kernel's row buffer";
         return -1;
     }
+    // weight channel: from the cache slot of (ns, flags), binned on a miss
+    Ctx &owner = (c.view && c.parent) ? *c.parent : c;
+    const long long key = ((long long)flags << 32) | (unsigned)ns;
+    int slot = -1;
+    for (int q = 0; q < Ctx::WCACHE; ++q)
+        if (owner.wkey[q] == key) slot = q;
+    const size_t cap = (size_t)c.spec_cap;
+    if (slot < 0) {
+        slot = owner.wnext;
+        owner.wnext = (owner.wnext + 1) % Ctx::WCACHE;
+        owner.wkey[slot] = -1;
+        hipLaunchKernelGGL((k_bin<T, 0, 4, true>), dim3(g), dim3(64 * BIN_WAVES), sizeof(double) * ns, c.stream, F1, F2,
+                           c.nx, c.ny, c.nz, c.nxh, c.P, ns, c.spec_part, flags, c.nk);
+        LAT_HIP(&c, hipGetLastError());
+        bin_reduce(c, g, ns, c.spec_out + (5 + slot) * cap);
+        LAT_HIP(&c, hipGetLastError());
+        owner.wkey[slot] = key;
+    }
+    const size_t lds = sizeof(double) * nch * ns;
     if (mode == 0)
-        hipLaunchKernelGGL((k_bin<T, 1, 0>), dim3(g), dim3(64 * BIN_WAVES), lds, c.stream, F1, F2, c.nx, c.ny, c.nz, c.nxh, c.P,
-                           ns, c.spec_part, flags, c.nk);
+        hipLaunchKernelGGL((k_bin<T, 1, 0, false>), dim3(g), dim3(64 * BIN_WAVES), lds, c.stream, F1, F2, c.nx, c.ny, c.nz,
+                           c.nxh, c.P, ns, c.spec_part, flags, c.nk);
     else if (mode == 1)
-        hipLaunchKernelGGL((k_bin<T, 2, 1>), dim3(g), dim3(64 * BIN_WAVES), lds, c.stream, F1, F2, c.nx, c.ny, c.nz, c.nxh, c.P,
-                           ns, c.spec_part, flags, c.nk);
+        hipLaunchKernelGGL((k_bin<T, 2, 1, false>), dim3(g), dim3(64 * BIN_WAVES), lds, c.stream, F1, F2, c.nx, c.ny, c.nz,
+                           c.nxh, c.P, ns, c.spec_part, flags, c.nk);
     else if (mode == 2)
-        hipLaunchKernelGGL((k_bin<T, 4, 2>), dim3(g), dim3(64 * BIN_WAVES), lds, c.stream, F1, F2, c.nx, c.ny, c.nz, c.nxh, c.P,
-                           ns, c.spec_part, flags, c.nk);
+        hipLaunchKernelGGL((k_bin<T, 4, 2, false>), dim3(g), dim3(64 * BIN_WAVES), lds, c.stream, F1, F2, c.nx, c.ny, c.nz,
+                           c.nxh, c.P, ns, c.spec_part, flags, c.nk);
     else
-        hipLaunchKernelGGL((k_bin<T, 1, 3>), dim3(g), dim3(64 * BIN_WAVES), lds, c.stream, F1, F2, c.nx, c.ny, c.nz, c.nxh, c.P,
-                           ns, c.spec_part, flags, c.nk);
+        hipLaunchKernelGGL((k_bin<T, 1, 3, false>), dim3(g), dim3(64 * BIN_WAVES), lds, c.stream, F1, F2, c.nx, c.ny, c.nz,
+                           c.nxh, c.P, ns, c.spec_part, flags, c.nk);
     LAT_HIP(&c, hipGetLastError());
-    const int nq = (nch + 1) * ns;
-    // two stages, both in fixed order: groups of BIN_GROUP blocks, then the groups
-    const int ngrp = (g + BIN_GROUP - 1) / BIN_GROUP;
-    double *grp = c.spec_part + (size_t)BIN_BLOCKS * nq;
-    hipLaunchKernelGGL(k_bin_reduce, dim3((nq + 255) / 256, ngrp), dim3(256), 0, c.stream, c.spec_part, g, BIN_GROUP,
-                       nq, grp);
-    hipLaunchKernelGGL(k_bin_reduce, dim3((nq + 255) / 256, 1), dim3(256), 0, c.stream, grp, ngrp, ngrp, nq,
-                       c.spec_out);
+    bin_reduce(c, g, nch * ns, c.spec_out);
     LAT_HIP(&c, hipGetLastError());
+    // W after the nch value channels, as the callers read it
+    LAT_HIP(&c, hipMemcpyAsync(c.spec_out + (size_t)nch * ns, c.spec_out + (5 + slot) * cap, sizeof(double) * ns,
+                               hipMemcpyDeviceToDevice, c.stream));
     return nch;
 }
 
