@@ -62,6 +62,11 @@ def test_mixed_tracks_fp64(gpu, lap):
           flush=True)
     assert err3 < 1e-5
     assert err6 < 1e-4
+    # bin 0 of the inflaton (N^3 times the FP64 mean plus the fluctuation sum,
+    # the mixed form's FP32 k = 0 coefficient): S_phi, S_pi and Re S_pi_phi
+    for (m0, _), (d0, _) in zip(sm, sd):
+        for q in range(3):
+            assert abs(m0[q][0] - d0[q][0]) <= 1e-8 * abs(d0[q][0]), (q, m0[q][0], d0[q][0])
     # fluctuation fields
     for l in range(2):
         df = stm[0][l] - std_[0][l]

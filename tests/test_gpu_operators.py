@@ -131,6 +131,39 @@ def test_discrete_nofft_spectra_match_oracle(gpu, n):
     assert np.allclose(c2, r2, rtol=1e-11, atol=1e-13 * scale)
 
 
+def test_spectra_weight_cache_across_bin_sets(gpu):
+    """The binning weights W(b) are binned once per (bin count, weighting
+    flags) and cached in four slots per lattice.  Six spectrum calls with
+    five or six distinct (bin count, flags) sets, run twice over on one
+    lattice (misses, hits and evictions), give bitwise what each call gives
+    on a fresh lattice (where the weights are binned for that call)."""
+    print("case spectra weight cache", flush=True)
+    n = 16
+    rng = np.random.default_rng(77)
+    f = rng.standard_normal((n, n, n))
+    F1 = np.fft.rfftn(rng.standard_normal((n, n, n)))
+    F2 = np.fft.rfftn(rng.standard_normal((n, n, n)))
+    calls = [
+        lambda lat: lat.spectrum_3d(f),
+        lambda lat: lat.spectrum_discrete(f, 0),
+        lambda lat: lat.spectrum_discrete(f, 1),
+        lambda lat: lat.spectrum_discrete_k(F1, 0),
+        lambda lat: lat.spectrum_discrete_k(F1, 1),
+        lambda lat: np.concatenate(lat.crossspec_discrete_k(F1, F2)),
+    ]
+    fresh = []
+    for call in calls:
+        with _lat(gpu, (n, n, n), nfld=1, model=gpu.MODEL_PHI4, model_params=(1.0,)) as lat:
+            fresh.append(call(lat))
+    with _lat(gpu, (n, n, n), nfld=1, model=gpu.MODEL_PHI4, model_params=(1.0,)) as lat:
+        for rep in range(2):
+            for q, call in enumerate(calls):
+                got = call(lat)
+                assert np.array_equal(got, fresh[q]), (rep, q)
+    # the fresh results themselves are pinned to the oracle by the tests above
+    assert np.allclose(fresh[1], O.spectrum_discrete(O.r2c_3d(f), n, 0), rtol=1e-12, atol=0)
+
+
 def test_discrete_spectra_need_cubic_lattice(gpu):
     with _lat(gpu, (16, 32, 8)) as lat:
         with pytest.raises(gpu.LatticeError):
