@@ -129,7 +129,7 @@ struct Ctx {
     DevScalars *dsc;
     double *partials;       // MAX_GRID * NRED
     double *spec_part;      // spectrum block partials
-    double *spec_out;       // 5 * ns reduced spectrum channels
+    double *spec_out;       // 5 * spec_cap reduced channels, then the W cache slots
     int ns;
     int spec_cap;           // bins the histogram buffers hold
     int spec_blocks;
