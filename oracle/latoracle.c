@@ -688,19 +688,21 @@ int orc_crossspec_discrete_k(int n, const double *Fk1, const double *Fk2, double
 /* ------------------------------------------------------------------ */
 
 /* hamiltonian_conformal.f90:215-240 (V = phi^4/4, lambda scaled) and
- * hamiltonian_cosmic.f90:215-232 (V = m2 phi^2/2 + g2 phi^2 chi^2 / 2). */
+ * hamiltonian_cosmic.f90:215-232 (V = m2 phi^2/2 + g2 phi^2 chi^2 / 2), with
+ * the chi mass term m2psi psi^2 of Discrete/Entropy/model.f90:35-55
+ * (modelv = 2V = (m2phi + g2 psi^2) phi^2 + m2psi psi^2) as mp[2]. */
 double orc_potential(const orc_sim *s, const double *f)
 {
     if (s->model == ORC_MODEL_PHI4) return 0.25 * s->mp[0] * (f[0] * f[0]) * (f[0] * f[0]);
     double p2 = f[0] * f[0], c2 = f[1] * f[1];
-    return 0.5 * s->mp[0] * p2 + 0.5 * s->mp[1] * p2 * c2;
+    return 0.5 * s->mp[0] * p2 + 0.5 * s->mp[1] * p2 * c2 + 0.5 * s->mp[2] * c2;
 }
 
 void orc_modeldv(const orc_sim *s, const double *f, double *dv)
 {
     if (s->model == ORC_MODEL_PHI4) { dv[0] = s->mp[0] * f[0] * f[0] * f[0]; return; }
     dv[0] = s->mp[0] * f[0] + s->mp[1] * f[0] * (f[1] * f[1]);
-    dv[1] = s->mp[1] * (f[0] * f[0]) * f[1];
+    dv[1] = s->mp[1] * (f[0] * f[0]) * f[1] + s->mp[2] * f[1];
 }
 
 /* ------------------------------------------------------------------ */
@@ -1258,11 +1260,20 @@ static void dst2(int n, double *x)
     free(z); free(t); plan_free(&p);
 }
 
-/* evolve_spectral.f90:421-506 with the RNG replaced: the per-mode Rayleigh
- * amplitude sqrt(-2 ln u1) and phase 2 pi u2 come from Philox counter
- * idx = i + nxh*(j + ny*k) of `stream` under `seed`. */
-int orc_sample(int nx, int ny, int nz, double len, double gamma, double m2eff, double mpl,
-               double kcut_fac, unsigned long long seed, unsigned long long stream, double *out)
+int orc_dst2(int n, double *x)
+{
+    if (n <= 0) return -1;
+    dst2(n, x);
+    return 0;
+}
+
+/* evolve_spectral.f90:436-491: the real-space convolution kernel of sample()
+ * before the noise multiply -- the oversampled radial profile
+ * ker(kk) = kk (kk^2 + m2eff)^gamma exp(-(kk/kcut)^2) at kk = (k - 1/2) dkos,
+ * DST-II, times norm / k, laid out on the lattice by linear interpolation at
+ * radius |(i,j,k) - nn| os (centre: (4 ker(1) - ker(2)) / 3). */
+int orc_sample_kernel(int nx, int ny, int nz, double len, double gamma, double m2eff, double mpl,
+                      double kcut_fac, double *out)
 {
     const int os = 16;
     const int nmax = nx > ny ? (nx > nz ? nx : nz) : (ny > nz ? ny : nz);
@@ -1282,7 +1293,6 @@ int orc_sample(int nx, int ny, int nz, double len, double gamma, double m2eff, d
     }
     dst2(nos, ker);
     for (int k = 1; k <= nos; ++k) ker[k - 1] = norm * ker[k - 1] / (double)k;
-    size_t nv = (size_t)nx * ny * nz;
     for (int k = 1; k <= nz; ++k)
         for (int j = 1; j <= ny; ++j)
             for (int i = 1; i <= nx; ++i) {
@@ -1295,6 +1305,17 @@ int orc_sample(int nx, int ny, int nz, double len, double gamma, double m2eff, d
                 out[(size_t)(i - 1) + (size_t)nx * ((j - 1) + (size_t)ny * (k - 1))] = v;
             }
     free(ker);
+    return 0;
+}
+
+/* evolve_spectral.f90:421-506 with the RNG replaced: the per-mode Rayleigh
+ * amplitude sqrt(-2 ln u1) and phase 2 pi u2 come from Philox counter
+ * idx = i + nxh*(j + ny*k) of `stream` under `seed`. */
+int orc_sample(int nx, int ny, int nz, double len, double gamma, double m2eff, double mpl,
+               double kcut_fac, unsigned long long seed, unsigned long long stream, double *out)
+{
+    const double twopi = 6.2831853071795864769252867665590;
+    orc_sample_kernel(nx, ny, nz, len, gamma, m2eff, mpl, kcut_fac, out);
     const int nxh = nx / 2 + 1;
     cplx *F = (cplx *)malloc(sizeof(cplx) * (size_t)nxh * ny * nz);
     r2c_3d(nx, ny, nz, out, F);
@@ -1307,6 +1328,5 @@ int orc_sample(int nx, int ny, int nz, double len, double gamma, double m2eff, d
     }
     c2r_3d(nx, ny, nz, F, out);
     free(F);
-    (void)nv;
     return 0;
 }

@@ -114,6 +114,11 @@ void orc_philox_normal_pairs(unsigned long long seed, unsigned long long stream,
 int  orc_sample(int nx, int ny, int nz, double len, double gamma, double m2eff,
                 double mpl, double kcut_fac, unsigned long long seed,
                 unsigned long long stream, double *out);
+/* the sampler's pieces: FFTW RODFT10 (DST-II) in place, and the real-space
+ * convolution kernel before the noise multiply */
+int  orc_dst2(int n, double *x);
+int  orc_sample_kernel(int nx, int ny, int nz, double len, double gamma, double m2eff,
+                       double mpl, double kcut_fac, double *out);
 
 #ifdef __cplusplus
 }

@@ -314,3 +314,21 @@ def sample(n, length, gamma, m2eff, mpl, seed, stream, kcut_fac=1.0):
                      ctypes.c_double(m2eff), ctypes.c_double(mpl), ctypes.c_double(kcut_fac),
                      ctypes.c_ulonglong(seed), ctypes.c_ulonglong(stream), _p(out))
     return out
+
+
+def dst2(x):
+    """FFTW RODFT10 (DST-II), unnormalised: Y_k = 2 sum_j X_j sin(pi (j+1/2)(k+1)/n)."""
+    x = _f64(x).copy()
+    lib().orc_dst2(len(x), _p(x))
+    return x
+
+
+def sample_kernel(n, length, gamma, m2eff, mpl, kcut_fac=1.0):
+    """The sampler's real-space convolution kernel (before the noise)."""
+    if isinstance(n, int):
+        n = (n, n, n)
+    nx, ny, nz = n
+    out = np.zeros((nz, ny, nx))
+    lib().orc_sample_kernel(nx, ny, nz, ctypes.c_double(length), ctypes.c_double(gamma), ctypes.c_double(m2eff),
+                            ctypes.c_double(mpl), ctypes.c_double(kcut_fac), _p(out))
+    return out

@@ -311,15 +311,17 @@ def hom_rhs(time_var, model, mp):
     """Hamilton's equations of the homogeneous system, from the Hamiltonians
     H = -p^2/12 + pi^2/(2a^2) + a^4 V (conformal, y = a) and
     H = -3 p^2/16 + pi^2/(2y^2) + y^2 V (cosmic, y = a^{3/2})."""
+    m2chi = mp[2] if len(mp) > 2 else 0.0
+
     def V(f):
         if model == O.MODEL_PHI4:
             return 0.25 * mp[0] * f[0] ** 4
-        return 0.5 * mp[0] * f[0] ** 2 + 0.5 * mp[1] * f[0] ** 2 * f[1] ** 2
+        return 0.5 * mp[0] * f[0] ** 2 + 0.5 * mp[1] * f[0] ** 2 * f[1] ** 2 + 0.5 * m2chi * f[1] ** 2
 
     def dV(f):
         if model == O.MODEL_PHI4:
             return np.array([mp[0] * f[0] ** 3])
-        return np.array([mp[0] * f[0] + mp[1] * f[0] * f[1] ** 2, mp[1] * f[0] ** 2 * f[1]])
+        return np.array([mp[0] * f[0] + mp[1] * f[0] * f[1] ** 2, mp[1] * f[0] ** 2 * f[1] + m2chi * f[1]])
 
     def rhs(t, u):
         nf = (len(u) - 2) // 2
@@ -334,9 +336,12 @@ def hom_rhs(time_var, model, mp):
 
 CASES = [(O.CONFORMAL, O.MODEL_PHI4, (1.0,), 1, [1.0], [-1.1547]),
          (O.COSMIC, O.MODEL_TWOFIELD, (1.0, 4.0), 2, [1.0093430384226379, 0.3], [-0.7137133070120812, 0.1])]
+# the Discrete tree's chi mass (modelv = (m2phi + g2 psi^2) phi^2 + m2psi psi^2,
+# Discrete/Entropy/model.f90:35-55)
+CASE_MCHI = (O.COSMIC, O.MODEL_TWOFIELD, (1.0, 4.0, 2.5), 2, [1.0093430384226379, 0.3], [-0.7137133070120812, 0.1])
 
 
-@pytest.mark.parametrize("case", CASES, ids=["conformal_phi4", "cosmic_twofield"])
+@pytest.mark.parametrize("case", CASES + [CASE_MCHI], ids=["conformal_phi4", "cosmic_twofield", "cosmic_twofield_mchi"])
 def test_homogeneous_limit_matches_hamilton_ode(case):
     tv, model, mp, nf, f0, p0 = case
     sim = hom_sim(tv, model, mp, nf, f0, p0)
@@ -480,7 +485,188 @@ def test_energy_components_independent_sums():
     assert abs(e["hubble"] - (0.4 / 1.3 / 4.0)) < 1e-15
 
 
+@pytest.mark.parametrize("lap", [O.LAP_SPECTRAL, O.LAP_STENCIL], ids=["spectral", "fd7"])
+def test_conformal_free_massless_modes_static_background(lap):
+    """Conformal time with V = 0 (lambda = 0) and a field amplitude of 1e-7,
+    so the background stays at a = a0 = 1.7 (its momentum moves by O(1e-14)):
+    the kick pi += h a^2 lap phi and the drift phi += h pi / a^2 give
+    phi'' = lap phi, every mode rotating with omega = |k| (spectral) or the
+    7-point eigenvalue (FD), independent of a0.  A wrong a-power in either
+    (hamiltonian_conformal.f90:98,183-186) would give omega = a0^m |k|."""
+    n, L, a0 = 8, 8.0, 1.7
+    rng = np.random.default_rng(31)
+    stencil = (-6.0, 1.0, 0.0, 0.0, 1.0)
+    sim = O.Simulation(n, 1, L, O.CONFORMAL, lap_kind=lap, stencil=stencil, model=O.MODEL_PHI4, mp=(0.0,))
+    f = 1e-7 * rng.standard_normal((n, n, n))
+    p = 1e-7 * rng.standard_normal((n, n, n))
+    sim.fld[0], sim.fldp[0] = f, p
+    sim.yscl, sim.ysclp = a0, 0.0
+    T, dt = 2.0, 2e-3
+    sim.symp(4, dt, int(round(T / dt)))
+    assert abs(sim.yscl - a0) < 1e-12
+    dk, dx = 2 * np.pi / L, L / n
+    ii = np.fft.fftfreq(n, 1.0 / n)
+    if lap == O.LAP_SPECTRAL:
+        kx = np.abs(ii).copy()
+        kx[n // 2] = n // 2
+        k2 = dk**2 * (kx[:, None, None] ** 2 + kx[None, :, None] ** 2 + kx[None, None, :] ** 2)
+    else:
+        s = np.sin(np.pi * ii / n) ** 2
+        k2 = 4.0 / dx**2 * (s[:, None, None] + s[None, :, None] + s[None, None, :])
+    w = np.sqrt(k2)
+    sinc = np.where(w > 0, np.sin(w * T) / np.where(w > 0, w, 1.0), T)
+    F, Pk = np.fft.fftn(f), np.fft.fftn(p) / a0**2
+    fe = np.fft.ifftn(F * np.cos(w * T) + Pk * sinc).real
+    assert np.linalg.norm(sim.fld[0] - fe) < 2e-5 * np.linalg.norm(fe)
+    # the same run with omega scaled by a0 (a wrong a-power) is far off
+    fw = np.fft.ifftn(F * np.cos(a0 * w * T) + Pk * np.where(w > 0, np.sin(a0 * w * T) / np.where(w > 0, a0 * w, 1.0),
+                                                               T)).real
+    assert np.linalg.norm(sim.fld[0] - fw) > 1e-1 * np.linalg.norm(fe)
+
+
+def test_conformal_friedmann_constraint_with_gradient_energy():
+    """Conformal lambda phi^4 with strong fluctuations (gradient energy a
+    sizeable part of rho) while a grows from 1: the Friedmann residual
+    (rho - 3H^2)/rho starts at 0 (calc_metric) and its violation converges
+    at 4th order in dt.  The a-momentum update uses GE2 / a^2 and 4 PE
+    (hamiltonian_conformal.f90:203-205) and GE in the energies carries 1/a^2
+    (evolve_spectral.f90:402): a wrong power of a in either leaves an O(1)
+    violation that does not shrink with dt."""
+    n, L = 8, 5.0
+    rng = np.random.default_rng(41)
+    f0 = 1.0 + 0.05 * rng.standard_normal((n, n, n))
+    p0 = -1.15 + 0.05 * rng.standard_normal((n, n, n))
+    res = []
+    for dt in (0.02, 0.01):
+        sim = O.Simulation(n, 1, L, O.CONFORMAL, model=O.MODEL_PHI4, mp=(1.0,))
+        sim.fld[0], sim.fldp[0] = f0, p0
+        sim.yscl = 1.0
+        sim.calc_metric()
+        e0 = sim.energies()
+        assert abs(e0["resid"]) < 1e-14
+        assert e0["ge"] > 0.02 * e0["rho"]
+        sim.symp(4, dt, int(round(1.0 / dt)))
+        e = sim.energies()
+        assert e["a"] > 1.3
+        res.append(abs(e["resid"]))
+    assert res[0] < 1e-6
+    assert 10.0 < res[0] / res[1] < 24.0, res
+
+
+def test_conformal_energy_components_independent_sums():
+    """dump_rho in conformal time (evolve_spectral.f90:305-415 with
+    hamiltonian_conformal.f90:242-261): KE = <pi^2>/2 / a^6, PE = <V>,
+    GE = sum k^2 |phi_k|^2 / (2 N^6) / a^2 (Parseval), H = -p_a / (6 a^2),
+    at a = 1.6."""
+    n, L, lam, a, pa = 8, 6.0, 0.7, 1.6, -0.9
+    rng = np.random.default_rng(43)
+    sim = O.Simulation(n, 1, L, O.CONFORMAL, model=O.MODEL_PHI4, mp=(lam,))
+    sim.fld[:] = rng.standard_normal(sim.fld.shape)
+    sim.fldp[:] = rng.standard_normal(sim.fldp.shape)
+    sim.yscl, sim.ysclp = a, pa
+    e = sim.energies()
+    ke = 0.5 * np.mean(sim.fldp[0] ** 2) / a**6
+    pe = np.mean(0.25 * lam * sim.fld[0] ** 4)
+    dk = 2 * np.pi / L
+    ii = np.abs(np.fft.fftfreq(n, 1.0 / n))
+    ii[n // 2] = n // 2
+    k2 = dk**2 * (ii[:, None, None] ** 2 + ii[None, :, None] ** 2 + ii[None, None, :] ** 2)
+    ge = 0.5 * np.sum(k2 * np.abs(np.fft.fftn(sim.fld[0])) ** 2) / n**6 / a**2
+    hub = -pa / 6.0 / a**2
+    assert abs(e["ke"] - ke) < 1e-13 * ke and abs(e["pe"] - pe) < 1e-13 * pe
+    assert abs(e["ge"] - ge) < 1e-12 * ge
+    assert abs(e["hubble"] - hub) < 1e-15
+    assert abs(e["rho"] - (ke + pe + ge)) < 1e-13 * e["rho"]
+    assert abs(e["resid"] - (ke + pe + ge - 3 * hub**2) / (ke + pe + ge)) < 1e-12
+
+
 # ---------------------------------------------------------------- IC sampler
+
+@pytest.mark.parametrize("n", [1, 3, 8, 17, 64])
+def test_dst2_matches_direct_sum_and_scipy(n):
+    """FFTW RODFT10, the sampler's radial transform (evolve_spectral.f90:457-459):
+    Y_k = 2 sum_j X_j sin(pi (j + 1/2)(k + 1) / n), against the O(n^2) sum and
+    scipy.fft.dst(type=2) (the same unnormalised definition)."""
+    from scipy.fft import dst
+
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n)
+    j = np.arange(n)
+    direct = 2.0 * np.sin(np.pi * np.outer(j + 1, j + 0.5) / n) @ x
+    got = O.dst2(x)
+    assert np.abs(got - direct).max() <= 1e-12 * max(1.0, np.abs(direct).max())
+    assert np.abs(got - dst(x, type=2)).max() <= 1e-12 * max(1.0, np.abs(direct).max())
+
+
+VAC_CASES = [(16, 10.0, -0.25, 1.0, 2e5), (32, 20.0, -0.25, 3.0, 1e7 / 3.0), (16, 10.0, 0.25, 1.0, 2e5),
+             (24, 7.0, -0.25, 50.0, 1.0)]
+
+
+def _vacuum_profile(n, L, gamma, m2, mpl):
+    """Continuum transform of the sampler's kernel: the inverse 3-D Fourier
+    transform of P(k) = (k^2 + m2)^gamma exp(-(k/kcut)^2), kcut = nn dk / 2,
+    radial by the DST, scaled by norm, sampled on the lattice and
+    transformed back is P(k) dk^(3/2) / (4 pi sqrt(2 pi) mpl), times the
+    phase exp(-i k.x0) of the kernel's centre x0 = (n/2) dx per axis."""
+    dk = 2 * np.pi / L
+    kz = np.fft.fftfreq(n, 1.0 / n)
+    kx = np.arange(n // 2 + 1)
+    k = dk * np.sqrt(kz[:, None, None] ** 2 + kz[None, :, None] ** 2 + kx[None, None, :] ** 2)
+    kcut = (n // 2 + 1) * dk / 2
+    P = (k**2 + m2) ** gamma * np.exp(-(k / kcut) ** 2)
+    sign = (-1.0) ** (np.abs(kz[:, None, None]) + np.abs(kz[None, :, None]) + kx[None, None, :])
+    return k, kcut, sign * P * dk**1.5 / (4 * np.pi * np.sqrt(2 * np.pi) * mpl)
+
+
+@pytest.mark.parametrize("case", VAC_CASES, ids=["16", "32-cfg0", "16-momentum", "24-heavy"])
+def test_sample_kernel_matches_continuum_profile(case):
+    """The kernel's lattice transform against the continuum transform of
+    the radial profile for every mode up to the cutoff: pins the DST, the
+    norm constant (evolve_spectral.f90:447-448), the exponent gamma, m2eff,
+    the cutoff, the division by k and the linear interpolation with the
+    centre extrapolation.  A wrong factor in norm, or gamma of the wrong
+    sign, is off by far more than the 0.5 % bound."""
+    n, L, gamma, m2, mpl = case
+    G = np.fft.rfftn(O.sample_kernel(n, L, gamma, m2, mpl))
+    k, kcut, pred = _vacuum_profile(n, L, gamma, m2, mpl)
+    m = k <= kcut
+    assert np.abs(G.imag[m]).max() < 1e-6 * np.abs(pred[m]).max()  # real: centred kernel
+    ratio = G.real[m] / pred[m]
+    assert np.all(np.abs(ratio - 1.0) < 5e-3), (ratio.min(), ratio.max())
+
+
+def test_sample_power_matches_vacuum_statistics():
+    """Sampled fields over 40 seeds: E |R2C(f)_k|^2 = N^6 * 2 |G_k|^2 for
+    modes with 0 < kx < N/2 (Rayleigh amplitude sqrt(-2 ln u): E amp^2 = 2,
+    uniform phase) and N^6 |G_k|^2 on the kx = 0 and kx = N/2 planes, whose
+    non-Hermitian noise the C2R symmetrises (the imaginary parts of the x-DC
+    and x-Nyquist coefficients are dropped after the y/z passes).  So the
+    sampled vacuum has E |phi_k|^2 = N^6 dk^3 / ((2 pi)^3 2 omega_k mpl^2)
+    (gamma = -1/4) in the interior.  Bounds: four standard deviations of the
+    seed average plus the kernel's 0.5 %."""
+    n, L, gamma, m2, mpl = 16, 10.0, -0.25, 1.0, 2e5
+    k, kcut, pred = _vacuum_profile(n, L, gamma, m2, mpl)
+    acc = np.zeros(k.shape)
+    M = 40
+    for seed in range(M):
+        acc += np.abs(np.fft.rfftn(O.sample(n, L, gamma, m2, mpl, 900 + seed, 0))) ** 2
+    ratio = acc / M / (n**6 * np.abs(pred) ** 2)
+    inner = (k <= kcut) & (k > 0)
+    interior = inner.copy()
+    interior[:, :, 0] = False
+    interior[:, :, n // 2] = False
+    planes = inner & ~interior
+    r_int, r_pl = ratio[interior].mean(), ratio[planes].mean()
+    assert abs(r_int - 2.0) < 2.0 * (4.0 / np.sqrt(M * interior.sum()) + 5e-3), r_int
+    # pairs k, -k of a plane share one symmetrised value: half as many samples
+    assert abs(r_pl - 1.0) < 4.0 * np.sqrt(2.0 / (M * planes.sum())) + 5e-3 + 0.02, r_pl
+    # gamma = -1/4: the interior power is the vacuum 1/(2 omega)
+    kk = k[interior]
+    vac = n**6 * (2 * np.pi / L) ** 3 / ((2 * np.pi) ** 3 * 2.0 * np.sqrt(kk**2 + m2) * mpl**2)
+    vac *= np.exp(-2.0 * (kk / kcut) ** 2)
+    got = acc[interior] / M
+    assert abs(np.mean(got / vac) - 1.0) < 4.0 / np.sqrt(M * interior.sum()) + 5e-3
+
 
 def test_sample_statistics_and_determinism():
     n, L = 16, 10.0
