@@ -47,7 +47,9 @@ enum lat_energy_slot {
     LAT_E_PMOM,       /* -p^2/12/a^4 (conformal) or -3 p_y^2/16/a^2       */
     LAT_E_MEAN1,      /* <phi_1>                                          */
     LAT_E_MEAN2,      /* <phi_2>                                          */
-    LAT_E_GEGRAD,     /* FD gradient energy, GRAD2 form (stencil only)    */
+    LAT_E_GEGRAD,     /* FD gradient energy, GRAD2 form (grad_en,
+                         Discrete/Entropy/homlat.f90:88-115); stencil
+                         contexts only, NaN for the spectral Laplacian    */
     LAT_NE
 };
 
@@ -62,7 +64,9 @@ typedef struct lat_config {
     double stencil[5];   /* c0,c1,c2,c3,cc (hamiltonian_conformal.f90:41)          */
     int time_var;        /* LAT_CONFORMAL (hamiltonian_conformal.f90) or LAT_COSMIC */
     int model;           /* LAT_MODEL_*                                           */
-    double model_params[4]; /* phi4: lambda; twofield: m^2, g^2                  */
+    double model_params[4]; /* phi4: lambda; twofield: m^2, g^2, m_chi^2 (the
+                               chi mass of Discrete/Entropy/model.f90:35-55,
+                               modparam.f90:9 m2psi; 0 = the Pseudospec form) */
     int symp6_tree;      /* symp6 w1 constant: 0 openmp_dev, 1 mpi_dev            */
     int gw;              /* 1: evolve the metric perturbations u_ij with the fields
                             (grav-waves/data-gw.f90:14 hij, pi_hij)                  */
@@ -83,7 +87,9 @@ void lat_destroy(lat_ctx *ctx);
  * lat_symp_o2step on a batch advance every member with one launch per sweep
  * (grid y = member).  Per-member calls (state, IC, energies, spectra) go
  * through lat_member views, which borrow the batch's memory and must not
- * outlive it; lat_destroy on a view frees only the view.  Members of a
+ * outlive it; lat_destroy on a view frees the view and the scratch that its
+ * own calls allocated on demand (e.g. the k-space arrays of the output
+ * spectra of a mixed-form stencil batch, which holds none).  Members of a
  * batch share dt; the metric perturbations are not available in batches. */
 int  lat_create_batch(const lat_config *cfg, int nmembers, const double *model_params, lat_ctx **out);
 int  lat_num_members(const lat_ctx *ctx);

@@ -54,6 +54,7 @@ static int ensure(Ctx &c, void **p, size_t bytes, const char *what)
     if (*p) return 0;
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) return hip_fail(&c, e, what);
+    if (c.view) c.owned.push_back(*p);
     return 0;
 }
 
@@ -431,7 +432,7 @@ static double host_v(const Ctx &c, const double *f)
 {
     const double *mp = c.cfg.model_params;
     if (c.cfg.model == LAT_MODEL_PHI4) return 0.25 * mp[0] * (f[0] * f[0]) * (f[0] * f[0]);
-    return 0.5 * mp[0] * (f[0] * f[0]) + 0.5 * mp[1] * (f[0] * f[0]) * (f[1] * f[1]);
+    return 0.5 * mp[0] * (f[0] * f[0]) + 0.5 * mp[1] * (f[0] * f[0]) * (f[1] * f[1]) + 0.5 * mp[2] * (f[1] * f[1]);
 }
 
 // Lattice sums of the physical fields after state_sums(): sum pi^2, sum V,
@@ -654,11 +655,13 @@ static int create_ctx(const lat_config *cfg, int B, const double *mparams, lat_c
     c.host_sc.mixed = c.prec == LAT_MIXED ? 1 : 0;
     c.host_sc.mp[0] = cfg->model_params[0];
     c.host_sc.mp[1] = cfg->model_params[1];
+    c.host_sc.mp[2] = cfg->model_params[2];
     {
         std::vector<DevScalars> all(B, c.host_sc);
         for (int m = 0; m < B && mparams; ++m) {
             all[m].mp[0] = mparams[4 * m];
             all[m].mp[1] = mparams[4 * m + 1];
+            all[m].mp[2] = mparams[4 * m + 2];
         }
         c.host_sc = all[0];
         if (hipMemcpy(c.dsc, all.data(), sizeof(DevScalars) * B, hipMemcpyHostToDevice) != hipSuccess) {
@@ -733,6 +736,7 @@ int lat_member(lat_ctx *h, int m, lat_ctx **out)
     if ((rc = pull_scalars(w))) { delete v; return rc; }
     w.cfg.model_params[0] = w.host_sc.mp[0];
     w.cfg.model_params[1] = w.host_sc.mp[1];
+    w.cfg.model_params[2] = w.host_sc.mp[2];
     w.member = m;
     *out = v;
     return 0;
@@ -742,7 +746,12 @@ void lat_destroy(lat_ctx *h)
 {
     if (!h) return;
     Ctx &c = h->c;
-    if (c.view) { // borrows everything from its batch
+    if (c.view) { // borrows everything from its batch but its on-demand scratch
+        if (!c.owned.empty()) {
+            (void)hipSetDevice(c.device);
+            (void)hipStreamSynchronize(c.stream);
+            for (void *p : c.owned) (void)hipFree(p);
+        }
         delete h;
         return;
     }
@@ -1418,7 +1427,13 @@ int lat_energies(lat_ctx *h, double *out)
     out[LAT_E_PMOM] = conf ? -s.ysclp * s.ysclp / 12.0 / std::pow(a, 4) : -s.ysclp * s.ysclp * 3.0 / 16.0 / (a * a);
     out[LAT_E_MEAN1] = ps.mean[0];
     out[LAT_E_MEAN2] = c.nfld > 1 ? ps.mean[1] : 0.0;
-    out[LAT_E_GEGRAD] = 0.0;
+    // grad_en (Discrete/Entropy/homlat.f90:88-115): enorm = 1/(4 (a dx)^2 cc)
+    // times the lattice sum of STENCIL(c, GRAD2), over n^3; by summation by
+    // parts it equals GE (grad_en_wlap, :120-147).  The spectral path has no
+    // stencil form of the gradient energy: NaN.
+    out[LAT_E_GEGRAD] = c.cfg.laplacian == LAT_LAP_STENCIL
+                            ? 0.25 / ((a * c.dx) * (a * c.dx)) / c.cfg.stencil[4] * s.red[RC_X] / nv
+                            : std::nan("");
     return 0;
 }
 

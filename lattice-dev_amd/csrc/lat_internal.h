@@ -47,7 +47,7 @@ struct DevScalars {
     double nu[2];             // re-centering: the next drift sweep subtracts nu from dphi
     double mu[2];             // re-centering: the next kick subtracts mu from dpi
     double sdphi[2];          // sum of dphi after the pending drifts (before nu)
-    double mp[2];             // model parameters of this member (lat_config.model_params)
+    double mp[3];             // model parameters of this member (lat_config.model_params[0..2])
 };
 
 // Member strides of a batch context (B realisations of one lattice size,
@@ -79,6 +79,7 @@ struct ModelDev {
     int nfld;
     double p0, p1; // phi4: lambda; twofield: m2, g2
     double gwc;    // u_ij source coefficient 2 / M_p^2 (EPI_GWKICK only)
+    double p2;     // twofield: m_chi^2 (0 unless lat_config.model_params[2] sets it)
 };
 
 // Per-stage HIP-event accounting (lat_profile / lat_profile_read).
@@ -96,6 +97,10 @@ struct Ctx {
     int B = 1;              // members (realisations) of a batch context
     MemberStride ms{0, 0};
     bool view = false;      // a member view of a batch: borrows every buffer
+    // scratch that per-member operations allocate on demand while running on
+    // a view (e.g. the k-space arrays of the output spectra when the batch has
+    // none): owned by the view and freed by lat_destroy on it
+    std::vector<void *> owned;
     Ctx *parent = nullptr;  // the batch a view belongs to
     int member = 0;         // index of a view's member in its batch
     bool owns_stream = true; // false when an ensemble shares its stream

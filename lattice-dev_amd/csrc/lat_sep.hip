@@ -201,6 +201,7 @@ __global__ __launch_bounds__(NT) void k_sep_col(T *__restrict__ fld, T *__restri
     if constexpr (EPI == SEP_KICK) {
         md.p0 = dsc->mp[0];
         md.p1 = dsc->mp[1];
+        md.p2 = dsc->mp[2];
         hA = T(dsc->hA);
         hB = T(dsc->hB);
         if constexpr (sizeof(T) == 4) {

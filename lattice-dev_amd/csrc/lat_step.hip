@@ -171,6 +171,7 @@ __global__ __launch_bounds__(TPB) void k_reduce_state(const T *__restrict__ f, c
     partials += (size_t)blockIdx.y * MAX_GRID * NRED;
     md.p0 = dsc->mp[0];
     md.p1 = dsc->mp[1];
+    md.p2 = dsc->mp[2];
     double pb64[2] = {0, 0};
     if constexpr (sizeof(T) == 4) {
         pb64[0] = dsc->mean_phi[0];
@@ -243,6 +244,41 @@ __device__ __forceinline__ T stencil_sum(const T *__restrict__ f, const int xs[3
     return v;
 }
 
+// STENCIL(c, GRAD2) at one site (Discrete/Entropy/macros.h:21-38, summed in
+// grad_en, homlat.f90:88-115): sum over the ranks r of c_r times the squared
+// differences to the rank-r neighbours, each summed over the fields
+// (GRAD2 = sum over fld(:) of (f(nbr) - f(0))^2), in RANK1..3's neighbour
+// order; the rank-0 term vanishes.  FP64 differences (in the mixed form the
+// homogeneous parts cancel in every difference).
+template <typename T>
+__device__ __forceinline__ double grad2_site(const T *__restrict__ fld, size_t nvol, int nf, const int xs[3],
+                                             const int ys[3], const int zs[3], int nx, int ny, const StencilC &sc)
+{
+    auto d2 = [&](int a, int b, int cc_) {
+        double s = 0.0;
+        for (int l = 0; l < nf; ++l) {
+            const T *f = fld + (size_t)l * nvol;
+            const double d = double(f[(size_t)xs[a] + (size_t)nx * ((size_t)ys[b] + (size_t)ny * zs[cc_])]) -
+                             double(f[(size_t)xs[1] + (size_t)nx * ((size_t)ys[1] + (size_t)ny * zs[1])]);
+            s += d * d;
+        }
+        return s;
+    };
+    const double r1 = d2(0, 1, 1) + d2(2, 1, 1) + d2(1, 0, 1) + d2(1, 2, 1) + d2(1, 1, 0) + d2(1, 1, 2);
+    double v = sc.c1 * r1;
+    if (sc.c2 != 0.0) {
+        const double r2 = d2(0, 0, 1) + d2(2, 0, 1) + d2(0, 2, 1) + d2(2, 2, 1) + d2(0, 1, 0) + d2(2, 1, 0) +
+                          d2(0, 1, 2) + d2(2, 1, 2) + d2(1, 0, 0) + d2(1, 2, 0) + d2(1, 0, 2) + d2(1, 2, 2);
+        v += sc.c2 * r2;
+    }
+    if (sc.c3 != 0.0) {
+        const double r3 = d2(0, 0, 0) + d2(2, 0, 0) + d2(0, 2, 0) + d2(2, 2, 0) + d2(0, 0, 2) + d2(2, 0, 2) +
+                          d2(0, 2, 2) + d2(2, 2, 2);
+        v += sc.c3 * r3;
+    }
+    return v;
+}
+
 template <typename T, int EPI>
 __global__ __launch_bounds__(TPB) void k_stencil(const T *__restrict__ fld, T *__restrict__ fldp, T *__restrict__ out,
                                                  int nx, int ny, int nz, int nf, StencilC sc, ModelDev md,
@@ -255,6 +291,7 @@ __global__ __launch_bounds__(TPB) void k_stencil(const T *__restrict__ fld, T *_
     partials += (size_t)blockIdx.y * MAX_GRID * NRED;
     md.p0 = dsc->mp[0];
     md.p1 = dsc->mp[1];
+    md.p2 = dsc->mp[2];
     const size_t nvol = (size_t)nx * ny * nz;
     const T c0 = T(sc.c0), c1 = T(sc.c1), c2 = T(sc.c2), c3 = T(sc.c3), el = T(sc.elap);
     const bool has2 = sc.c2 != 0.0, has3 = sc.c3 != 0.0;
@@ -308,6 +345,7 @@ __global__ __launch_bounds__(TPB) void k_stencil(const T *__restrict__ fld, T *_
             }
             acc[RC_M1] += double(phi[0]);
             acc[RC_M2] += double(phi[1]);
+            if constexpr (EPI == EPI_REDUCE) acc[RC_X] += grad2_site(fld, nvol, nf, xs, ys, zs, nx, ny, sc);
         }
     }
     if constexpr (EPI != EPI_STORE) block_reduce_store2<NRED>(acc, partials + (size_t)blockIdx.x * NRED);
@@ -335,6 +373,7 @@ __global__ __launch_bounds__(TPB) void k_stencil7(const T *__restrict__ fld, T *
     partials += (size_t)blockIdx.y * MAX_GRID * NRED;
     md.p0 = dsc->mp[0];
     md.p1 = dsc->mp[1];
+    md.p2 = dsc->mp[2];
     const size_t nvol = (size_t)nx * ny * nz, plane = (size_t)nx * ny;
     const T c0 = T(sc.c0), c1 = T(sc.c1), el = T(sc.elap);
     T hA = 0, hB = 0;
@@ -420,6 +459,22 @@ __global__ __launch_bounds__(TPB) void k_stencil7(const T *__restrict__ fld, T *
             }
             acc[RC_M1] += double(phi[0]);
             acc[RC_M2] += double(phi[1]);
+            if constexpr (EPI == EPI_REDUCE) {
+                // STENCIL(c, GRAD2) with c2 = c3 = 0, neighbours in RANK1 order
+                // (x-, x+, y-, y+, z-, z+), each summed over the fields
+                auto d2 = [&](T v0, T v1) {
+                    const double d0 = double(v0) - double(cur[0]);
+                    double s2 = d0 * d0;
+                    if (nf > 1) {
+                        const double d1 = double(v1) - double(cur[1]);
+                        s2 += d1 * d1;
+                    }
+                    return s2;
+                };
+                const double r1 = d2(nb[0][0], nb[1][0]) + d2(nb[0][1], nb[1][1]) + d2(nb[0][2], nb[1][2]) +
+                                  d2(nb[0][3], nb[1][3]) + d2(prev[0], prev[1]) + d2(nxt[0], nxt[1]);
+                acc[RC_X] += sc.c1 * r1;
+            }
 #pragma unroll
             for (int l = 0; l < 2; ++l) {
                 prev[l] = cur[l];
@@ -571,6 +626,7 @@ __global__ __launch_bounds__(SCALAR_TPB) void k_scalar(DevScalars *__restrict__ 
     partials += (size_t)blockIdx.x * MAX_GRID * NRED;
     env.md.p0 = sc->mp[0];
     env.md.p1 = sc->mp[1];
+    env.md.p2 = sc->mp[2];
     constexpr int LANES = SCALAR_TPB / NRED;
     __shared__ double sh[NRED][LANES];
     const int ch = threadIdx.x / LANES, lane = threadIdx.x % LANES;

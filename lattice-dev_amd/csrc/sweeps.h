@@ -374,6 +374,7 @@ __device__ __forceinline__ void x_c2r_body(const cpx<T> *__restrict__ Fk, const 
     partials += (size_t)bk.y * MAX_GRID * NRED;
     md.p0 = dsc->mp[0];
     md.p1 = dsc->mp[1];
+    md.p2 = dsc->mp[2];
     double acc[NRED] = {0, 0, 0, 0, 0, 0, 0, 0}; // RedChan order
     T hA = 0, hB = 0;
     if constexpr (EPI == EPI_KICK || EPI == EPI_GWKICK) {

@@ -6,7 +6,10 @@
 ! spectrum_3d, crossspec_3d (Pseudospec/openmp_dev/fftw_mod.f90:177-207,
 ! analysis_spec.f90:48-178), Hamiltonian_Split, symp_o2step, calc_metric,
 ! get_scale_factor, get_hubble, kinetic_norm, grav_energy
-! (hamiltonian_conformal.f90:50-304) and symp2/4/6/8
+! (hamiltonian_conformal.f90:50-304), the mpi_dev accessors scalar_rho,
+! kinetic_energy, grad_energy, potential_energy
+! (Pseudospec/mpi_dev/hamiltonian_cosmic.f90:255-349), the Discrete-tree
+! grad_en / grad_en_wlap (Discrete/Entropy/homlat.f90:88-147) and symp2/4/6/8
 ! (Pseudospec/mpi_dev/integrator.f90:11-110), replacing `use fftw3`,
 ! `use analysis`, `use Hamiltonian`, `use integrator` by `use lattice_dropin`.
 !
@@ -50,11 +53,15 @@ module lattice_dropin
   type(c_ptr), save :: ctx = c_null_ptr
   integer, save :: gnx = 0, gny = 0, gnz = 0, gnf = 0, gtime = 0
   integer, save, public :: ns = 0
+  ! energies of the last scalar_rho() (hamiltonian_cosmic.f90:335-338 kecur,
+  ! pecur, gecur, rhocur)
+  real(dl), save :: kecur = 0._dl, pecur = 0._dl, gecur = 0._dl, rhocur = 0._dl
 
   public :: lattice_init, lattice_finalize, lattice_put_state, lattice_get_state
   public :: laplacian_3d, init_spectrum_3d, spectrum_3d, crossspec_3d
   public :: Hamiltonian_Split, symp_o2step, symp2, symp4, symp6, symp8, calc_metric
   public :: get_scale_factor, get_hubble, kinetic_norm, grav_energy, lattice_energies
+  public :: scalar_rho, kinetic_energy, grad_energy, potential_energy, grad_en, grad_en_wlap
   public :: spectrum, spectrum_bin, spectrum_nofft, spectrum_bin_nofft, crossspec_nofft
   public :: lattice_put_gw, lattice_get_gw, gw_spectrum
 
@@ -396,23 +403,30 @@ contains
     call check(c_lat_energies(ctx, e), "energies")
   end subroutine lattice_energies
 
-  ! hamiltonian_conformal.f90:242-261 / hamiltonian_cosmic.f90:234-253
+  ! hamiltonian_conformal.f90:242-261 / hamiltonian_cosmic.f90:234-253: scalar
+  ! reads of the background (yscl, ysclp), no lattice work
   real(dl) function get_scale_factor()
-    real(dl) :: e(LAT_NE)
-    call lattice_energies(e)
-    get_scale_factor = e(1)
+    real(dl) :: y, p
+    call check(c_lat_get_scalars(ctx, y, p), "get_scalars")
+    if (gtime == LAT_CONFORMAL) then
+       get_scale_factor = y
+    else
+       get_scale_factor = y**(2._dl / 3._dl)
+    end if
   end function get_scale_factor
 
   real(dl) function get_hubble()
-    real(dl) :: e(LAT_NE)
-    call lattice_energies(e)
-    get_hubble = e(2)
+    real(dl) :: y, p
+    call check(c_lat_get_scalars(ctx, y, p), "get_scalars")
+    if (gtime == LAT_CONFORMAL) then
+       get_hubble = -p / 6._dl / y**2
+    else
+       get_hubble = -p / y / 4._dl
+    end if
   end function get_hubble
 
   real(dl) function grav_energy()
-    real(dl) :: e(LAT_NE)
-    call lattice_energies(e)
-    grav_energy = e(7)
+    grav_energy = -3._dl * get_hubble()**2
   end function grav_energy
 
   real(dl) function kinetic_norm()
@@ -425,5 +439,43 @@ contains
        kinetic_norm = 1._dl / y**4
     end if
   end function kinetic_norm
+
+  ! mpi_dev/hamiltonian_cosmic.f90:268-349: rho = KE + PE + GE of the current
+  ! state (one lattice reduction with the Laplacian); keeps the components
+  ! for kinetic_energy / grad_energy / potential_energy (:255-266)
+  real(dl) function scalar_rho()
+    real(dl) :: e(LAT_NE)
+    call lattice_energies(e)
+    kecur = e(4); pecur = e(5); gecur = e(6); rhocur = e(3)
+    scalar_rho = e(3)
+  end function scalar_rho
+
+  real(dl) function kinetic_energy()
+    kinetic_energy = kecur
+  end function kinetic_energy
+
+  real(dl) function grad_energy()
+    grad_energy = gecur
+  end function grad_energy
+
+  real(dl) function potential_energy()
+    potential_energy = pecur
+  end function potential_energy
+
+  ! Discrete/Entropy/homlat.f90:88-115: gradient energy from STENCIL(c,GRAD2)
+  ! (finite-difference contexts; NaN for the spectral Laplacian)
+  real(dl) function grad_en()
+    real(dl) :: e(LAT_NE)
+    call lattice_energies(e)
+    grad_en = e(12)
+  end function grad_en
+
+  ! homlat.f90:120-147: gradient energy from -phi STENCIL(c,LAPLACE) (the GE
+  ! of dump_rho for the spectral Laplacian)
+  real(dl) function grad_en_wlap()
+    real(dl) :: e(LAT_NE)
+    call lattice_energies(e)
+    grad_en_wlap = e(6)
+  end function grad_en_wlap
 
 end module lattice_dropin

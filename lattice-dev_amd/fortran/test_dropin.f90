@@ -81,8 +81,39 @@ program test_dropin
   call lattice_energies(e)
   print *, "after symp4: a", get_scale_factor(), " H", get_hubble(), " resid", e(8)
   if (abs(e(8)) > 1e-9_dl) stop 1
+  ! O(1) background reads agree with the reduction-based diagnostics
+  if (abs(get_scale_factor() - e(1)) > 1e-15_dl * e(1)) stop 1
+  if (abs(get_hubble() - e(2)) > 1e-15_dl * abs(e(2))) stop 1
+  if (abs(grav_energy() - e(7)) > 1e-15_dl * abs(e(7))) stop 1
+  if (abs(kinetic_norm() - 1._dl / ysc**4) > 1e-15_dl / ysc**4) stop 1
+  ! mpi_dev dump_rho sequence (mpi_dev/evolve_spectral.f90:304-315)
+  block
+    real(dl) :: rho, ke, ge, pe
+    rho = scalar_rho()
+    ke = kinetic_energy(); ge = grad_energy(); pe = potential_energy()
+    print *, "scalar_rho", rho, " KE", ke, " GE", ge, " PE", pe, " resid", (rho + grav_energy()) / rho
+    if (abs(rho - e(3)) > 1e-14_dl * e(3)) stop 1
+    if (abs(ke + pe + ge - rho) > 1e-14_dl * rho) stop 1
+    if (abs(ke - e(4)) > 1e-14_dl * e(4) .or. abs(pe - e(5)) > 1e-14_dl * e(5)) stop 1
+    if (abs((rho + grav_energy()) / rho - e(8)) > 1e-12_dl) stop 1
+    if (grad_en() == grad_en()) stop 1   ! spectral context: NaN
+  end block
   if (abs(fld(1,0,5,5) - fld(1,n,5,5)) > 0._dl) stop 1   ! periodic ghost
   if (.not. (ysc > 1._dl)) stop 1
+
+  ! Discrete-tree gradient energies on a 7-point FD context: GRAD2 form ==
+  ! Laplacian form (summation by parts, homlat.f90:88-147)
+  call lattice_init(n, n, n, nfld, len, LAT_F64, LAT_LAP_STENCIL, LAT_COSMIC, LAT_MODEL_TWOFIELD, &
+                    (/ 1._dl, 2.5e5_dl /))
+  do k = 1, n; do j = 1, n; do i = 1, n
+     fld(1,i,j,k) = 1._dl + 0.01_dl*f(i,j,k)
+     fld(2,i,j,k) = 0.01_dl*f(j,k,i)
+  end do; end do; end do
+  fldp = 0._dl
+  call lattice_put_state(fld, fldp, 1, 1._dl, 0._dl)
+  print *, "grad_en", grad_en(), " grad_en_wlap", grad_en_wlap()
+  if (.not. (grad_en() > 0._dl)) stop 1
+  if (abs(grad_en() - grad_en_wlap()) > 1e-12_dl * grad_en_wlap()) stop 1
 
   ! metric perturbations sourced by a two-field configuration with gradients
   call lattice_init(n, n, n, nfld, len, LAT_F64, LAT_LAP_SPECTRAL, LAT_COSMIC, LAT_MODEL_TWOFIELD, &

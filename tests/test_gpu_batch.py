@@ -191,14 +191,18 @@ def test_ensemble_single_launch_matches_per_class_launches(gpu, precision):
 
 @pytest.mark.parametrize("n", [32, 64, 128, 192])
 @pytest.mark.parametrize("precision", ["f64", "mixed"])
-def test_tuned_launch_shapes_keep_results(gpu, n, precision):
-    """The alternative launch shapes (fewer rows per x workgroup, fewer
-    columns per y/z tile) compute the same transforms; only the order of the
-    block partial sums changes."""
+def test_tuned_launch_shapes_match_oracle(gpu, n, precision):
+    """Both launch shapes of every sweep (default, and fewer rows per x
+    workgroup / fewer columns per y, z tile) against the oracle: FP64 to
+    1e-12; the mixed form to its FP32 level on the fluctuations."""
     print("case tuning", n, precision, flush=True)
     L, dt, nsteps = 10.0, 2e-4, 2
     fld, fldp = member_ic(n, L, 55)
     prec = gpu.F64 if precision == "f64" else gpu.MIXED
+    sim = O.Simulation(n, 2, L, O.COSMIC, model=O.MODEL_TWOFIELD, mp=(1.0, 2.5e5))
+    sim.set_state(fld, fldp, 1.0, 0.0)
+    sim.calc_metric()
+    sim.symp(4, dt, nsteps)
     out = []
     for shape in ((0, 0, 0, 0), (1, 1, 1, 1)):
         with gpu.Lattice(n, nfld=2, length=L, precision=prec) as lat:
@@ -208,10 +212,84 @@ def test_tuned_launch_shapes_keep_results(gpu, n, precision):
             lat.calc_metric()
             lat.symp4(dt, nsteps)
             out.append(lat.get_state())
-    tol = 1e-13 if precision == "f64" else 1e-6
-    assert rel(out[1][0], out[0][0]) < tol
-    assert rel(out[1][1], out[0][1]) < tol
-    assert abs(out[1][2] - out[0][2]) < 1e-13
+    for st in out:
+        if precision == "f64":
+            assert rel(st[0], sim.fld) < 1e-12 and rel(st[1], sim.fldp) < 1e-12
+        else:
+            for l in range(2):
+                for a, b in ((st[0][l], sim.fld[l]), (st[1][l], sim.fldp[l])):
+                    assert rel(a - a.mean(), b - b.mean()) < 1e-5, l
+        assert abs(st[2] - sim.yscl) < 1e-13
     with gpu.Lattice(n, nfld=2, length=L, precision=prec) as lat:
         t = lat.autotune()
         assert set(t.values()) <= {0, 1}
+
+
+def test_ensemble_96_members_match_oracle(gpu):
+    """The 96-realisation ensemble (32 x 64^3 + 48 x 128^3 + 16 x 192^3, fixed
+    L = 10, g^2/m^2 log-uniform in [1e4, 1e6]), autotuned launch mode and
+    shapes, graph replay: one member of each class against the oracle from
+    the member's own initial state."""
+    print("case ensemble 96", flush=True)
+    classes = [(64, 32, 10.0), (128, 48, 10.0), (192, 16, 10.0)]
+    rng = np.random.default_rng(11)
+    g2s = np.exp(rng.uniform(np.log(1e4), np.log(1e6), 96))
+    params = [(1.0, float(g)) for g in g2s]
+    dts = [1e-3, 1e-3, 1e-3]  # omega dt <= 1 for g^2 up to 1e6
+    picks = [5, 32 + 17, 80 + 9]
+    with gpu.Ensemble(classes, members=params) as ens:
+        views = [ens.member(i) for i in range(ens.size)]
+        ics = {}
+        for i, v in enumerate(views):
+            g2 = float(g2s[i])
+            v.sample_field(0, 0, PHI0, -0.25, 1.0, 2e5, 3000 + i, 0)
+            v.sample_field(0, 1, DPHI0, 0.25, 1.0, 2e5, 3000 + i, 1)
+            v.sample_field(1, 0, 0.0, -0.25, g2 * PHI0 ** 2, 2e5, 3000 + i, 2)
+            v.sample_field(1, 1, 0.0, 0.25, g2 * PHI0 ** 2, 2e5, 3000 + i, 3)
+            v.set_scalars(1.0, 0.0)
+            v.calc_metric()
+            if i in picks:
+                ics[i] = v.get_state()
+        for _ in range(2):  # capture, then replay
+            ens.symp(4, dts, 5)
+        ens.synchronize()
+        got = {i: views[i].get_state() for i in picks}
+        for v in views:
+            v.close()
+    for i in picks:
+        n = 64 if i < 32 else (128 if i < 80 else 192)
+        sim = O.Simulation(n, 2, 10.0, O.COSMIC, model=O.MODEL_TWOFIELD, mp=params[i])
+        f, p, y, py = ics[i]
+        sim.set_state(f, p, y, py)
+        for _ in range(2):
+            sim.symp(4, 1e-3, 5)
+        print("ensemble member", i, n, rel(got[i][0], sim.fld), rel(got[i][1], sim.fldp), flush=True)
+        assert rel(got[i][0], sim.fld) < 1e-12, i
+        assert rel(got[i][1], sim.fldp) < 1e-12, i
+        assert abs(got[i][2] - sim.yscl) < 1e-13 * abs(sim.yscl), i
+        assert abs(got[i][3] - sim.ysclp) < 1e-11 * abs(sim.ysclp), i
+
+
+def test_member_views_free_their_scratch(gpu):
+    """A mixed-form 7-point batch holds no k-space arrays; a member view's
+    output spectra allocate them on demand, and destroying the view frees
+    them: device memory returns to its level before the views."""
+    import torch
+
+    print("case view scratch", flush=True)
+    n = 128
+    with gpu.Lattice(n, nfld=2, length=10.0, precision=gpu.MIXED, laplacian=gpu.LAP_STENCIL,
+                     members=[(1.0, 2.5e5)] * 2) as batch:
+        v = batch.member(0)
+        v.output_spectra(0)  # first use: the batch's shared spectrum buffers
+        v.close()
+        torch.cuda.synchronize()
+        free0 = torch.cuda.mem_get_info()[0]
+        for _ in range(6):
+            v = batch.member(1)
+            v.output_spectra(1)
+            v.close()
+        torch.cuda.synchronize()
+        free1 = torch.cuda.mem_get_info()[0]
+    nk_bytes = 2 * 8 * (n // 2 + 8) * n * n  # one view's two complex FP32 arrays
+    assert free0 - free1 < nk_bytes, (free0, free1, nk_bytes)

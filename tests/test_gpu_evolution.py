@@ -68,6 +68,9 @@ CASES = {
                                 lap=O.LAP_STENCIL, stencil=(-6.0, 1.0, 0.0, 0.0, 1.0)),
     "conformal_phi4_fd27": dict(nfld=1, time_var=O.CONFORMAL, model=O.MODEL_PHI4, mp=(1.0,),
                                 lap=O.LAP_STENCIL, stencil=(-128.0, 14.0, 3.0, 1.0, 30.0)),
+    # the chi mass of the Discrete model (model.f90:35-55), m2psi = 3e3
+    "cosmic_twofield_mchi_fd7": dict(nfld=2, time_var=O.COSMIC, model=O.MODEL_TWOFIELD, mp=(1.0, 2.5e5, 3.0e3),
+                                     lap=O.LAP_STENCIL, stencil=(-6.0, 1.0, 0.0, 0.0, 1.0)),
 }
 
 

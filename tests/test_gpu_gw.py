@@ -198,3 +198,36 @@ def test_gw_mixed_form_tracks_f64(gpu):
     s_m, s_d = out["mixed"][2], out["f64"][2]
     m = s_d > 1e-6 * s_d.max()
     assert np.max(np.abs(s_m[m] - s_d[m]) / s_d[m]) < 1e-4
+
+
+def test_gw_mixed_form_tracks_f64_at_128(gpu):
+    """The headline physics with the six u_ij at 128^3 (u = pi_u = 0 at the
+    start, sourced by the vacuum fluctuations of both fields), 100 Yoshida-4
+    steps of dt = dx/40 in both forms from identical device-sampled initial
+    conditions: GW spectra of pi_u and u (bins >= 1e-6 of the peak) and the
+    u_ij fields of the mixed form against FP64."""
+    print("case gw mixed 128", flush=True)
+    n, L, g2 = 128, 10.0, 2.5e5
+    phi0, dphi0 = 1.0093430384226378929425913902459, -0.7137133070120812430962278466136
+    dt = (L / n) / 40.0
+    out = {}
+    for name, prec in (("f64", gpu.F64), ("mixed", gpu.MIXED)):
+        with gpu.Lattice(n, nfld=2, length=L, precision=prec, time_var=gpu.COSMIC, model=gpu.MODEL_TWOFIELD,
+                         model_params=(1.0, g2), gw=True, gw_mpl=1.0) as lat:
+            lat.sample_field(0, 0, phi0, -0.25, 1.0, 2e5, 8, 0)
+            lat.sample_field(0, 1, dphi0, 0.25, 1.0, 2e5, 8, 1)
+            lat.sample_field(1, 0, 0.0, -0.25, g2 * phi0 ** 2, 2e5, 8, 2)
+            lat.sample_field(1, 1, 0.0, 0.25, g2 * phi0 ** 2, 2e5, 8, 3)
+            lat.set_scalars(1.0, 0.0)
+            lat.calc_metric()
+            lat.symp4(dt, 100)
+            out[name] = lat.get_gw() + (lat.gw_spectrum(),)
+    errs = []
+    for ch in (0, 1):
+        s_m, s_d = out["mixed"][2][ch][1:], out["f64"][2][ch][1:]
+        m = s_d > 1e-6 * s_d.max()
+        errs.append(float(np.max(np.abs(s_m[m] - s_d[m]) / s_d[m])))
+    fu = [rel(out["mixed"][k], out["f64"][k]) for k in (0, 1)]
+    print("gw mixed 128: spectra (pi_u, u)", errs, "u, pi_u fields", fu, flush=True)
+    assert max(errs) < 1e-4
+    assert max(fu) < 1e-4

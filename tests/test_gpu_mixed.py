@@ -74,6 +74,92 @@ def test_mixed_tracks_fp64(gpu, lap):
         assert np.linalg.norm(df) <= 1e-4 * np.linalg.norm(ref)
 
 
+def test_mixed_bin0_of_zero_mean_field(gpu):
+    """Bin 0 of chi, whose mean is ~0: S(0) = F(0)^2 with F(0) = N^3 <chi>
+    taken from the FP32 transform's k = 0 coefficient (plus the FP64 mean).
+    Its rounding is that of an FP32 sum of N^3 fluctuations, so
+    |sqrt S_mixed(0) - sqrt S_f64(0)| <= C eps32 sqrt(N^3) rms(chi) with
+    C = 4 sqrt(log2 N^3) (a random walk over the transform's passes), checked
+    on the identical initial state of both forms for S_chi and S_pi_chi."""
+    print("case mixed bin0 chi", flush=True)
+    n = 32
+    eps32 = 2.0 ** -24
+    out = {}
+    for prec in (gpu.F64, gpu.MIXED):
+        lat = gpu.Lattice(n, nfld=2, length=10.0, precision=prec, time_var=gpu.COSMIC,
+                          model=gpu.MODEL_TWOFIELD, model_params=(1.0, 2.5e5))
+        lat.sample_field(0, 0, PHI0, -0.25, 1.0, 2e5, 11, 0)
+        lat.sample_field(0, 1, DPHI0, 0.25, 1.0, 2e5, 11, 1)
+        lat.sample_field(1, 0, 0.0, -0.25, 2.5e5 * PHI0 ** 2, 2e5, 11, 2)
+        lat.sample_field(1, 1, 0.0, 0.25, 2.5e5 * PHI0 ** 2, 2e5, 11, 3)
+        out[prec] = (lat.output_spectra(1), lat.get_state())
+        lat.close()
+    (sd, st), (sm, _) = out[gpu.F64], out[gpu.MIXED]
+    c = 4.0 * np.sqrt(np.log2(n ** 3))
+    for q, arr in ((0, st[0][1]), (1, st[1][1])):
+        bound = c * eps32 * np.sqrt(n ** 3) * np.sqrt(np.mean(arr ** 2))
+        d = abs(np.sqrt(sm[q][0]) - np.sqrt(sd[q][0]))
+        print("chi bin0", q, np.sqrt(sd[q][0]), d, bound, flush=True)
+        assert d <= bound, (q, d, bound)
+
+
+def test_mixed_tracks_fp64_256_1000_steps(gpu):
+    """Config-1 physics at 256^3 over 1000 Yoshida-4 steps (dt = dx/40), both
+    forms from identical device-sampled initial conditions, spectra every
+    100 steps.  Stated: fluctuation fields, spectra (bins >= 1e-3 and >= 1e-6
+    of the peak, and all bins), energy and Friedmann residual of the mixed
+    form against the FP64 form."""
+    print("case mixed 256 x 1000", flush=True)
+    n, steps, chunks = 256, 1000, 10
+    dt = (10.0 / n) / 40.0
+    e0d, e1d, sd, std_ = run(gpu, gpu.F64, n, gpu.LAP_SPECTRAL, steps, dt, chunks, seed=21)
+    e0m, e1m, sm, stm = run(gpu, gpu.MIXED, n, gpu.LAP_SPECTRAL, steps, dt, chunks, seed=21)
+    fl = [float(np.linalg.norm(stm[0][l] - std_[0][l]) / np.linalg.norm(std_[0][l] - std_[0][l].mean()))
+          for l in range(2)]
+    flp = [float(np.linalg.norm(stm[1][l] - std_[1][l]) / np.linalg.norm(std_[1][l] - std_[1][l].mean()))
+           for l in range(2)]
+    err3, err6, errall = spec_err(sm, sd, 1e-3), spec_err(sm, sd, 1e-6), spec_err(sm, sd, 0.0)
+    erho = abs(e1m["rho"] - e1d["rho"]) / abs(e1d["rho"])
+    dres = abs(e1m["resid"] - e1d["resid"])
+    print("mixed 256^3 x 1000:", {"fluct_fld": fl, "fluct_fldp": flp, "spec_ge_1e-3": err3, "spec_ge_1e-6": err6,
+                                  "spec_all": errall, "rho": erho, "resid_mixed": e1m["resid"],
+                                  "resid_f64": e1d["resid"]}, flush=True)
+    assert max(fl) < 1e-4 and max(flp) < 1e-4
+    assert err3 < 1e-5
+    assert err6 < 1e-4
+    assert erho < 1e-8
+    assert dres < 1e-7
+
+
+def test_mixed_with_chi_mass_tracks_oracle(gpu):
+    """The chi mass term of the Discrete model (V += m2chi chi^2 / 2) in the
+    mixed form's difference forms of V and V' (fluctuations about the FP64
+    homogeneous parts) and in the FP64 form, against the oracle."""
+    import oracle as O
+
+    print("case mixed chi mass", flush=True)
+    n, L, dt, nsteps = 32, 10.0, 1e-3, 20
+    mp = (1.0, 2.5e5, 4.0e3)
+    rng = np.random.default_rng(9)
+    fld = np.stack([PHI0 + 1e-5 * rng.standard_normal((n, n, n)), 2e-3 + 1e-5 * rng.standard_normal((n, n, n))])
+    fldp = np.stack([DPHI0 + 1e-4 * rng.standard_normal((n, n, n)), 1e-4 * rng.standard_normal((n, n, n))])
+    sim = O.Simulation(n, 2, L, O.COSMIC, model=O.MODEL_TWOFIELD, mp=mp)
+    sim.set_state(fld, fldp, 1.0, 0.0)
+    sim.calc_metric()
+    sim.symp(4, dt, nsteps)
+    for prec, tol in ((gpu.F64, 1e-12), (gpu.MIXED, 1e-5)):
+        with gpu.Lattice(n, nfld=2, length=L, precision=prec, model_params=mp) as lat:
+            lat.set_state(fld, fldp, 1.0, 0.0)
+            lat.calc_metric()
+            lat.symp4(dt, nsteps)
+            f, p, y, py = lat.get_state()
+        for l in range(2):
+            for a, b in ((f[l], sim.fld[l]), (p[l], sim.fldp[l])):
+                e = np.linalg.norm(a - b) / np.linalg.norm(b - b.mean())
+                assert e < tol, (prec, l, e)
+        assert abs(y - sim.yscl) < 1e-13
+
+
 def test_mixed_state_roundtrip(gpu):
     n = 16
     rng = np.random.default_rng(3)

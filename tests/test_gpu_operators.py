@@ -7,8 +7,12 @@ import oracle as O
 
 pytestmark = pytest.mark.gpu
 
+# every line length of the kernels' size list along every axis: x rows (R2C /
+# C2R) and y / z columns (the column passes, the fused z pass of the
+# Laplacian) of 96, 128, 192 and 256, the config-1 and ensemble sizes
 SIZES = [(8, 8, 8), (16, 16, 16), (32, 32, 32), (64, 64, 64), (16, 32, 8), (48, 48, 48), (96, 32, 16),
-         (192, 16, 8), (128, 16, 16), (256, 8, 8), (1024, 8, 8), (8, 512, 8), (8, 1024, 8), (8, 8, 1024)]
+         (192, 16, 8), (128, 16, 16), (256, 8, 8), (1024, 8, 8), (8, 512, 8), (8, 1024, 8), (8, 8, 1024),
+         (16, 96, 128), (8, 256, 192), (16, 128, 96), (8, 192, 256), (32, 256, 256)]
 
 
 def _lat(la, shape, **kw):
@@ -133,35 +137,76 @@ def test_discrete_nofft_spectra_match_oracle(gpu, n):
 
 def test_spectra_weight_cache_across_bin_sets(gpu):
     """The binning weights W(b) are binned once per (bin count, weighting
-    flags) and cached in four slots per lattice.  Six spectrum calls with
-    five or six distinct (bin count, flags) sets, run twice over on one
-    lattice (misses, hits and evictions), give bitwise what each call gives
-    on a fresh lattice (where the weights are binned for that call)."""
+    flags) and cached in four first-in-first-out slots per lattice.  At n = 16
+    the calls below carry five keys (spectrum_3d and discrete kind 0 share
+    ns = 15 with flags 0): A = (15, 0), B = counts, C = half space,
+    D = counts + half + no x-Nyquist, E = half + no x-Nyquist.  The order
+    A B C A D B E A C D A visits: A, B, C miss (slots 0-2); A hits slot 0
+    after two other slots were written; D misses (slot 3); B hits slot 1
+    after two more writes; E misses and evicts A (slot 0); A misses and
+    evicts B (slot 1); C hits slot 2 after three writes; D hits slot 3; A
+    hits slot 1.  Every call must give bitwise what it gives on a fresh
+    lattice (where its weights are binned for that call)."""
     print("case spectra weight cache", flush=True)
     n = 16
     rng = np.random.default_rng(77)
     f = rng.standard_normal((n, n, n))
     F1 = np.fft.rfftn(rng.standard_normal((n, n, n)))
     F2 = np.fft.rfftn(rng.standard_normal((n, n, n)))
-    calls = [
-        lambda lat: lat.spectrum_3d(f),
-        lambda lat: lat.spectrum_discrete(f, 0),
-        lambda lat: lat.spectrum_discrete(f, 1),
-        lambda lat: lat.spectrum_discrete_k(F1, 0),
-        lambda lat: lat.spectrum_discrete_k(F1, 1),
-        lambda lat: np.concatenate(lat.crossspec_discrete_k(F1, F2)),
-    ]
-    fresh = []
-    for call in calls:
+    calls = {
+        "A": lambda lat: lat.spectrum_3d(f),
+        "A'": lambda lat: lat.spectrum_discrete(f, 0),
+        "B": lambda lat: lat.spectrum_discrete(f, 1),
+        "C": lambda lat: lat.spectrum_discrete_k(F1, 0),
+        "D": lambda lat: lat.spectrum_discrete_k(F1, 1),
+        "E": lambda lat: np.concatenate(lat.crossspec_discrete_k(F1, F2)),
+    }
+    fresh = {}
+    for key, call in calls.items():
         with _lat(gpu, (n, n, n), nfld=1, model=gpu.MODEL_PHI4, model_params=(1.0,)) as lat:
-            fresh.append(call(lat))
+            fresh[key] = call(lat)
+    order = ["A", "B", "C", "A'", "D", "B", "E", "A", "C", "D", "A'"]
     with _lat(gpu, (n, n, n), nfld=1, model=gpu.MODEL_PHI4, model_params=(1.0,)) as lat:
-        for rep in range(2):
-            for q, call in enumerate(calls):
-                got = call(lat)
-                assert np.array_equal(got, fresh[q]), (rep, q)
+        for q, key in enumerate(order):
+            got = calls[key](lat)
+            assert np.array_equal(got, fresh[key]), (q, key)
     # the fresh results themselves are pinned to the oracle by the tests above
-    assert np.allclose(fresh[1], O.spectrum_discrete(O.r2c_3d(f), n, 0), rtol=1e-12, atol=0)
+    assert np.allclose(fresh["A'"], O.spectrum_discrete(O.r2c_3d(f), n, 0), rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("precision", ["f64", "mixed"])
+def test_spectra_weight_cache_shared_by_batch_views(gpu, precision):
+    """Member views use their batch's cache: view 0's output spectra bin the
+    weights (miss), view 1's reuse them (hit); both equal single-context
+    output spectra of the same state bitwise, and a later spectrum with other
+    flags on view 1 does not disturb view 0's next call."""
+    print("case spectra cache views", precision, flush=True)
+    n, L = 16, 10.0
+    prec = gpu.F64 if precision == "f64" else gpu.MIXED
+    rng = np.random.default_rng(5)
+    ics = []
+    for m in range(2):
+        fld = np.stack([1.0 + 1e-3 * rng.standard_normal((n, n, n)), 1e-3 * rng.standard_normal((n, n, n))])
+        fldp = np.stack([-0.7 + 1e-3 * rng.standard_normal((n, n, n)), 1e-3 * rng.standard_normal((n, n, n))])
+        ics.append((fld, fldp))
+    single = []
+    for m in range(2):
+        with gpu.Lattice(n, nfld=2, length=L, precision=prec) as lat:
+            lat.set_state(*ics[m], 1.0, 0.0)
+            single.append([lat.output_spectra(0), lat.output_spectra(1)])
+    with gpu.Lattice(n, nfld=2, length=L, precision=prec, members=[(1.0, 2.5e5)] * 2) as batch:
+        views = [batch.member(m) for m in range(2)]
+        for m, v in enumerate(views):
+            v.set_state(*ics[m], 1.0, 0.0)
+        got0 = [views[0].output_spectra(0), views[0].output_spectra(1)]
+        got1 = [views[1].output_spectra(0), views[1].output_spectra(1)]
+        views[1].spectrum_discrete(ics[1][0][0], 1)
+        again0 = views[0].output_spectra(1)
+        for v in views:
+            v.close()
+    for l in range(2):
+        assert np.array_equal(got0[l], single[0][l]) and np.array_equal(got1[l], single[1][l]), l
+    assert np.array_equal(again0, single[0][1])
 
 
 def test_discrete_spectra_need_cubic_lattice(gpu):
