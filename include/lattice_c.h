@@ -204,6 +204,18 @@ int  lat_spectrum_discrete(lat_ctx *ctx, const double *f, double *S, int kind);
  *          x-Nyquist plane skipped (:69).  The reference bins the module
  *          array Fk there, not its argument famp; pass the array it holds. */
 int  lat_spectrum_discrete_k(lat_ctx *ctx, const double *Fk, double *S, int kind);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  lat_spectrum_discrete_k's spectrum (kind 0 or 1, same
+ * arithmetic, same order of additions) of a DEVICE-resident transform Fk_dev
+ * (FFTW layout (n/2+1, n, n), rows unpadded) into the DEVICE array S_dev
+ * (ns_d doubles), enqueued on the context's stream with no host
+ * synchronisation and no allocation: the block histograms and channels live
+ * in a library-static device array of 32 MiB per GPU, enough for every
+ * supported lattice up to 1024^3 (larger needs are rejected with an error).
+ * Every call on a GPU shares that array: calls that overlap on one GPU
+ * (several contexts' streams, several host threads) race. */
+int  lat_spectrum_discrete_k_noalloc(lat_ctx *ctx, const double *Fk_dev, double *S_dev, int kind);
 /* Replaces crossspec_nofft(famp1, famp2, S1, S2), analysis.f90:149-186:
  * Re and Im of F1 conj(F2), two-bin weights, x-Nyquist plane skipped, S / W. */
 int  lat_crossspec_discrete_k(lat_ctx *ctx, const double *Fk1, const double *Fk2, double *S1, double *S2);

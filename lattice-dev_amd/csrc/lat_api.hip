@@ -677,6 +677,7 @@ static int create_ctx(const lat_config *cfg, int B, const double *mparams, lat_c
         (void)hipMemset(c.gwp, 0, es * 6 * c.nvol);
     }
     set_ke_valid(c, false);
+    if ((rc = noalloc_setup(c))) { lat_destroy(h); return rc; }
     *out = h;
     return 0;
 }
@@ -977,6 +978,18 @@ int lat_spectrum_discrete_k(lat_ctx *h, const double *Fk, double *S, int kind)
     const double n3 = (double)c.nvol;
     finish_spectra(raw, ns, 1, outs, kind == 0 ? n3 * n3 : 1.0); // analysis.f90:146 / :82
     return 0;
+}
+
+int lat_spectrum_discrete_k_noalloc(lat_ctx *h, const double *Fk_dev, double *S_dev, int kind)
+{
+    CTX_OR_FAIL(h);
+    ONE_MEMBER(c);
+    if (!S_dev || !Fk_dev || (kind != 0 && kind != 1)) return -1;
+    if (c.nx != c.ny || c.nx != c.nz) {
+        c.last_error = "Discrete-tree spectra need a cubic lattice";
+        return -2;
+    }
+    return spectrum_discrete_k_noalloc(c, Fk_dev, S_dev, kind);
 }
 
 int lat_crossspec_discrete_k(lat_ctx *h, const double *Fk1, const double *Fk2, double *S1, double *S2)

@@ -226,6 +226,10 @@ enum { BIN_COUNTS = 1, BIN_HALF = 2, BIN_NO_NYQ = 4 }; // launch_bin flags
 constexpr int BIN_BLOCKS = 4096, BIN_GROUP = 64;
 template <typename T>
 int launch_bin(Ctx &c, const cpx<T> *F1, const cpx<T> *F2, int mode, int nbins = 0, int flags = 0);
+// allocation-free Discrete-tree nofft spectrum (lat_spectrum_discrete_k_noalloc):
+// per-device scratch lookup at context creation, then the launch
+int noalloc_setup(Ctx &c);
+int spectrum_discrete_k_noalloc(Ctx &c, const double *Fk_dev, double *S_dev, int kind);
 
 // ---- initial conditions (lat_ic.hip) ----
 int sample_to_io(Ctx &c, double gamma, double m2eff, double mpl, double kcut_fac, uint64_t seed, uint64_t stream);

@@ -14,6 +14,9 @@
 // update the histogram, one wave after the other in wave order (no atomics).
 // Workgroup histograms are reduced in block order by k_bin_reduce (groups of
 // 64 blocks, then the groups).  Results are bitwise reproducible run to run.
+#include <mutex>
+#include <string>
+
 #include "lat_internal.h"
 
 namespace lat {
@@ -350,6 +353,97 @@ int launch_bin(Ctx &c, const cpx<T> *F1, const cpx<T> *F2, int mode, int nbins, 
     LAT_HIP(&c, hipMemcpyAsync(c.spec_out + (size_t)nch * ns, c.spec_out + (5 + slot) * cap, sizeof(double) * ns,
                                hipMemcpyDeviceToDevice, c.stream));
     return nch;
+}
+
+// ---------------------------------------------------------------- allocation-free spectra
+// The Discrete-tree nofft spectra of a device-resident transform with every
+// piece of scratch (block histograms, group sums, weight and value channels)
+// in one file-scope device array.  Host code never passes the array's name
+// to a kernel: its device address is looked up once per device at context
+// creation (noalloc_setup) and kept in noalloc_ptr.  Every call on a device
+// rewrites the whole of the part it uses (each block stores its full
+// histogram, each reduction stage its full output), so nothing has to start
+// at zero; calls that overlap on one device race on the array.
+constexpr size_t NOALLOC_DOUBLES = (size_t)4 << 20; // 32 MiB
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+constexpr int NOALLOC_MAXDEV = 64;
+static double *noalloc_ptr[NOALLOC_MAXDEV];
+static std::mutex noalloc_mu;
+
+int noalloc_setup(Ctx &c)
+{
+    if (c.device < 0 || c.device >= NOALLOC_MAXDEV) {
+        c.last_error = "device index beyond the allocation-free scratch table";
+        return -2;
+    }
+    std::lock_guard<std::mutex> lk(noalloc_mu);
+    if (noalloc_ptr[c.device]) return 0;
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(&c, e, "hipGetSymbolAddress(scratch)");
+    noalloc_ptr[c.device] = (double *)ptr;
+    return 0;
+}
+
+// S[b] = raw[b] / W[b] / div where W[b] != 0, else raw[b]: finish_spectra's
+// arithmetic on the device
+__global__ __launch_bounds__(256) void k_spec_finish(const double *__restrict__ raw, const double *__restrict__ W,
+                                                     int ns, double div, double *__restrict__ S)
+{
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= ns) return;
+    double s = raw[b];
+    if (W[b] != 0.0) s = s / W[b] / div;
+    S[b] = s;
+}
+
+int spectrum_discrete_k_noalloc(Ctx &c, const double *Fk_dev, double *S_dev, int kind)
+{
+    if (c.device < 0 || c.device >= NOALLOC_MAXDEV || !noalloc_ptr[c.device]) {
+        c.last_error = "allocation-free scratch not set up on this device";
+        return -2;
+    }
+    const int ns = lat_num_bins_discrete(c.nx);
+    const int rows = c.ny * c.nz;
+    const int gr = (rows + BIN_WAVES - 1) / BIN_WAVES;
+    const int g = gr < BIN_BLOCKS ? gr : BIN_BLOCKS;
+    const int ngrp = (g + BIN_GROUP - 1) / BIN_GROUP;
+    const size_t need = ((size_t)g + ngrp + 2) * ns;
+    if (need > NOALLOC_DOUBLES) {
+        c.last_error = "lattice too large for the allocation-free scratch (" + std::to_string(need * 8) +
+                       " bytes needed, " + std::to_string(NOALLOC_DOUBLES * 8) + " held)";
+        return -2;
+    }
+    if (c.nxh > BIN_ROWCHUNKS * 64) {
+        c.last_error = "spectrum rows longer than the binning kernel's row buffer";
+        return -2;
+    }
+    double *part = noalloc_ptr[c.device];
+    double *grp = part + (size_t)g * ns;
+    double *W = grp + (size_t)ngrp * ns;
+    double *raw = W + ns;
+    const int flags = kind == 0 ? BIN_HALF : (BIN_HALF | BIN_NO_NYQ | BIN_COUNTS);
+    const cpx<double> *F = (const cpx<double> *)Fk_dev;
+    const int P = c.nxh; // FFTW layout: rows of nx/2 + 1, unpadded
+    auto reduce = [&](double *out) {
+        hipLaunchKernelGGL(k_bin_reduce, dim3((ns + 255) / 256, ngrp), dim3(256), 0, c.stream, part, g, BIN_GROUP,
+                           ns, grp);
+        hipLaunchKernelGGL(k_bin_reduce, dim3((ns + 255) / 256, 1), dim3(256), 0, c.stream, grp, ngrp, ngrp, ns,
+                           out);
+    };
+    // the weight channel, then the value channel, with the kernels, grid and
+    // reduction order of launch_bin (same additions in the same order)
+    hipLaunchKernelGGL((k_bin<double, 0, 4, true>), dim3(g), dim3(64 * BIN_WAVES), sizeof(double) * ns, c.stream, F, F,
+                       c.nx, c.ny, c.nz, c.nxh, P, ns, part, flags, c.nk);
+    reduce(W);
+    hipLaunchKernelGGL((k_bin<double, 1, 0, false>), dim3(g), dim3(64 * BIN_WAVES), sizeof(double) * ns, c.stream, F,
+                       F, c.nx, c.ny, c.nz, c.nxh, P, ns, part, flags, c.nk);
+    reduce(raw);
+    const double n3 = (double)c.nvol;
+    hipLaunchKernelGGL(k_spec_finish, dim3((ns + 255) / 256), dim3(256), 0, c.stream, raw, W, ns,
+                       kind == 0 ? n3 * n3 : 1.0, S_dev);
+    LAT_HIP(&c, hipGetLastError());
+    return 0;
 }
 
 template int launch_bin<double>(Ctx &, const cpx<double> *, const cpx<double> *, int, int, int);

@@ -291,6 +291,13 @@ class Lattice:
                     "spectrum_discrete_k")
         return S
 
+    def spectrum_discrete_k_noalloc(self, fk_dev_ptr, s_dev_ptr, kind=0):
+        """lat_spectrum_discrete_k on a device-resident transform into device
+        memory, allocating nothing (pointers are device addresses)."""
+        self._check(lib().lat_spectrum_discrete_k_noalloc(self._h, ctypes.c_void_p(fk_dev_ptr),
+                                                          ctypes.c_void_p(s_dev_ptr), kind),
+                    "spectrum_discrete_k_noalloc")
+
     def crossspec_discrete_k(self, Fk1, Fk2):
         """crossspec_nofft (Discrete/Entropy/analysis.f90:149-186) -> (S1, S2)."""
         Fk1 = np.ascontiguousarray(Fk1, dtype=np.complex128)
