@@ -129,25 +129,55 @@ def roofline(prof, n, nf, w):
 
 
 def cpu_baseline(n_sample, steps_sample):
-    """The oracle (C restatement, OpenMP) on a bounded sample of the same workload."""
+    """The oracle (the C restatement of the reference's FP64 CPU path, OpenMP
+    over all host threads) on a bounded sample of the same workload: the
+    same two-field physics, dt = dx/40, Yoshida-4, at n_sample^3.  Built here
+    with -O3 -march=native -fopenmp for this host (gcc, a few seconds)."""
+    import ctypes
+    import platform
+    import shutil
+    import subprocess
+    import tempfile
+
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
-    dt = (10.0 / n_sample) / 40.0
-    sim = O.Simulation(n_sample, 2, 10.0, O.COSMIC, model=O.MODEL_TWOFIELD, mp=(1.0, G2))
-    rng = np.random.default_rng(1)
-    sim.fld[0] = PHI0 + 1e-6 * rng.standard_normal(sim.fld[0].shape)
-    sim.fld[1] = 1e-6 * rng.standard_normal(sim.fld[1].shape)
-    sim.fldp[0] = DPHI0
-    sim.yscl = 1.0
-    sim.calc_metric()
-    t0 = time.perf_counter()
-    sim.symp(4, dt, steps_sample)
-    el = time.perf_counter() - t0
+    tmp = tempfile.mkdtemp(prefix="latoracle_native_")
+    so = os.path.join(tmp, "liblatoracle_native.so")
+    flags = ["-O3", "-march=native", "-fcx-limited-range", "-fopenmp", "-fPIC", "-std=gnu11", "-shared"]
+    subprocess.run(["gcc", *flags, os.path.join(ROOT, "oracle", "latoracle.c"), "-o", so, "-lm"], check=True)
+    O._LIB_PATH = so
+    O._lib = None
+    try:
+        dt = (10.0 / n_sample) / 40.0
+        sim = O.Simulation(n_sample, 2, 10.0, O.COSMIC, model=O.MODEL_TWOFIELD, mp=(1.0, G2))
+        rng = np.random.default_rng(1)
+        blk = rng.standard_normal((2, 32, 32, 32))
+        reps = n_sample // 32
+        sim.fld[0] = PHI0 + 1e-6 * np.tile(blk[0], (reps, reps, reps))
+        sim.fld[1] = 1e-6 * np.tile(blk[1], (reps, reps, reps))
+        sim.fldp[0] = DPHI0
+        sim.yscl = 1.0
+        sim.calc_metric()
+        t0 = time.perf_counter()
+        sim.symp(4, dt, steps_sample)
+        el = time.perf_counter() - t0
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    model = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {"value": n_sample ** 3 * steps_sample / el, "unit": "site-steps/s", "cores": threads, "kind": "port",
-            "sample": f"oracle/liblatoracle.so FP64, {n_sample}^3 two-field cosmic spectral symp4, "
-                      f"{steps_sample} steps, {el:.1f} s"}
+            "sample": f"oracle/latoracle.c FP64 built -O3 -march=native -fopenmp on this host, {threads} OpenMP "
+                      f"threads ({model}, nproc {os.cpu_count()}), {n_sample}^3 two-field cosmic spectral "
+                      f"Yoshida-4, dt = dx/40, {steps_sample} steps, {el:.1f} s"}
 
 
 def spectrum_error(sa, sb, floor):
@@ -396,16 +426,16 @@ def gw_leg(la, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--output-every", type=int, default=20)
     ap.add_argument("--forms", default="f64,mixed")
     ap.add_argument("--lap-form", default="3d", choices=["3d", "sep"],
                     help="mixed form: five-sweep 3-D spectral Laplacian or the separable three-sweep form")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-n", type=int, default=256)
-    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--cpu-n", type=int, default=512)
+    ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the library's plain 2 GiB device copy (4 launches) to calibrate HBM byte "
                          "counters")

@@ -47,9 +47,20 @@
  *       traceless, idempotent), plane-wave source closed forms (spectral and
  *       central-difference), the free-wave cos(k t) solution in a static
  *       background, and the vanishing TT part of a one-dimensional source.
- *   orc_sample         Pseudospec/openmp_dev/evolve_spectral.f90:421-506
+ *   orc_sample, orc_sample_kernel, orc_dst2
+ *       Pseudospec/openmp_dev/evolve_spectral.f90:421-506
  *       with a Philox-4x32-10 stream in place of the compiler's random_number
- *       -> DST-II vs direct sum, Hermitian-consistent statistics.
+ *       -> DST-II vs the direct O(n^2) sum and scipy; the kernel's lattice
+ *          transform vs the continuum transform of its radial profile (norm,
+ *          gamma, cutoff, interpolation); seed-averaged mode power vs
+ *          2 |G_k|^2 inside and |G_k|^2 on the kx = 0 / Nyquist planes (the
+ *          Rayleigh amplitude and the C2R's Hermitian handling).
+ *   orc_discrete_output
+ *       Discrete/Entropy/iofile.f90:388-582,617-647 (get_variables, output_reg,
+ *       output_log_file), outputfile.f90:526-735 (the PSD column binning)
+ *       -> per-site densities and pressures against numpy direct sums; the
+ *          two spectra against numpy rfftn with the same weights (half space,
+ *          no x-Nyquist plane).
  */
 #include "latoracle.h"
 
@@ -75,7 +86,9 @@ typedef struct {
     int n;
     int nfac;
     int fac[64];
-    cplx *w; /* w[m] = exp(-2 pi i m / n), m = 0..n-1 */
+    cplx *w;      /* w[m] = exp(-2 pi i m / n), m = 0..n-1 */
+    cplx *stw;    /* per stage f: stw[stoff[f] + k*(r-1) + q-1] = w[(k*q*n/(ns*r)) % n] */
+    int stoff[64];
 } fft_plan;
 
 static void plan_init(fft_plan *p, int n)
@@ -94,9 +107,27 @@ static void plan_init(fft_plan *p, int n)
         long double ang = -TWOPI_L * (long double)k / (long double)n;
         p->w[k] = (double)cosl(ang) + I * (double)sinl(ang);
     }
+    /* the stage twiddles, copied from w[] at the indices the butterflies use */
+    size_t tot = 0;
+    int ns = 1;
+    for (int f = 0; f < p->nfac; ++f) {
+        p->stoff[f] = (int)tot;
+        tot += (size_t)ns * (size_t)(p->fac[f] - 1);
+        ns *= p->fac[f];
+    }
+    p->stw = (cplx *)malloc(sizeof(cplx) * (tot > 0 ? tot : 1));
+    ns = 1;
+    for (int f = 0; f < p->nfac; ++f) {
+        const int r = p->fac[f];
+        const int tstride = n / (ns * r);
+        for (int k = 0; k < ns; ++k)
+            for (int q = 1; q < r; ++q)
+                p->stw[p->stoff[f] + k * (r - 1) + q - 1] = p->w[(int)(((long long)k * q * tstride) % n)];
+        ns *= r;
+    }
 }
 
-static void plan_free(fft_plan *p) { free(p->w); }
+static void plan_free(fft_plan *p) { free(p->w); free(p->stw); }
 
 /* Out-of-place Stockham: x (length n, stride 1) -> result in x; y is scratch.
  * sign = -1 forward (FFTW_FORWARD), +1 backward. */
@@ -108,18 +139,18 @@ static void fft_exec(const fft_plan *p, cplx *x, cplx *y, int sign)
     for (int f = 0; f < p->nfac; ++f) {
         const int r = p->fac[f];
         const int nr = n / r;
+        const cplx *tws = p->stw + p->stoff[f];
         for (int j = 0; j < nr; ++j) {
             const int k = j % ns;
             cplx v[64];
             cplx t[64];
             /* gather with twiddles w^(k*q*n/(ns*r)) */
-            const int tstride = n / (ns * r);
-            for (int q = 0; q < r; ++q) {
-                cplx a = in[j + q * nr];
-                int widx = (int)(((long long)k * q * tstride) % n);
-                cplx tw = p->w[widx];
+            const cplx *tk = tws + (size_t)k * (r - 1);
+            v[0] = in[j] * p->w[0];
+            for (int q = 1; q < r; ++q) {
+                cplx tw = tk[q - 1];
                 if (sign > 0) tw = conj(tw);
-                v[q] = a * tw;
+                v[q] = in[j + q * nr] * tw;
             }
             /* radix-r DFT */
             if (r == 2) {
@@ -168,53 +199,47 @@ int orc_fft_c2c_1d(int n, double *re_im, int sign)
     return 0;
 }
 
-/* complex FFT along y (stride nxh) and z (stride nxh*ny) of a half-complex array */
+/* complex FFT along y (stride nxh) and z (stride nxh*ny) of a half-complex
+ * array.  Columns go through in groups of CB adjacent x-indices, gathered
+ * row by row (CB consecutive complex values per row) into a contiguous
+ * buffer, transformed one by one, and scattered back. */
+#define CB 8
+static void fft_cols(int nxh, int ncol, int nother, size_t s_col, size_t s_other, cplx *F, const fft_plan *p,
+                     int sign)
+{
+    const int ngrp = (nxh + CB - 1) / CB;
+#pragma omp parallel
+    {
+        cplx *a = (cplx *)malloc(sizeof(cplx) * (size_t)ncol * CB);
+        cplx *b = (cplx *)malloc(sizeof(cplx) * (size_t)ncol);
+#pragma omp for collapse(2) schedule(static)
+        for (int o = 0; o < nother; ++o)
+            for (int g = 0; g < ngrp; ++g) {
+                const int i0 = g * CB, cnt = (nxh - i0) < CB ? (nxh - i0) : CB;
+                cplx *col = F + i0 + s_other * o;
+                for (int j = 0; j < ncol; ++j)
+                    for (int c = 0; c < cnt; ++c) a[(size_t)c * ncol + j] = col[s_col * j + c];
+                for (int c = 0; c < cnt; ++c) fft_exec(p, a + (size_t)c * ncol, b, sign);
+                for (int j = 0; j < ncol; ++j)
+                    for (int c = 0; c < cnt; ++c) col[s_col * j + c] = a[(size_t)c * ncol + j];
+            }
+        free(a); free(b);
+    }
+}
+#undef CB
+
 static void fft_yz(int nxh, int ny, int nz, cplx *F, int sign)
 {
     fft_plan py, pz;
     plan_init(&py, ny);
     plan_init(&pz, nz);
-#pragma omp parallel
-    {
-        int nmax = ny > nz ? ny : nz;
-        cplx *a = (cplx *)malloc(sizeof(cplx) * (size_t)nmax);
-        cplx *b = (cplx *)malloc(sizeof(cplx) * (size_t)nmax);
-        if (sign < 0) {
-#pragma omp for collapse(2) schedule(static)
-            for (int k = 0; k < nz; ++k)
-                for (int i = 0; i < nxh; ++i) {
-                    cplx *col = F + i + (size_t)nxh * ny * k;
-                    for (int j = 0; j < ny; ++j) a[j] = col[(size_t)nxh * j];
-                    fft_exec(&py, a, b, sign);
-                    for (int j = 0; j < ny; ++j) col[(size_t)nxh * j] = a[j];
-                }
-#pragma omp for collapse(2) schedule(static)
-            for (int j = 0; j < ny; ++j)
-                for (int i = 0; i < nxh; ++i) {
-                    cplx *col = F + i + (size_t)nxh * j;
-                    for (int k = 0; k < nz; ++k) a[k] = col[(size_t)nxh * ny * k];
-                    fft_exec(&pz, a, b, sign);
-                    for (int k = 0; k < nz; ++k) col[(size_t)nxh * ny * k] = a[k];
-                }
-        } else {
-#pragma omp for collapse(2) schedule(static)
-            for (int j = 0; j < ny; ++j)
-                for (int i = 0; i < nxh; ++i) {
-                    cplx *col = F + i + (size_t)nxh * j;
-                    for (int k = 0; k < nz; ++k) a[k] = col[(size_t)nxh * ny * k];
-                    fft_exec(&pz, a, b, sign);
-                    for (int k = 0; k < nz; ++k) col[(size_t)nxh * ny * k] = a[k];
-                }
-#pragma omp for collapse(2) schedule(static)
-            for (int k = 0; k < nz; ++k)
-                for (int i = 0; i < nxh; ++i) {
-                    cplx *col = F + i + (size_t)nxh * ny * k;
-                    for (int j = 0; j < ny; ++j) a[j] = col[(size_t)nxh * j];
-                    fft_exec(&py, a, b, sign);
-                    for (int j = 0; j < ny; ++j) col[(size_t)nxh * j] = a[j];
-                }
-        }
-        free(a); free(b);
+    const size_t sy = (size_t)nxh, sz = (size_t)nxh * ny;
+    if (sign < 0) {
+        fft_cols(nxh, ny, nz, sy, sz, F, &py, sign);
+        fft_cols(nxh, nz, ny, sz, sy, F, &pz, sign);
+    } else {
+        fft_cols(nxh, nz, ny, sz, sy, F, &pz, sign);
+        fft_cols(nxh, ny, nz, sy, sz, F, &py, sign);
     }
     plan_free(&py);
     plan_free(&pz);
@@ -1071,6 +1096,173 @@ int orc_energies(orc_sim *s, double *out)
         out[ORC_E_GEGRAD] = 0.25 / ((a * dx) * (a * dx)) / s->c[4] * g / nv; /* homlat.f90:101-108 */
     }
     return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Discrete-tree diagnostics (make_output of Discrete/Entropy)          */
+/* ------------------------------------------------------------------ */
+
+/* STENCIL(c, O) at one site for the operators of Discrete/Entropy/macros.h:20-24,
+ * each summed over the 3^3 - 1 neighbours in RANK1..3 order (a rank's sum in
+ * the macro's neighbour order, then c0*RANK0 + c1*RANK1 + c2*RANK2 + c3*RANK3):
+ *   lap[l]  = LAPLACE(x,y,z) = f_l(nbr)                      (rank 0 included)
+ *   grad2   = GRAD2 = sum_l (f_l(nbr) - f_l(0))^2            (rank 0 term 0)
+ *   piphi[l]= PI_PHI = (p_l(nbr) - p_l(0)) (f_l(nbr) - f_l(0)) (rank 0 term 0) */
+static const int RK1[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+static const int RK2[12][3] = {{-1, -1, 0}, {1, -1, 0}, {-1, 1, 0}, {1, 1, 0}, {-1, 0, -1}, {1, 0, -1},
+                               {-1, 0, 1},  {1, 0, 1},  {0, -1, -1}, {0, 1, -1}, {0, -1, 1}, {0, 1, 1}};
+static const int RK3[8][3] = {{-1, -1, -1}, {1, -1, -1}, {-1, 1, -1}, {1, 1, -1},
+                              {-1, -1, 1},  {1, -1, 1},  {-1, 1, 1},  {1, 1, 1}};
+
+static void site_ops(const orc_sim *s, int i, int j, int k, double *lap, double *grad2, double *piphi)
+{
+    const size_t nv = nvol_of(s);
+    const int nx = s->nx, ny = s->ny, nz = s->nz, nf = s->nfld;
+    const size_t o0 = pidx(i, j, k, nx, ny, nz);
+    double rl[4][2] = {{0}}, rg[4] = {0}, rp[4][2] = {{0}};
+    for (int l = 0; l < nf; ++l) rl[0][l] = s->fld[(size_t)l * nv + o0];
+    for (int r = 1; r <= 3; ++r) {
+        const int cnt = r == 1 ? 6 : (r == 2 ? 12 : 8);
+        for (int q = 0; q < cnt; ++q) {
+            const int *d = r == 1 ? RK1[q] : (r == 2 ? RK2[q] : RK3[q]);
+            const size_t o = pidx(i + d[0], j + d[1], k + d[2], nx, ny, nz);
+            double g = 0.0;
+            for (int l = 0; l < nf; ++l) {
+                const double df = s->fld[(size_t)l * nv + o] - s->fld[(size_t)l * nv + o0];
+                rl[r][l] += s->fld[(size_t)l * nv + o];
+                g += df * df;
+                rp[r][l] += (s->fldp[(size_t)l * nv + o] - s->fldp[(size_t)l * nv + o0]) * df;
+            }
+            rg[r] += g;
+        }
+    }
+    const double *c = s->c;
+    for (int l = 0; l < nf; ++l) {
+        lap[l] = c[0] * rl[0][l] + c[1] * rl[1][l] + c[2] * rl[2][l] + c[3] * rl[3][l];
+        piphi[l] = c[0] * 0.0 + c[1] * rp[1][l] + c[2] * rp[2][l] + c[3] * rp[3][l];
+    }
+    *grad2 = c[0] * 0.0 + c[1] * rg[1] + c[2] * rg[2] + c[3] * rg[3];
+}
+
+/* modelv = 2V (Discrete/Entropy/model.f90:35-40) */
+static double modelv(const orc_sim *s, const double *f) { return 2.0 * orc_potential(s, f); }
+
+int orc_discrete_output(orc_sim *s, double time, double *log_out, double *lnrho, double *dlnrho, double *psd)
+{
+    if (s->lap_kind != ORC_LAP_STENCIL || s->nx != s->ny || s->nx != s->nz) return -1;
+    const size_t nv = nvol_of(s);
+    const int n = s->nx, nf = s->nfld;
+    const double a = scale_factor(s), dx = dx_of(s), cc = s->c[4];
+    const double hub = s->time_var == ORC_CONFORMAL ? -s->ysclp / s->yscl / 6.0 / s->yscl
+                                                    : -s->ysclp / s->yscl / 4.0; /* scafld.f90:67-75 */
+    /* homlat.f90:32-44 get_energy_coeffs */
+    const double ekin = 0.5 / pow(a, 6), egrad = 0.25 / ((a * dx) * (a * dx)) / cc, epot = 0.5;
+    const double lap_coeff = 1.0 / ((a * dx) * (a * dx)) / cc, piphi_coeff = 0.5 / ((a * dx) * (a * dx)) / cc;
+    double *rho = (double *)malloc(sizeof(double) * nv), *prs = (double *)malloc(sizeof(double) * nv);
+    double lnr = 0.0, eos = 0.0, KE = 0.0, GE = 0.0, PE = 0.0, phiave[2] = {0, 0}, phidot[2] = {0, 0};
+    /* get_variables (iofile.f90:478-582): one loop over the sites */
+    for (int k = 0; k < n; ++k)
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                const size_t q = (size_t)i + (size_t)n * ((size_t)j + (size_t)n * k);
+                double f[2] = {0, 0}, T = 0.0, lap[2], g, pp[2];
+                for (int l = 0; l < nf; ++l) {
+                    f[l] = s->fld[(size_t)l * nv + q];
+                    T += s->fldp[(size_t)l * nv + q] * s->fldp[(size_t)l * nv + q];
+                    phiave[l] += f[l];
+                    phidot[l] += s->fldp[(size_t)l * nv + q];
+                }
+                site_ops(s, i, j, k, lap, &g, pp);
+                const double V = modelv(s, f);
+                rho[q] = ekin * T + egrad * g + epot * V;
+                prs[q] = ekin * T - (1.0 / 3.0) * egrad * g - epot * V;
+                GE += g;
+                PE += V;
+                KE += T;
+                lnr += log(rho[q]);
+                eos += prs[q] / rho[q];
+            }
+    const double n3 = (double)n * (double)n * (double)n;
+    GE = egrad * GE / n3;
+    PE = epot * PE / n3;
+    KE = ekin * KE / n3;
+    eos /= n3;
+    lnr /= n3;
+    for (int l = 0; l < 2; ++l) { phiave[l] /= n3; phidot[l] /= n3; }
+    const double rhoave = GE + PE + KE;                  /* :578 */
+    const double prsave = KE - PE - (1.0 / 3.0) * GE;     /* :579 */
+    const double lnrhoave = lnr - log(rhoave);            /* :580 */
+    /* output_log_file (:643) */
+    const double grav = -3.0 * hub * hub;
+    int m = 0;
+    log_out[m++] = time;
+    log_out[m++] = a;
+    log_out[m++] = hub;
+    log_out[m++] = KE + GE + PE;    /* scalar_en, homlat.f90:194-196 */
+    log_out[m++] = prsave;
+    log_out[m++] = lnrhoave;
+    log_out[m++] = eos;
+    log_out[m++] = KE;
+    log_out[m++] = PE;
+    log_out[m++] = GE;
+    log_out[m++] = grav;
+    log_out[m++] = GE + KE + PE + grav; /* hamiltonian, homlat.f90:198-200 */
+    for (int l = 0; l < nf; ++l) log_out[m++] = phiave[l];
+    for (int l = 0; l < nf; ++l) log_out[m++] = phidot[l] / (a * a * a);
+    /* output_reg (:416-423): the two fields */
+    for (int k = 0; k < n; ++k)
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                const size_t q = (size_t)i + (size_t)n * ((size_t)j + (size_t)n * k);
+                double lap[2], g, pp[2];
+                site_ops(s, i, j, k, lap, &g, pp);
+                double num = 0.0;
+                for (int l = 0; l < nf; ++l) num += s->fldp[(size_t)l * nv + q] * (lap_coeff * lap[l]);
+                for (int l = 0; l < nf; ++l) num += piphi_coeff * pp[l];
+                lnrho[q] = log(rho[q]) - log(rhoave) - lnrhoave;
+                dlnrho[q] = -3.0 * hub * (prs[q] / rho[q] - prsave / rhoave) + num / rho[q] / (a * a * a);
+            }
+    free(rho);
+    free(prs);
+    /* PSD columns: two-bin weighted spectra over the stored half space without
+     * the x-Nyquist plane (outputfile.f90:569-596,656-663, S/W, no n^6), and
+     * crossspec_nofft(F(dlnrho), F(lnrho)) (analysis.f90:149-186) */
+    const int ns = (int)(1.7320508075688772935274463415059 * (double)(n / 2) + 2.0);
+    const int nxh = n / 2 + 1;
+    cplx *F1 = (cplx *)malloc(sizeof(cplx) * (size_t)nxh * n * n);
+    cplx *F2 = (cplx *)malloc(sizeof(cplx) * (size_t)nxh * n * n);
+    r2c_3d(n, n, n, lnrho, F1);
+    r2c_3d(n, n, n, dlnrho, F2);
+    double *W = (double *)calloc((size_t)ns, sizeof(double));
+    memset(psd, 0, sizeof(double) * 4 * (size_t)ns);
+    for (int k = 0; k < n; ++k) {
+        const int kk = wmag(k, n);
+        for (int j = 0; j < n; ++j) {
+            const int jj = wmag(j, n);
+            for (int ii = 0; ii < n / 2; ++ii) {
+                const size_t id = ii + (size_t)nxh * (j + (size_t)n * k);
+                const double p = sqrt((double)(ii * ii + jj * jj + kk * kk));
+                const int l = (int)floor(p);
+                const double d1 = (double)l - p, d2 = (double)(l + 1) - p;
+                const double c1 = (1.0 - d1 * d1) * (1.0 - d1 * d1), c2 = (1.0 - d2 * d2) * (1.0 - d2 * d2);
+                const cplx x = F2[id] * conj(F1[id]);
+                const double v[4] = {creal(F1[id] * conj(F1[id])), creal(F2[id] * conj(F2[id])), creal(x), cimag(x)};
+                for (int ch = 0; ch < 4; ++ch) psd[(size_t)ch * ns + l] += c1 * v[ch];
+                W[l] += c1;
+                if (l + 1 < ns) {
+                    for (int ch = 0; ch < 4; ++ch) psd[(size_t)ch * ns + l + 1] += c2 * v[ch];
+                    W[l + 1] += c2;
+                }
+            }
+        }
+    }
+    for (int b = 0; b < ns; ++b)
+        if (W[b] != 0.0)
+            for (int ch = 0; ch < 4; ++ch) psd[(size_t)ch * ns + b] /= W[b];
+    free(W);
+    free(F1);
+    free(F2);
+    return 12 + 2 * nf;
 }
 
 /* ------------------------------------------------------------------ */

@@ -94,6 +94,15 @@ enum {
 };
 int  orc_energies(orc_sim *s, double *out);
 
+/* Discrete-tree diagnostics of make_output (Discrete/Entropy/iofile.f90:317-335)
+ * on a stencil simulation.  log[12 + 2 nfld] = output_log_file's line
+ * (:643): t, a, H, scalar_en, prsave, lnrhoave, eosave, KE, PE, GE (GRAD2),
+ * grav_en, hamiltonian, <phi_l>, <pi_l>/a^3; lnrho[nvol], dlnrho[nvol] = the
+ * two fields output_reg bins (:416-423); psd[4 ns_d] = the PSD columns of
+ * those fields (get_fourier_stats, outputfile.f90:526-735, and
+ * crossspec_nofft of (dlnrho, lnrho)). */
+int  orc_discrete_output(orc_sim *s, double time, double *log, double *lnrho, double *dlnrho, double *psd);
+
 /* Model plug-in */
 double orc_potential(const orc_sim *s, const double *phi_site);
 void   orc_modeldv(const orc_sim *s, const double *phi_site, double *dv);

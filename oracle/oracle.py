@@ -262,6 +262,19 @@ class Simulation:
         lib().orc_energies(ctypes.byref(self.s), _p(out))
         return dict(zip(E_NAMES, out))
 
+    def discrete_output(self, time):
+        """Discrete-tree make_output quantities: (LOG.out line values,
+        lnrho field, dlnrho field, PSD columns (4, ns_d))."""
+        n = self.nx
+        ns = int(1.7320508075688772935274463415059 * (n // 2) + 2.0)
+        log = np.zeros(12 + 2 * self.nfld)
+        lnr, dln = np.zeros((n, n, n)), np.zeros((n, n, n))
+        psd = np.zeros((4, ns))
+        rc = lib().orc_discrete_output(ctypes.byref(self.s), ctypes.c_double(time), _p(log), _p(lnr), _p(dln),
+                                       _p(psd))
+        assert rc > 0
+        return log, lnr, dln, psd
+
 
 # ---------------------------------------------------------------- GW
 

@@ -815,3 +815,98 @@ def test_gw_one_dimensional_source_has_no_tt_part():
     Uk = np.stack([O.r2c_3d(s.gwp[c]) for c in range(6)])
     TT = O.tt_project(Uk, n)
     assert np.abs(TT).max() < 1e-12 * np.abs(Uk).max()
+
+
+# ---------------------------------------------------------------- Discrete make_output
+
+def _np_discrete(fld, fldp, c, dx, a, hub, mp):
+    """Discrete/Entropy get_variables + output_reg + output_log_file in numpy:
+    stencil sums with np.roll over the 26 neighbours (rank = number of nonzero
+    offsets), per-site rho and P, their statistics, the two fields and the
+    half-space spectra of their transforms."""
+    nf = fld.shape[0]
+    offs = [(dz, dy, dx_) for dz in (-1, 0, 1) for dy in (-1, 0, 1) for dx_ in (-1, 0, 1)]
+    rank = lambda o: sum(1 for v in o if v)
+    sh = lambda f, o: np.roll(f, shift=(-o[0], -o[1], -o[2]), axis=(0, 1, 2))  # f(x + o)
+    lap = [c[0] * fld[l] for l in range(nf)]
+    g2 = np.zeros(fld.shape[1:])
+    pp = [np.zeros(fld.shape[1:]) for _ in range(nf)]
+    for o in offs:
+        r = rank(o)
+        if r == 0:
+            continue
+        for l in range(nf):
+            fn, pn = sh(fld[l], o), sh(fldp[l], o)
+            lap[l] = lap[l] + c[r] * fn
+            g2 = g2 + c[r] * (fn - fld[l]) ** 2
+            pp[l] = pp[l] + c[r] * (pn - fldp[l]) * (fn - fld[l])
+    T = np.sum(fldp ** 2, axis=0)
+    if nf == 1:
+        V = 2.0 * 0.25 * mp[0] * fld[0] ** 4
+    else:
+        V = 2.0 * (0.5 * mp[0] * fld[0] ** 2 + 0.5 * mp[1] * fld[0] ** 2 * fld[1] ** 2 + 0.5 * mp[2] * fld[1] ** 2)
+    ekin, egrad, epot = 0.5 / a ** 6, 0.25 / (a * dx) ** 2 / c[4], 0.5
+    rho = ekin * T + egrad * g2 + epot * V
+    prs = ekin * T - egrad * g2 / 3.0 - epot * V
+    KE, GE, PE = ekin * T.mean(), egrad * g2.mean(), epot * V.mean()
+    rhoave, prsave = KE + GE + PE, KE - PE - GE / 3.0
+    lnrhoave = np.log(rho).mean() - np.log(rhoave)
+    grav = -3.0 * hub ** 2
+    log = [a, hub, KE + GE + PE, prsave, lnrhoave, (prs / rho).mean(), KE, PE, GE, grav, KE + GE + PE + grav]
+    log += [fld[l].mean() for l in range(nf)] + [fldp[l].mean() / a ** 3 for l in range(nf)]
+    lapc, ppc = 1.0 / (a * dx) ** 2 / c[4], 0.5 / (a * dx) ** 2 / c[4]
+    num = sum(fldp[l] * lapc * lap[l] for l in range(nf)) + sum(ppc * pp[l] for l in range(nf))
+    lnr = np.log(rho) - np.log(rhoave) - lnrhoave
+    dln = -3.0 * hub * (prs / rho - prsave / rhoave) + num / rho / a ** 3
+    n = fld.shape[1]
+    ns = int(np.sqrt(3.0) * (n // 2) + 2)
+    F1, F2 = np.fft.rfftn(lnr), np.fft.rfftn(dln)
+    kz = np.abs(np.fft.fftfreq(n, 1.0 / n))
+    kz[n // 2] = n // 2
+    kx = np.arange(n // 2 + 1)
+    p = np.sqrt(kz[:, None, None] ** 2 + kz[None, :, None] ** 2 + kx[None, None, :] ** 2)
+    keep = kx[None, None, :] < n // 2 + 0 * p
+    l = np.floor(p).astype(int)
+    c1, c2 = (1 - (l - p) ** 2) ** 2, (1 - (l + 1 - p) ** 2) ** 2
+    x = F2 * np.conj(F1)
+    vals = [np.abs(F1) ** 2, np.abs(F2) ** 2, x.real, x.imag]
+    psd = np.zeros((4, ns))
+    W = np.zeros(ns)
+    m = keep
+    np.add.at(W, l[m], c1[m])
+    m2 = m & (l + 1 < ns)
+    np.add.at(W, l[m2] + 1, c2[m2])
+    for ch, v in enumerate(vals):
+        np.add.at(psd[ch], l[m], (c1 * v)[m])
+        np.add.at(psd[ch], l[m2] + 1, (c2 * v)[m2])
+    psd = np.where(W > 0, psd / np.where(W > 0, W, 1.0), psd)
+    return np.array(log), lnr, dln, psd
+
+
+@pytest.mark.parametrize("stencil,nf", [((-6.0, 1.0, 0.0, 0.0, 1.0), 2), ((-128.0, 14.0, 3.0, 1.0, 30.0), 2),
+                                        ((-6.0, 1.0, 0.0, 0.0, 1.0), 1)], ids=["7pt", "27pt", "7pt-phi4"])
+def test_discrete_output_matches_numpy(stencil, nf):
+    """orc_discrete_output (get_variables, output_reg, output_log_file of
+    Discrete/Entropy/iofile.f90) against a numpy restatement built on np.roll
+    stencils and np.fft.rfftn, on a random state with y != 1 and p_y != 0."""
+    n, L = 8, 5.0
+    rng = np.random.default_rng(17 + nf)
+    model = O.MODEL_TWOFIELD if nf == 2 else O.MODEL_PHI4
+    mp = (1.0, 3.0, 0.5) if nf == 2 else (0.8,)
+    sim = O.Simulation(n, nf, L, O.COSMIC, lap_kind=O.LAP_STENCIL, stencil=stencil, model=model, mp=mp)
+    sim.fld[:] = 1.0 + 0.1 * rng.standard_normal(sim.fld.shape)
+    sim.fldp[:] = 0.3 + 0.1 * rng.standard_normal(sim.fldp.shape)
+    sim.yscl, sim.ysclp = 1.4, -0.6
+    log, lnr, dln, psd = sim.discrete_output(2.5)
+    a, hub = 1.4 ** (2.0 / 3.0), 0.6 / 1.4 / 4.0
+    nlog, nlnr, ndln, npsd = _np_discrete(sim.fld, sim.fldp, stencil, L / n, a, hub, mp + (0.0,) * 3)
+    assert log[0] == 2.5
+    assert np.allclose(log[1:], nlog, rtol=1e-12, atol=1e-14)
+    assert np.allclose(lnr, nlnr, rtol=1e-12, atol=1e-13) and np.allclose(dln, ndln, rtol=1e-11, atol=1e-12)
+    assert np.allclose(psd, npsd, rtol=1e-10, atol=1e-12 * np.abs(npsd).max())
+    # a homogeneous state: no gradients, uniform rho: every field statistic is
+    # trivial and both transformed fields vanish
+    sim.fld[:] = 1.1
+    sim.fldp[:] = 0.2
+    log, lnr, dln, psd = sim.discrete_output(0.0)
+    assert abs(log[5]) < 1e-13 and np.abs(lnr).max() < 1e-13 and np.abs(psd).max() < 1e-20
