@@ -155,10 +155,18 @@ def test_fd_chunked_z_march_and_grad2(gpu, shape, stencil):
         assert abs(eg[k] - o[0][-1]["e"][k]) <= 1e-11 * abs(o[0][-1]["e"][k]), k
     # GRAD2 form on identical state: same differences, other summation order
     assert abs(e0["ge_grad"] - o0["ge_grad"]) <= 1e-12 * abs(o0["ge_grad"])
-    # summation by parts: GE_grad == GE_lap up to the FP64 resolution of the
-    # differences (fluctuations 1e-6 on a mean of order 1)
-    assert abs(e0["ge_grad"] - e0["ge"]) <= 1e-8 * abs(e0["ge"])
-    assert np.isfinite(eg["ge_grad"]) and abs(eg["ge_grad"] - eg["ge"]) <= 1e-8 * abs(eg["ge"])
+    # after the run: the evolved fluctuations agree to ~1e-10 (above), and so
+    # do the squared differences
+    assert abs(eg["ge_grad"] - o[0][-1]["e"]["ge_grad"]) <= 1e-8 * abs(o[0][-1]["e"]["ge_grad"])
+    # summation by parts: GE_grad == GE_lap.  On phi = mean + 1e-6 noise the
+    # Laplacian form loses ~1e-6 relative (the mean times the rounding of each
+    # site's stencil sum), so the identity is checked on the fluctuations
+    # alone, where both forms are well conditioned
+    with gpu.Lattice(shape, nfld=2, length=L, laplacian=gpu.LAP_STENCIL, stencil=stencil,
+                     time_var=gpu.COSMIC, model=gpu.MODEL_TWOFIELD, model_params=(1.0, 2.5e5)) as lat:
+        lat.set_state(fld - fld.mean(axis=(1, 2, 3), keepdims=True), fldp, 1.0, 0.0)
+        ez = lat.energies()
+    assert abs(ez["ge_grad"] - ez["ge"]) <= 1e-12 * abs(ez["ge"]), (ez["ge_grad"], ez["ge"])
 
 
 PHI0, DPHI0 = 1.0093430384226378929425913902459, -0.7137133070120812430962278466136

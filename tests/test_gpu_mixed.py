@@ -124,11 +124,14 @@ def test_mixed_tracks_fp64_256_1000_steps(gpu):
     print("mixed 256^3 x 1000:", {"fluct_fld": fl, "fluct_fldp": flp, "spec_ge_1e-3": err3, "spec_ge_1e-6": err6,
                                   "spec_all": errall, "rho": erho, "resid_mixed": e1m["resid"],
                                   "resid_f64": e1d["resid"]}, flush=True)
-    assert max(fl) < 1e-4 and max(flp) < 1e-4
+    # measured on MI355X: fluctuations 9.0e-6 / 2.7e-6 (fields), 8.6e-6 /
+    # 2.5e-6 (momenta); spectra 1.2e-6 (>= 1e-3 of peak), 4.9e-6 (>= 1e-6),
+    # 2.0e-2 (all bins); rho 6.0e-11; residual 9e-11 apart
+    assert max(fl) < 5e-5 and max(flp) < 5e-5
     assert err3 < 1e-5
-    assert err6 < 1e-4
-    assert erho < 1e-8
-    assert dres < 1e-7
+    assert err6 < 3e-5
+    assert erho < 1e-9
+    assert dres < 1e-9
 
 
 def test_mixed_with_chi_mass_tracks_oracle(gpu):
@@ -147,12 +150,17 @@ def test_mixed_with_chi_mass_tracks_oracle(gpu):
     sim.set_state(fld, fldp, 1.0, 0.0)
     sim.calc_metric()
     sim.symp(4, dt, nsteps)
-    for prec, tol in ((gpu.F64, 1e-12), (gpu.MIXED, 1e-5)):
+    # errors relative to the fluctuations (1e-5 .. 1e-4 on means of order 1):
+    # FP64 resolves them to ~1e-16 / 1e-5, the mixed form to FP32 rounding
+    for prec, tol in ((gpu.F64, 1e-9), (gpu.MIXED, 1e-5)):
         with gpu.Lattice(n, nfld=2, length=L, precision=prec, model_params=mp) as lat:
             lat.set_state(fld, fldp, 1.0, 0.0)
             lat.calc_metric()
             lat.symp4(dt, nsteps)
             f, p, y, py = lat.get_state()
+        if prec == gpu.F64:
+            assert np.linalg.norm(f - sim.fld) <= 1e-12 * np.linalg.norm(sim.fld)
+            assert np.linalg.norm(p - sim.fldp) <= 1e-12 * np.linalg.norm(sim.fldp)
         for l in range(2):
             for a, b in ((f[l], sim.fld[l]), (p[l], sim.fldp[l])):
                 e = np.linalg.norm(a - b) / np.linalg.norm(b - b.mean())
