@@ -250,6 +250,26 @@ int  lat_gw_spectrum(lat_ctx *ctx, double *out);
  * columns written as 0) plus an empty record to <dir>/spectrum.out. */
 int  lat_make_output(lat_ctx *ctx, double time, int tree, const char *dir);
 
+/* Replaces make_output() of the Discrete tree (Discrete/Entropy/iofile.f90:
+ * 317-335) on a cubic finite-difference context: get_variables (:432-582),
+ * the PSD block of output_reg (:388-430; outputfile.f90:200-216,526-735) and
+ * output_log_file (:617-647).  Appends one '(32g25.16e2)' line to
+ * <dir>/LOG.out: t, a, H, <rho>, <P>, <ln rho> - ln<rho>, <P/rho>, KE, PE,
+ * GE (GRAD2 form), -3H^2, rho - 3H^2, <phi_l> (l = 1..nfld), <pi_l>/a^3 --
+ * with rho = KE + GE + PE and P = KE - PE - GE/3 per site (T = sum pi^2,
+ * G = STENCIL(c,GRAD2), V = modelv = 2V; homlat.f90:32-44) -- and one block
+ * of ns_d lines to <dir>/PSD: t, a, H, (k-1) dk and the spectra of
+ * ln rho - <ln rho> and d ln rho/dt (iofile.f90:416-423) and the Re, Im of
+ * their cross spectrum (half space without the x-Nyquist plane, two-bin
+ * weights, S / W), then two empty records.  A new file starts with head()
+ * (outputfile.f90:1216-1250) for one process: the revision line with L, dt,
+ * mpl and seed, rank and decomposition lines, the model line and the
+ * variable list.  vals (nullable) receives the LOG.out values (12 + 2 nfld),
+ * psd (nullable) the four PSD columns (4 x ns_d, column-major as written);
+ * dir = NULL computes without writing. */
+int  lat_discrete_output(lat_ctx *ctx, double time, double dt, double mpl, int seed, const char *dir, double *vals,
+                         double *psd);
+
 /* Replace the checkpoint write/read of Discrete/Entropy/driver.f90:68-118
  * (make_check_pt / read_check_pt, scafld.f90:224-284) for one process:
  * <dir>/fld_<i>.raw and fldp_<i>.raw (field i = 1..nfld, native FP64, x

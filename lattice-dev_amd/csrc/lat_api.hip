@@ -468,7 +468,7 @@ static PhysSums phys_sums(const Ctx &c)
 }
 
 // ---------------------------------------------------------------- host transforms (FP64)
-static int ensure_io(Ctx &c)
+int ensure_io(Ctx &c)
 {
     int rc = ensure(c, (void **)&c.io64, sizeof(double) * c.nvol * 2, "hipMalloc io64");
     if (rc) return rc;
@@ -682,8 +682,6 @@ static int create_ctx(const lat_config *cfg, int B, const double *mparams, lat_c
     return 0;
 }
 
-static int ensure_spec(Ctx &c);
-
 extern "C" {
 
 int lat_create(const lat_config *cfg, lat_ctx **out) { return create_ctx(cfg, 1, nullptr, out); }
@@ -760,7 +758,7 @@ void lat_destroy(lat_ctx *h)
     if (c.stream) (void)hipStreamSynchronize(c.stream);
     void *ptrs[] = {c.fld, c.fldp, c.Fk, c.Fk2, c.io64, (c.Fk64 != c.Fk) ? c.Fk64 : nullptr, c.dsc, c.partials,
                     c.spec_part, c.spec_out, c.twx64, c.twy64, c.twz64, c.twx32, c.twy32, c.twz32,
-                    c.gwu, c.gwp, c.gwsrc, c.gwk, c.lap};
+                    c.gwu, c.gwp, c.gwsrc, c.gwk, c.lap, c.disc_part};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : c.prof.pool) (void)hipEventDestroy(e);
@@ -881,7 +879,14 @@ int lat_stencil_laplacian(lat_ctx *h, const double *f, double *lap)
     return sync(c);
 }
 
-static int ensure_spec(Ctx &c)
+} // extern "C"
+
+int lat::ensure_disc(Ctx &c)
+{
+    return ensure(c, (void **)&c.disc_part, sizeof(double) * MAX_GRID * 16, "hipMalloc disc_part");
+}
+
+int lat::ensure_spec(Ctx &c)
 {
     const size_t cap = (size_t)c.spec_cap;
     int rc = ensure(c, (void **)&c.spec_part, sizeof(double) * (BIN_BLOCKS + BIN_BLOCKS / BIN_GROUP) * 5 * cap,
@@ -889,6 +894,8 @@ static int ensure_spec(Ctx &c)
     rc |= ensure(c, (void **)&c.spec_out, sizeof(double) * (5 + Ctx::WCACHE) * cap, "hipMalloc spec_out");
     return rc;
 }
+
+extern "C" {
 
 // S(b) / W(b) / div where W(b) != 0 (analysis_spec.f90:101)
 static void finish_spectra(const std::vector<double> &raw, int ns, int nch, double *const *outs, double div = 1.0)
@@ -1494,6 +1501,24 @@ static int append_line(Ctx &c, const std::string &path, const std::vector<double
 }
 
 extern "C" {
+
+int lat_discrete_output(lat_ctx *h, double time, double dt, double mpl, int seed, const char *dir, double *vals,
+                        double *psd)
+{
+    CTX_OR_FAIL(h);
+    ONE_MEMBER(c);
+    if (c.cfg.laplacian != LAT_LAP_STENCIL) {
+        c.last_error = "Discrete-tree output needs a finite-difference (stencil) context";
+        return -2;
+    }
+    if (c.nx != c.ny || c.nx != c.nz) {
+        c.last_error = "Discrete-tree output needs a cubic lattice";
+        return -2;
+    }
+    int rc = pull_scalars(c);
+    if (rc) return rc;
+    return discrete_output(c, time, dt, mpl, seed, dir, vals, psd);
+}
 
 int lat_make_output(lat_ctx *h, double time, int tree, const char *dir)
 {

@@ -134,6 +134,7 @@ struct Ctx {
     DevScalars *dsc;
     double *partials;       // MAX_GRID * NRED
     double *spec_part;      // spectrum block partials
+    double *disc_part = nullptr; // Discrete-tree output: block partials of the per-site sums
     double *spec_out;       // 5 * spec_cap reduced channels, then the W cache slots
     int ns;
     int spec_cap;           // bins the histogram buffers hold
@@ -230,6 +231,18 @@ int launch_bin(Ctx &c, const cpx<T> *F1, const cpx<T> *F2, int mode, int nbins =
 // per-device scratch lookup at context creation, then the launch
 int noalloc_setup(Ctx &c);
 int spectrum_discrete_k_noalloc(Ctx &c, const double *Fk_dev, double *S_dev, int kind);
+
+// ---- Discrete-tree make_output (lat_discrete.hip) ----
+// LOG.out line values into vals and the PSD columns into psd (both
+// nullable) and, with dir, the LOG.out line and the PSD block appended to
+// <dir>/LOG.out and <dir>/PSD
+int discrete_output(Ctx &c, double time, double dt, double mpl, int seed, const char *dir, double *vals,
+                    double *psd);
+// scratch on demand (lat_api.hip): FP64 staging (io64, Fk64), spectrum
+// histograms, Discrete-output partials
+int ensure_io(Ctx &c);
+int ensure_spec(Ctx &c);
+int ensure_disc(Ctx &c);
 
 // ---- initial conditions (lat_ic.hip) ----
 int sample_to_io(Ctx &c, double gamma, double m2eff, double mpl, double kcut_fac, uint64_t seed, uint64_t stream);

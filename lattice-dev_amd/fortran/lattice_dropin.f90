@@ -62,6 +62,7 @@ module lattice_dropin
   public :: Hamiltonian_Split, symp_o2step, symp2, symp4, symp6, symp8, calc_metric
   public :: get_scale_factor, get_hubble, kinetic_norm, grav_energy, lattice_energies
   public :: scalar_rho, kinetic_energy, grad_energy, potential_energy, grad_en, grad_en_wlap
+  public :: discrete_make_output
   public :: spectrum, spectrum_bin, spectrum_nofft, spectrum_bin_nofft, crossspec_nofft
   public :: lattice_put_gw, lattice_get_gw, gw_spectrum
 
@@ -185,6 +186,15 @@ module lattice_dropin
        type(c_ptr), value :: h
        real(c_double) :: out(*)
      end function c_lat_gw_spectrum
+     integer(c_int) function c_lat_discrete_output(h, time, dt, mpl, seed, dir, vals, psd) &
+          bind(C, name="lat_discrete_output")
+       import :: c_int, c_ptr, c_double, c_char
+       type(c_ptr), value :: h
+       real(c_double), value :: time, dt, mpl
+       integer(c_int), value :: seed
+       character(kind=c_char) :: dir(*)
+       type(c_ptr), value :: vals, psd
+     end function c_lat_discrete_output
      integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
        import :: c_size_t, c_ptr
        type(c_ptr), value :: s
@@ -461,6 +471,24 @@ contains
   real(dl) function potential_energy()
     potential_energy = pecur
   end function potential_energy
+
+  ! Discrete/Entropy/iofile.f90:317-335 make_output on a finite-difference
+  ! context: appends the LOG.out line and the PSD block (head() headers on
+  ! new files) in directory dir (default: the working directory); dt, mpl
+  ! and seed go into the header's run line as in outputfile.f90:1231
+  subroutine discrete_make_output(time, dt, mpl, seed, dir)
+    real(dl), intent(in) :: time, dt, mpl
+    integer, intent(in) :: seed
+    character(*), intent(in), optional :: dir
+    character(kind=c_char, len=:), allocatable :: d
+    if (present(dir)) then
+       d = trim(dir) // c_null_char
+    else
+       d = "." // c_null_char
+    end if
+    call check(c_lat_discrete_output(ctx, time, dt, mpl, int(seed, c_int), d, c_null_ptr, c_null_ptr), &
+               "discrete_make_output")
+  end subroutine discrete_make_output
 
   ! Discrete/Entropy/homlat.f90:88-115: gradient energy from STENCIL(c,GRAD2)
   ! (finite-difference contexts; NaN for the spectral Laplacian)

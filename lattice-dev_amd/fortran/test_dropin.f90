@@ -114,6 +114,14 @@ program test_dropin
   print *, "grad_en", grad_en(), " grad_en_wlap", grad_en_wlap()
   if (.not. (grad_en() > 0._dl)) stop 1
   if (abs(grad_en() - grad_en_wlap()) > 1e-12_dl * grad_en_wlap()) stop 1
+  ! the Discrete tree's make_output writes LOG.out and PSD in the working directory
+  call discrete_make_output(0._dl, 1e-3_dl, 2e5_dl, 1)
+  block
+    logical :: ok1, ok2
+    inquire(file="LOG.out", exist=ok1)
+    inquire(file="PSD", exist=ok2)
+    if (.not. (ok1 .and. ok2)) stop 1
+  end block
 
   ! metric perturbations sourced by a two-field configuration with gradients
   call lattice_init(n, n, n, nfld, len, LAT_F64, LAT_LAP_SPECTRAL, LAT_COSMIC, LAT_MODEL_TWOFIELD, &

@@ -367,6 +367,19 @@ class Lattice:
         self._check(lib().lat_gw_spectrum(self._h, _p(out)), "gw_spectrum")
         return out.reshape(2, self.ns)
 
+    def discrete_output(self, time, dt=0.0, mpl=1.0, seed=0, directory=None):
+        """Discrete-tree make_output (iofile.f90:317-335): returns the LOG.out
+        values and the PSD columns (4, ns_d); with `directory`, appends LOG.out
+        and PSD there."""
+        vals = np.empty(12 + 2 * self.nfld)
+        ns = lib().lat_num_bins_discrete(self.nx)
+        psd = np.empty((4, ns))
+        d = directory.encode() if directory is not None else None
+        self._check(lib().lat_discrete_output(self._h, ctypes.c_double(time), ctypes.c_double(dt),
+                                              ctypes.c_double(mpl), int(seed), d, _p(vals), _p(psd)),
+                    "discrete_output")
+        return vals, psd
+
     def make_output(self, time, directory=".", tree=0):
         """make_output/dump_rho: append to energy_spec.out and spectrum.out."""
         self._check(lib().lat_make_output(self._h, ctypes.c_double(time), int(tree),

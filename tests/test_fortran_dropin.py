@@ -22,9 +22,10 @@ def test_dropin_compiles_and_links():
 
 
 @pytest.mark.gpu
-def test_dropin_driver_runs_on_gpu(gpu):
+def test_dropin_driver_runs_on_gpu(gpu, tmp_path):
     if not os.path.exists(BIN):
         pytest.fail("Fortran driver not built: run __graft_entry__.build()")
-    r = subprocess.run([BIN], capture_output=True, text=True, timeout=120)
+    # the driver writes the Discrete tree's LOG.out / PSD in its working directory
+    r = subprocess.run([BIN], capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
     print(r.stdout, r.stderr)
     assert r.returncode == 0 and "DROPIN OK" in r.stdout

@@ -129,3 +129,142 @@ def test_checkpoint_restore_resumes_run(gpu, tmp_path, precision):
     tol = 1e-13 if precision == "f64" else 1e-6
     assert np.linalg.norm(again[0] - full[0]) <= tol * np.linalg.norm(full[0])
     assert np.linalg.norm(again[1] - full[1]) <= tol * np.linalg.norm(full[1])
+
+
+DISCRETE_FORTRAN = """program disc
+  implicit none
+  integer, parameter :: dl = kind(1.d0), namelength = 12
+  integer :: nfld, n, seed, nlog, ns, k, rep, nrep
+  real(dl) :: len, dt, mpl, m2phi, m2psi, g2
+  real(dl) :: v(64)
+  open(10, file='in.txt')
+  read(10,*) nfld, n, len, dt, mpl, seed, m2phi, m2psi, g2, nrep
+  open(11, file='LOG.out')
+  open(12, file='PSD')
+  call head(11, [character(len=namelength) :: "t    ", "a    ", "H    ", "<rho>", "<P>  ", "<lnrho>", "<EOS>", &
+                 "<E_KIN>", "E_POT>", "<E_grad>", "GRAVEN", "Ham"])
+  call head(12, [character(len=namelength) :: "t  ", "a  ", "H   ", "k   ", "logrho_ave", "dlnrho", "lnr_dlnr", &
+                 "lnr_dlnr"])
+  do rep = 1, nrep
+     read(10,*) nlog
+     read(10,*) v(1:nlog)
+     write(11,'(32g25.16e2)') v(1:nlog)
+     read(10,*) ns
+     do k = 1, ns
+        read(10,*) v(1:8)
+        write(12,'(32g25.16e2)') v(1:8)
+     enddo
+     write(12,'(a)') "",""
+  enddo
+contains
+  subroutine head(fd, vars)
+    integer fd
+    character(*) :: vars(:)
+    character(512) :: buffer
+    integer l
+    write (fd,'(a,4(i0,a),3(g32.15e2,a),i0)') "# This is DEFROST revision ", 1, ".", 0, " (", nfld, " fields, ", &
+         n, "^3 grid) L=", len, " dt = ", dt, " mpl = ", mpl, " seed = ", seed
+    write (fd,'(a,i0,a,i0)') "# MPI rank: ", 0, "; total nodes: ", 1
+    write (fd,'(a,i0,a,i0,a)') "# Using ", 1, " y division(s) and ", 1, " z division(s)"
+    write (fd,'(a,i0,a,i0)') "# Local y division: ", 0, "; local z division: ", 0
+    write (fd,'(a,i0,a,i0)') "# neighbor ranks: ", 0, " <- y -> ", 0
+    write (fd,'(a,i0,a,i0)') "# neighbor ranks: ", 0, " <- z -> ", 0
+    write (fd,'(a,3(f0.5,a))') "# V(phi,psi) = ", m2phi, "phi^2/2 + ", m2psi, "psi^2/2 + ", g2, "phi^2*phi^2/2"
+    write (buffer,'(a,g12.12",",32(g24.12","))') "OUTPUT:", adjustr(vars)
+    l = index(buffer, ',', .true.)
+    write (fd,'(2a)') "# ", repeat('=', l)
+    write (fd,'(2a)') "# ", buffer(1:l-1)//';'
+    write (fd,'(2a)') "# ", repeat('=', l)
+  end subroutine head
+end program disc
+"""
+
+
+def _fortran_discrete_files(workdir, header, reps):
+    src = os.path.join(workdir, "disc.f90")
+    with open(src, "w") as fh:
+        fh.write(DISCRETE_FORTRAN)
+    subprocess.run([FC, "-O0", src, "-o", os.path.join(workdir, "disc")], check=True, cwd=workdir)
+    with open(os.path.join(workdir, "in.txt"), "w") as fh:
+        fh.write(" ".join(repr(x) for x in header) + f" {len(reps)}\n")
+        for log, rows in reps:
+            fh.write(f"{len(log)}\n" + " ".join(repr(float(x)) for x in log) + "\n")
+            fh.write(f"{len(rows)}\n")
+            for r in rows:
+                fh.write(" ".join(repr(float(x)) for x in r) + "\n")
+    subprocess.run([os.path.join(workdir, "disc")], check=True, cwd=workdir)
+    out = {}
+    for name in ("LOG.out", "PSD"):
+        with open(os.path.join(workdir, name), "rb") as fh:
+            out[name] = fh.read()
+    return out
+
+
+def _psd_rows(path):
+    rows = []
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("#") or not line.strip():
+                continue
+            rows.append([float(line[25 * q:25 * (q + 1)]) for q in range(8)])
+    return np.array(rows)
+
+
+@pytest.mark.parametrize("stencil", [(-6.0, 1.0, 0.0, 0.0, 1.0), (-128.0, 14.0, 3.0, 1.0, 30.0)],
+                         ids=["7pt", "27pt"])
+@pytest.mark.parametrize("precision", ["f64", "mixed"])
+def test_discrete_output_matches_oracle_and_fortran_format(gpu, tmp_path, stencil, precision):
+    """Discrete-tree make_output (LOG.out line and PSD block with their head()
+    headers): values against the oracle's get_variables / output_reg /
+    output_log_file restatement, files byte for byte against the same values
+    written by amdflang with the reference's format statements, two calls
+    appended to the same files."""
+    print("case discrete output", stencil, precision, flush=True)
+    if not shutil.which(FC) and not os.path.exists(FC):
+        pytest.skip("amdflang not installed")
+    n, L, mp = 16, 10.0, (1.0, 1e4, 2.0)
+    rng = np.random.default_rng(33)
+    fld = np.stack([1.0 + 1e-2 * rng.standard_normal((n, n, n)), 1e-2 * rng.standard_normal((n, n, n))])
+    fldp = np.stack([-0.7 + 1e-2 * rng.standard_normal((n, n, n)), 1e-2 * rng.standard_normal((n, n, n))])
+    prec = gpu.F64 if precision == "f64" else gpu.MIXED
+    gdir = tmp_path / "gpu"
+    gdir.mkdir()
+    reps, vals = [], []
+    ns = int(np.sqrt(3.0) * (n // 2) + 2)
+    with gpu.Lattice(n, nfld=2, length=L, precision=prec, laplacian=gpu.LAP_STENCIL, stencil=stencil,
+                     model_params=mp) as lat:
+        lat.set_state(fld, fldp, 1.3, -0.4)
+        for t in (0.0, 0.25):
+            if t > 0:
+                lat.symp4(5e-3, 5)
+            v, gpsd = lat.discrete_output(t, 5e-3, 2e5, 7, str(gdir))
+            f, p, y, py = lat.get_state()
+            sim = O.Simulation(n, 2, L, O.COSMIC, lap_kind=O.LAP_STENCIL, stencil=stencil, model=O.MODEL_TWOFIELD,
+                               mp=mp)
+            sim.set_state(f, p, y, py)
+            olog, _, _, opsd = sim.discrete_output(t)
+            vals.append((v, olog, opsd, gpsd))
+    rows = _psd_rows(gdir / "PSD")
+    assert rows.shape == (2 * ns, 8)
+    for r, (v, olog, opsd, gpsd) in enumerate(vals):
+        tol = 1e-11 if precision == "f64" else 1e-6
+        # lnrhoave and eosave are averages of O(1) numbers with O(1e-4)
+        # spread, the rest are sums of positive terms or fluctuation means
+        assert np.allclose(v, olog, rtol=tol, atol=tol * 1e-2), (r, v, olog)
+        got = rows[r * ns:(r + 1) * ns]
+        assert np.allclose(got[:, 3], np.arange(ns) * (2 * np.pi / L), rtol=1e-15)
+        ptol = 1e-9 if precision == "f64" else 2e-3
+        for ch in range(4):
+            ref = opsd[ch]
+            assert np.allclose(gpsd[ch], ref, rtol=ptol, atol=ptol * np.abs(ref).max()), (r, ch)
+            assert np.allclose(got[:, 4 + ch], gpsd[ch], rtol=1e-15, atol=1e-15 * np.abs(gpsd[ch]).max())
+        dk = 2 * np.pi / L
+        reps.append((v, [[v[0], v[1], v[2], b * dk] + [gpsd[ch][b] for ch in range(4)] for b in range(ns)]))
+    # byte compatibility with the Fortran writer, given the same values
+    fdir = tmp_path / "fortran"
+    fdir.mkdir()
+    want = _fortran_discrete_files(str(fdir), [2, n, L, 5e-3, 2e5, 7, mp[0], mp[2], mp[1]], reps)
+    for name in ("LOG.out", "PSD"):
+        with open(gdir / name, "rb") as fh:
+            have = fh.read()
+        assert have == want[name], (name, have[:400], want[name][:400])
