@@ -603,6 +603,7 @@ static int create_ctx(const lat_config *cfg, int B, const double *mparams, lat_c
     Ctx &c = h->c;
     c.cfg = *cfg;
     (void)hipGetDevice(&c.device);
+    (void)hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, c.device);
     c.nx = cfg->nx;
     c.ny = cfg->ny;
     c.nz = cfg->nz;
@@ -715,6 +716,7 @@ int lat_member(lat_ctx *h, int m, lat_ctx **out)
     w.nvol = c.nvol; w.nk = c.nk; w.dx = c.dx; w.dk = c.dk; w.prec = c.prec; w.nfld = c.nfld;
     w.stream = c.stream;
     w.device = c.device;
+    w.cus = c.cus;
     w.fld = (char *)c.fld + es * c.ms.f * m;
     w.fldp = (char *)c.fldp + es * c.ms.f * m;
     w.Fk = c.Fk ? (void *)((char *)c.Fk + 2 * es * c.ms.k * m) : nullptr;

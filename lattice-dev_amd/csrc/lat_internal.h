@@ -138,7 +138,7 @@ struct Ctx {
     double *spec_out;       // 5 * spec_cap reduced channels, then the W cache slots
     int ns;
     int spec_cap;           // bins the histogram buffers hold
-    int spec_blocks;
+    int cus = 0;            // compute units of the device (sizes the binning grid)
     // The weight channel W(b) of the binning depends on the lattice geometry,
     // the bin count and the weighting flags only, so launch_bin computes it
     // once per (bin count, flags) and keeps it in one of WCACHE slots of

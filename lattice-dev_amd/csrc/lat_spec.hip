@@ -7,13 +7,20 @@
 // all nx x-indices and re-reads Fk(nx-i) for i > nx/2; on the half-complex
 // array that is a multiplicity of 2 for 0 < kx < nx/2.
 //
-// Determinism: each wave walks one (ky,kz) row at a time with the 64 lanes
-// on consecutive kx (4 waves per workgroup share one LDS histogram).  Along a
-// row p is non-decreasing in kx, so equal bins form contiguous lane
-// segments; a segmented DPP scan sums each segment and only segment tails
-// update the histogram, one wave after the other in wave order (no atomics).
-// Workgroup histograms are reduced in block order by k_bin_reduce (groups of
-// 64 blocks, then the groups).  Results are bitwise reproducible run to run.
+// Work unit: a row class (|ky|, |kz|).  The up to four rows (ky, kz),
+// (-ky, kz), (ky, -kz), (-ky, -kz) share every mode's |k|, bin and weights,
+// so their values are summed first (in that order) and the binning runs once
+// per class.  One wave per workgroup walks its classes; each lane takes 8
+// consecutive kx (at 1024^3; ceil((nx/2)/64) in general), and the lane that
+// reaches the row's last mode takes that one too.  Along a row p is
+// non-decreasing in kx, so a lane sums runs of equal bin in registers and
+// adds every run that starts and ends inside it to the wave's own LDS
+// histogram; the runs that cross lanes are joined by one segmented DPP scan
+// per class.  Within one histogram instruction the lanes' bins are distinct,
+// and the instructions run in program order (no atomics, no cross-wave
+// ordering).  The block histograms are reduced in block order by
+// k_bin_reduce (groups of 64 blocks, then the groups).  Results are bitwise
+// reproducible run to run.
 #include <mutex>
 #include <string>
 
@@ -84,8 +91,8 @@ __device__ __forceinline__ double tt_norm2(const double kv[3], const double ur[6
 //               `nofft` routines, :54-186);
 //   BIN_NO_NYQ  skip the x-Nyquist plane (spectrum_bin_nofft :69,
 //               crossspec_nofft :166).
-constexpr int BIN_ROWCHUNKS = 9; // 64-mode chunks per k-space row (nx <= 1024)
-constexpr int BIN_WAVES = 4;      // waves per workgroup, sharing one histogram
+constexpr int BIN_LANEMODES = 9;          // modes per lane and row: 8 + the row's last (nx <= 1024)
+constexpr size_t BIN_LDS_MAX = 64 * 1024;  // bytes of one block's LDS histogram
 
 // Cross-lane moves for the segmented scan with DPP (GFX9 data-parallel
 // primitives: row_shr:n within 16-lane rows, row_bcast:15 / row_bcast:31
@@ -138,140 +145,257 @@ __device__ __forceinline__ void seg_scan(int key, double (&v1)[NC], double (&v2)
 }
 
 template <typename T, int NCH, int MODE, bool WCH>
-__global__ __launch_bounds__(64 * BIN_WAVES) void k_bin(const cpx<T> *__restrict__ F1, const cpx<T> *__restrict__ F2,
-                                                        int nx, int ny, int nz, int nxh, int P, int ns,
-                                                        double *__restrict__ part, int flags, size_t nk)
+__global__ __launch_bounds__(64) void k_bin(const cpx<T> *__restrict__ F1, const cpx<T> *__restrict__ F2, int nx,
+                                            int ny, int nz, int nxh, int P, int ns, double *__restrict__ part,
+                                            int flags, size_t nk)
 {
     constexpr int NC = NCH + (WCH ? 1 : 0); // binned channels
-    extern __shared__ double hist[];       // NC * ns
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int q = threadIdx.x; q < NC * ns; q += 64 * BIN_WAVES) hist[q] = 0.0;
+    constexpr int NV = NCH > 0 ? NCH : 1;
+    constexpr int NOKEY = 0x7fffffff;
+    constexpr int LM = BIN_LANEMODES;
+    constexpr int LA = MODE <= 2 ? LM : 1;
+    // the next class's loads are issued before the current class is binned
+    // (FP32 data: two register sets fit)
+    constexpr bool PF = MODE <= 2 && sizeof(T) == 4;
+    extern __shared__ double hist[]; // NC * ns
+    const int lane = threadIdx.x;
+    for (int q = lane; q < NC * ns; q += 64) hist[q] = 0.0;
     __syncthreads();
-    const int rows = ny * nz;
     const bool counts = (flags & BIN_COUNTS) != 0;
-    // one 64-lane chunk of a row: value(s) a, b of mode i = i0 + lane
-    // one 64-lane chunk of a row: segment sums of bins l (v1) and l + 1 (v2)
-    // at the segment tails (tail = true)
-    auto chunk = [&](bool has_row, int row, int j, int k, int jj, int kk, int i0, cpx<T> a, cpx<T> b, int &key,
-                     bool &tail, double (&v1)[NC], double (&v2)[NC]) {
-        const int i = i0 + lane;
-        const bool valid = has_row && i < nxh && !((flags & BIN_NO_NYQ) && 2 * i == nx);
-        key = 0x7fffffff;
-        if (valid) {
-            const double p = sqrt(double(i * i + jj * jj + kk * kk));
-            const int l = (int)floor(p);
+    // the lane's modes [s, e) of a row: M consecutive ones, and the row's
+    // last mode for the lane whose range reaches it
+    const int M = nxh > 1 ? (nxh - 1 + 63) / 64 : 1;
+    const int s = lane * M;
+    int e = s + M;
+    if (e >= nxh - 1) e = nxh;
+    if (s > 0 && s >= nxh - 1) e = s;
+    const int nyq = ((flags & BIN_NO_NYQ) && nx % 2 == 0) ? nx / 2 : -1;
+    const int ncy = ny / 2 + 1, ncz = nz / 2 + 1;
+    const int ncls = ncy * ncz;
+    const int G = gridDim.x;
+
+    // the row class (jj, kk): rows (j, k) with j in {jj, ny - jj} and k in
+    // {kk, nz - kk}, each member once
+    struct Cls {
+        int jj, kk, nm, mj[4], mk[4];
+    };
+    auto members = [&](int q) {
+        Cls c;
+        c.jj = q % ncy;
+        c.kk = q / ncy;
+        const bool jm = c.jj > 0 && 2 * c.jj != ny, km = c.kk > 0 && 2 * c.kk != nz;
+        c.nm = 1 + (jm ? 1 : 0) + (km ? 1 : 0) + (jm && km ? 1 : 0);
+        c.mj[0] = c.jj; c.mk[0] = c.kk;
+        c.mj[1] = jm ? ny - c.jj : c.jj; c.mk[1] = jm ? c.kk : nz - c.kk;
+        c.mj[2] = c.jj; c.mk[2] = nz - c.kk;
+        c.mj[3] = ny - c.jj; c.mk[3] = nz - c.kk;
+        return c;
+    };
+    // The class's values, mode-major, modes sb .. sb + M of each member row:
+    // the lane's own (sb = s) or, for a lane without modes, the row's first
+    // ones; absent members read member 0.  Every load is in bounds and
+    // unpredicated; values of modes or members outside the class are never
+    // used.
+    const int sb = e > s ? s : 0;
+    struct Buf {
+        cpx<T> a[LA][4], b[LA][4];
+    };
+    auto load = [&](Buf &u, int q) {
+        if constexpr (MODE <= 2) {
+            const Cls c = members(q);
+            const cpx<T> *r1[4], *r2[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int mm = m < c.nm ? m : 0;
+                const size_t off = ((size_t)c.mk[mm] * ny + c.mj[mm]) * P + sb;
+                r1[m] = F1 + off;
+                r2[m] = MODE != 0 ? F2 + off : F1 + off;
+            }
+#pragma unroll
+            for (int t = 0; t < LM; ++t) {
+                if (t > M) continue; // (uniform)
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    u.a[t][m] = r1[m][t];
+                    if constexpr (MODE != 0) u.b[t][m] = r2[m][t];
+                    else u.b[t][m] = cpx<T>{0, 0};
+                }
+            }
+        }
+    };
+    auto bin_class = [&](const Buf &u, int q) {
+        const Cls c = members(q);
+        const int rr2 = c.jj * c.jj + c.kk * c.kk;
+        // value channels of mode i summed over the members, in member order
+        auto values = [&](int i, const cpx<T> (&a_)[4], const cpx<T> (&b_)[4], double (&val)[NV]) {
+#pragma unroll
+            for (int ch = 0; ch < NV; ++ch) val[ch] = 0.0;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                if (m >= c.nm) break;
+                if constexpr (MODE == 3) {
+                    const size_t id = ((size_t)c.mk[m] * ny + c.mj[m]) * P + i;
+                    const double kv[3] = {double(i), double(c.mj[m] <= ny / 2 ? c.mj[m] : c.mj[m] - ny),
+                                          double(c.mk[m] <= nz / 2 ? c.mk[m] : c.mk[m] - nz)};
+                    double ur[6], ui[6];
+#pragma unroll
+                    for (int c6 = 0; c6 < 6; ++c6) {
+                        const cpx<T> w = F1[(size_t)c6 * nk + id];
+                        ur[c6] = double(w.x);
+                        ui[c6] = double(w.y);
+                    }
+                    val[0] += tt_norm2(kv, ur, ui);
+                } else if constexpr (MODE <= 2) {
+                    const cpx<T> x = a_[m], y = b_[m];
+                    if constexpr (MODE == 0) {
+                        val[0] += double(x.x) * x.x + double(x.y) * x.y;
+                    } else {
+                        const double re = double(x.x) * y.x + double(x.y) * y.y;
+                        const double im = double(x.y) * y.x - double(x.x) * y.y;
+                        if constexpr (MODE == 1) {
+                            val[0] += re;
+                            val[1] += im;
+                        } else {
+                            val[0] += double(y.x) * y.x + double(y.y) * y.y;
+                            val[1] += double(x.x) * x.x + double(x.y) * x.y;
+                            val[2] += re;
+                            val[3] += im;
+                        }
+                    }
+                }
+            }
+        };
+        // Runs of equal bin along the lane's modes.  The first run may have
+        // begun in earlier lanes and the last one may go on in later lanes;
+        // every run in between is whole and goes straight to the histogram.
+        // Held: the first run (fk; fa, fb) once it has closed and the open
+        // run (lk; ra, rb), as their bin-l and bin-(l+1) weighted sums.  A
+        // mode outside the lane's range keeps the open run's bin and adds
+        // zero weight.  Runs are reset and captured by multiplying with 0/1,
+        // so a non-finite value can spread into later bins of its row class.
+        int fk = NOKEY, lk = NOKEY;
+        double fa[NC], fb[NC], ra[NC], rb[NC];
+#pragma unroll
+        for (int ch = 0; ch < NC; ++ch) fa[ch] = fb[ch] = ra[ch] = rb[ch] = 0.0;
+#pragma unroll
+        for (int t = 0; t < LM; ++t) {
+            if (t > M) continue; // (uniform)
+            const int i = s + t;
+            const bool valid = i < e && i != nyq;
+            const int ic = min(i, nxh - 1);
+            const double p = sqrt(double(ic * ic + rr2));
+            int l = (int)floor(p);
             const double d1 = double(l) - p, d2 = double(l + 1) - p;
             const double c1 = counts ? 1.0 : (1.0 - d1 * d1) * (1.0 - d1 * d1);
             const double c2 = counts ? 0.0 : (1.0 - d2 * d2) * (1.0 - d2 * d2);
-            const double mult = ((flags & BIN_HALF) || i == 0 || (2 * i == nx)) ? 1.0 : 2.0;
-            double val[NCH > 0 ? NCH : 1];
-            if constexpr (MODE == 4) {
-                val[0] = 0.0;
-            } else if constexpr (MODE == 3) {
-                const size_t id = (size_t)row * P + i;
-                const double kv[3] = {double(i), double(j <= ny / 2 ? j : j - ny), double(k <= nz / 2 ? k : k - nz)};
-                double ur[6], ui[6];
-#pragma unroll
-                for (int q = 0; q < 6; ++q) {
-                    const cpx<T> u = F1[(size_t)q * nk + id];
-                    ur[q] = double(u.x);
-                    ui[q] = double(u.y);
-                }
-                val[0] = tt_norm2(kv, ur, ui);
-            } else if constexpr (MODE == 0) {
-                val[0] = double(a.x) * a.x + double(a.y) * a.y;
-            } else {
-                const double re = double(a.x) * b.x + double(a.y) * b.y;
-                const double im = double(a.y) * b.x - double(a.x) * b.y;
-                if constexpr (MODE == 1) {
-                    val[0] = re;
-                    val[1] = im;
-                } else {
-                    val[0] = double(b.x) * b.x + double(b.y) * b.y;
-                    val[1] = double(a.x) * a.x + double(a.y) * a.y;
-                    val[2] = re;
-                    val[3] = im;
-                }
+            const double mult = ((flags & BIN_HALF) || ic == 0 || (2 * ic == nx)) ? 1.0 : 2.0;
+            double w1 = mult * c1, w2 = mult * c2;
+            double val[NV];
+            values(ic, u.a[MODE <= 2 ? t : 0], u.b[MODE <= 2 ? t : 0], val);
+            if (!valid) {
+                l = lk;
+                w1 = 0.0;
+                w2 = 0.0;
             }
+            const bool brk = lk != NOKEY && l != lk;
+            const bool whole = brk && fk != NOKEY;
+            // a whole run: bin l, then (after every lane's bin-l additions)
+            // bin l + 1
+            if (whole && lk < ns) {
 #pragma unroll
-            for (int ch = 0; ch < NCH; ++ch) {
-                v1[ch] = mult * c1 * val[ch];
-                v2[ch] = mult * c2 * val[ch];
-            }
-            if constexpr (WCH) {
-                v1[NCH] = mult * c1;
-                v2[NCH] = mult * c2;
-            }
-            key = l;
-        } else {
-#pragma unroll
-            for (int ch = 0; ch < NC; ++ch) v1[ch] = v2[ch] = 0.0;
-        }
-        // segmented inclusive scan over lanes with equal key
-        seg_scan<NC>(key, v1, v2);
-        const int nkey = __shfl_down(key, 1, 64);
-        tail = valid && (lane == 63 || nkey != key);
-    };
-    // Histogram updates, one wave at a time in wave order (a barrier after
-    // each), so the order of the additions into every bin is fixed.  Within
-    // a wave the tails have distinct keys; bin l + 1 of one segment may be
-    // bin l of the next, so the l + 1 additions follow the l additions in
-    // program order.
-    auto update = [&](int key, bool tail, const double (&v1)[NC], const double (&v2)[NC]) {
-        for (int w = 0; w < BIN_WAVES; ++w) {
-            if (wave == w && tail) {
-#pragma unroll
-                for (int ch = 0; ch < NC; ++ch) hist[ch * ns + key] += v1[ch];
-                if (key + 1 < ns) {
-#pragma unroll
-                    for (int ch = 0; ch < NC; ++ch) hist[ch * ns + key + 1] += v2[ch];
-                }
+                for (int ch = 0; ch < NC; ++ch) hist[ch * ns + lk] += ra[ch];
             }
             __syncthreads();
+            if (whole && lk + 1 < ns) {
+#pragma unroll
+                for (int ch = 0; ch < NC; ++ch) hist[ch * ns + lk + 1] += rb[ch];
+            }
+            __syncthreads();
+            const bool first = brk && fk == NOKEY;
+            const double kf = first ? 1.0 : 0.0, kr = brk ? 0.0 : 1.0;
+#pragma unroll
+            for (int ch = 0; ch < NC; ++ch) {
+                fa[ch] = fma(ra[ch], kf, fa[ch]);
+                fb[ch] = fma(rb[ch], kf, fb[ch]);
+            }
+            if (first) fk = lk;
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch) {
+                ra[ch] = fma(ra[ch], kr, w1 * val[ch]);
+                rb[ch] = fma(rb[ch], kr, w2 * val[ch]);
+            }
+            if constexpr (WCH) {
+                ra[NCH] = fma(ra[NCH], kr, w1 * c.nm);
+                rb[NCH] = fma(rb[NCH], kr, w2 * c.nm);
+            }
+            lk = l;
         }
+        // Runs across lanes: lanes whose open run has the same bin form one
+        // contiguous segment (bins do not decrease along the row), so a
+        // segmented scan over the open runs gives, at each lane, the sum of
+        // its bin's segment up to the lane's last mode.
+        seg_scan<NC>(lk, ra, rb);
+        const int head = fk != NOKEY ? fk : lk; // bin of the lane's first mode
+        const int nhead = __shfl_down(head, 1, 64);
+        const int head_next = lane == 63 ? NOKEY : nhead;
+        const int plk = __shfl_up(lk, 1, 64);
+        const int lk_prev = lane == 0 ? NOKEY : plk;
+        double pa[NC], pb[NC];
+#pragma unroll
+        for (int ch = 0; ch < NC; ++ch) {
+            pa[ch] = __shfl_up(ra[ch], 1, 64);
+            pb[ch] = __shfl_up(rb[ch], 1, 64);
+        }
+        // the first run closes in this lane and takes the segment before it;
+        // the open run closes here unless the next lane starts with its bin
+        const bool ta = fk != NOKEY;
+        const bool tb = lk != NOKEY && head_next != lk;
+        if (ta && lk_prev == fk) {
+#pragma unroll
+            for (int ch = 0; ch < NC; ++ch) {
+                fa[ch] += pa[ch];
+                fb[ch] += pb[ch];
+            }
+        }
+        if (ta && fk < ns) {
+#pragma unroll
+            for (int ch = 0; ch < NC; ++ch) hist[ch * ns + fk] += fa[ch];
+        }
+        if (tb && lk < ns) {
+#pragma unroll
+            for (int ch = 0; ch < NC; ++ch) hist[ch * ns + lk] += ra[ch];
+        }
+        __syncthreads();
+        if (ta && fk + 1 < ns) {
+#pragma unroll
+            for (int ch = 0; ch < NC; ++ch) hist[ch * ns + fk + 1] += fb[ch];
+        }
+        if (tb && lk + 1 < ns) {
+#pragma unroll
+            for (int ch = 0; ch < NC; ++ch) hist[ch * ns + lk + 1] += rb[ch];
+        }
+        __syncthreads();
     };
-    // each wave walks its own row; the block's BIN_WAVES rows advance together
-    for (int base = blockIdx.x * BIN_WAVES; base < rows; base += gridDim.x * BIN_WAVES) {
-        const int row = base + wave;
-        const bool has_row = row < rows;
-        const int rr = has_row ? row : 0;
-        const int j = rr % ny, k = rr / ny;
-        const int jj = (j <= ny / 2) ? j : ny - j;
-        const int kk = (k <= nz / 2) ? k : nz - k;
-        int key;
-        bool tail;
-        double v1[NC], v2[NC];
-        if constexpr (MODE == 3 || MODE == 4) {
-            for (int i0 = 0; i0 < nxh; i0 += 64) {
-                chunk(has_row, rr, j, k, jj, kk, i0, cpx<T>{0, 0}, cpx<T>{0, 0}, key, tail, v1, v2);
-                update(key, tail, v1, v2);
-            }
-        } else {
-            // the whole row's loads are issued first (up to BIN_ROWCHUNKS
-            // chunks of 64 modes), so their latency is paid once per row
-            cpx<T> a[BIN_ROWCHUNKS], b[BIN_ROWCHUNKS];
-            const cpx<T> *r1 = F1 + (size_t)rr * P;
-            const cpx<T> *r2 = MODE != 0 ? F2 + (size_t)rr * P : nullptr;
-#pragma unroll
-            for (int c = 0; c < BIN_ROWCHUNKS; ++c) {
-                const int i = c * 64 + lane;
-                a[c] = b[c] = cpx<T>{0, 0};
-                if (has_row && i < nxh) {
-                    a[c] = r1[i];
-                    if constexpr (MODE != 0) b[c] = r2[i];
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < BIN_ROWCHUNKS; ++c)
-                if (c * 64 < nxh) {
-                    chunk(has_row, rr, j, k, jj, kk, c * 64, a[c], b[c], key, tail, v1, v2);
-                    update(key, tail, v1, v2);
-                }
+    if constexpr (PF) {
+        Buf u0, u1;
+        int q = blockIdx.x;
+        if (q < ncls) load(u0, q);
+        for (; q < ncls; q += 2 * G) {
+            if (q + G < ncls) load(u1, q + G);
+            bin_class(u0, q);
+            if (q + G >= ncls) break;
+            if (q + 2 * G < ncls) load(u0, q + 2 * G);
+            bin_class(u1, q + G);
+        }
+    } else {
+        Buf u;
+        for (int q = blockIdx.x; q < ncls; q += G) {
+            load(u, q);
+            bin_class(u, q);
         }
     }
-    __syncthreads();
     double *dst = part + (size_t)blockIdx.x * NC * ns;
-    for (int q = threadIdx.x; q < NC * ns; q += 64 * BIN_WAVES) dst[q] = hist[q];
+    for (int q = lane; q < NC * ns; q += 64) dst[q] = hist[q];
 }
 
 // out[g][q] = sum over blocks b of group g (in order) of part[b][q]; with
@@ -299,6 +423,21 @@ static void bin_reduce(Ctx &c, int g, int nq, double *out)
     hipLaunchKernelGGL(k_bin_reduce, dim3((nq + 255) / 256, 1), dim3(256), 0, c.stream, grp, ngrp, ngrp, nq, out);
 }
 
+// Blocks of a binning launch: one wave each, with an LDS histogram of lds
+// bytes.  A whole number of rounds of the blocks the device holds at once (at
+// most BIN_BLOCKS) and no more blocks than row classes; the block count fixes
+// the order of the additions, so every caller of a kernel uses this one.
+template <typename K>
+static int bin_grid(const Ctx &c, K kern, size_t lds)
+{
+    const int ncls = (c.ny / 2 + 1) * (c.nz / 2 + 1);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, lds) != hipSuccess || per_cu < 1) per_cu = 1;
+    const int res = per_cu * (c.cus > 0 ? c.cus : 1);
+    const int g = res >= BIN_BLOCKS ? BIN_BLOCKS : (BIN_BLOCKS / res) * res;
+    return g < ncls ? g : ncls;
+}
+
 template <typename T>
 int launch_bin(Ctx &c, const cpx<T> *F1, const cpx<T> *F2, int mode, int nbins, int flags)
 {
@@ -308,10 +447,11 @@ int launch_bin(Ctx &c, const cpx<T> *F1, const cpx<T> *F2, int mode, int nbins, 
         return -1;
     }
     const int nch = (mode == 0 || mode == 3) ? 1 : (mode == 1 ? 2 : 4);
-    const int rows = c.ny * c.nz;
-    const int gr = (rows + BIN_WAVES - 1) / BIN_WAVES; // row groups
-    const int g = gr < BIN_BLOCKS ? gr : BIN_BLOCKS;
-    if (mode != 3 && c.nxh > BIN_ROWCHUNKS * 64) {
+    if (sizeof(double) * nch * ns > BIN_LDS_MAX) {
+        c.last_error = "spectrum bin count exceeds the binning kernel's LDS histogram";
+        return -1;
+    }
+    if ((c.nxh - 1 + 63) / 64 + 1 > BIN_LANEMODES) {
         c.last_error = "spectrum rows longer than the binning kernel's row buffer";
         return -1;
     }
@@ -326,26 +466,34 @@ int launch_bin(Ctx &c, const cpx<T> *F1, const cpx<T> *F2, int mode, int nbins, 
         slot = owner.wnext;
         owner.wnext = (owner.wnext + 1) % Ctx::WCACHE;
         owner.wkey[slot] = -1;
-        hipLaunchKernelGGL((k_bin<T, 0, 4, true>), dim3(g), dim3(64 * BIN_WAVES), sizeof(double) * ns, c.stream, F1, F2,
-                           c.nx, c.ny, c.nz, c.nxh, c.P, ns, c.spec_part, flags, c.nk);
+        const size_t lw = sizeof(double) * ns;
+        const int g = bin_grid(c, k_bin<T, 0, 4, true>, lw);
+        hipLaunchKernelGGL((k_bin<T, 0, 4, true>), dim3(g), dim3(64), lw, c.stream, F1, F2, c.nx, c.ny, c.nz, c.nxh,
+                           c.P, ns, c.spec_part, flags, c.nk);
         LAT_HIP(&c, hipGetLastError());
         bin_reduce(c, g, ns, c.spec_out + (5 + slot) * cap);
         LAT_HIP(&c, hipGetLastError());
         owner.wkey[slot] = key;
     }
     const size_t lds = sizeof(double) * nch * ns;
-    if (mode == 0)
-        hipLaunchKernelGGL((k_bin<T, 1, 0, false>), dim3(g), dim3(64 * BIN_WAVES), lds, c.stream, F1, F2, c.nx, c.ny, c.nz,
-                           c.nxh, c.P, ns, c.spec_part, flags, c.nk);
-    else if (mode == 1)
-        hipLaunchKernelGGL((k_bin<T, 2, 1, false>), dim3(g), dim3(64 * BIN_WAVES), lds, c.stream, F1, F2, c.nx, c.ny, c.nz,
-                           c.nxh, c.P, ns, c.spec_part, flags, c.nk);
-    else if (mode == 2)
-        hipLaunchKernelGGL((k_bin<T, 4, 2, false>), dim3(g), dim3(64 * BIN_WAVES), lds, c.stream, F1, F2, c.nx, c.ny, c.nz,
-                           c.nxh, c.P, ns, c.spec_part, flags, c.nk);
-    else
-        hipLaunchKernelGGL((k_bin<T, 1, 3, false>), dim3(g), dim3(64 * BIN_WAVES), lds, c.stream, F1, F2, c.nx, c.ny, c.nz,
-                           c.nxh, c.P, ns, c.spec_part, flags, c.nk);
+    int g = 0;
+    if (mode == 0) {
+        g = bin_grid(c, k_bin<T, 1, 0, false>, lds);
+        hipLaunchKernelGGL((k_bin<T, 1, 0, false>), dim3(g), dim3(64), lds, c.stream, F1, F2, c.nx, c.ny, c.nz, c.nxh,
+                           c.P, ns, c.spec_part, flags, c.nk);
+    } else if (mode == 1) {
+        g = bin_grid(c, k_bin<T, 2, 1, false>, lds);
+        hipLaunchKernelGGL((k_bin<T, 2, 1, false>), dim3(g), dim3(64), lds, c.stream, F1, F2, c.nx, c.ny, c.nz, c.nxh,
+                           c.P, ns, c.spec_part, flags, c.nk);
+    } else if (mode == 2) {
+        g = bin_grid(c, k_bin<T, 4, 2, false>, lds);
+        hipLaunchKernelGGL((k_bin<T, 4, 2, false>), dim3(g), dim3(64), lds, c.stream, F1, F2, c.nx, c.ny, c.nz, c.nxh,
+                           c.P, ns, c.spec_part, flags, c.nk);
+    } else {
+        g = bin_grid(c, k_bin<T, 1, 3, false>, lds);
+        hipLaunchKernelGGL((k_bin<T, 1, 3, false>), dim3(g), dim3(64), lds, c.stream, F1, F2, c.nx, c.ny, c.nz, c.nxh,
+                           c.P, ns, c.spec_part, flags, c.nk);
+    }
     LAT_HIP(&c, hipGetLastError());
     bin_reduce(c, g, nch * ns, c.spec_out);
     LAT_HIP(&c, hipGetLastError());
@@ -404,9 +552,16 @@ int spectrum_discrete_k_noalloc(Ctx &c, const double *Fk_dev, double *S_dev, int
         return -2;
     }
     const int ns = lat_num_bins_discrete(c.nx);
-    const int rows = c.ny * c.nz;
-    const int gr = (rows + BIN_WAVES - 1) / BIN_WAVES;
-    const int g = gr < BIN_BLOCKS ? gr : BIN_BLOCKS;
+    const size_t lds = sizeof(double) * ns;
+    if (lds > BIN_LDS_MAX) {
+        c.last_error = "spectrum bin count exceeds the binning kernel's LDS histogram";
+        return -2;
+    }
+    // the grids of launch_bin for the same kernels (same additions in the
+    // same order)
+    const int gw = bin_grid(c, k_bin<double, 0, 4, true>, lds);
+    const int gv = bin_grid(c, k_bin<double, 1, 0, false>, lds);
+    const int g = gw > gv ? gw : gv;
     const int ngrp = (g + BIN_GROUP - 1) / BIN_GROUP;
     const size_t need = ((size_t)g + ngrp + 2) * ns;
     if (need > NOALLOC_DOUBLES) {
@@ -414,7 +569,7 @@ int spectrum_discrete_k_noalloc(Ctx &c, const double *Fk_dev, double *S_dev, int
                        " bytes needed, " + std::to_string(NOALLOC_DOUBLES * 8) + " held)";
         return -2;
     }
-    if (c.nxh > BIN_ROWCHUNKS * 64) {
+    if ((c.nxh - 1 + 63) / 64 + 1 > BIN_LANEMODES) {
         c.last_error = "spectrum rows longer than the binning kernel's row buffer";
         return -2;
     }
@@ -425,20 +580,20 @@ int spectrum_discrete_k_noalloc(Ctx &c, const double *Fk_dev, double *S_dev, int
     const int flags = kind == 0 ? BIN_HALF : (BIN_HALF | BIN_NO_NYQ | BIN_COUNTS);
     const cpx<double> *F = (const cpx<double> *)Fk_dev;
     const int P = c.nxh; // FFTW layout: rows of nx/2 + 1, unpadded
-    auto reduce = [&](double *out) {
-        hipLaunchKernelGGL(k_bin_reduce, dim3((ns + 255) / 256, ngrp), dim3(256), 0, c.stream, part, g, BIN_GROUP,
-                           ns, grp);
-        hipLaunchKernelGGL(k_bin_reduce, dim3((ns + 255) / 256, 1), dim3(256), 0, c.stream, grp, ngrp, ngrp, ns,
-                           out);
+    auto reduce = [&](int nb, double *out) {
+        const int ng = (nb + BIN_GROUP - 1) / BIN_GROUP;
+        hipLaunchKernelGGL(k_bin_reduce, dim3((ns + 255) / 256, ng), dim3(256), 0, c.stream, part, nb, BIN_GROUP, ns,
+                           grp);
+        hipLaunchKernelGGL(k_bin_reduce, dim3((ns + 255) / 256, 1), dim3(256), 0, c.stream, grp, ng, ng, ns, out);
     };
     // the weight channel, then the value channel, with the kernels, grid and
-    // reduction order of launch_bin (same additions in the same order)
-    hipLaunchKernelGGL((k_bin<double, 0, 4, true>), dim3(g), dim3(64 * BIN_WAVES), sizeof(double) * ns, c.stream, F, F,
-                       c.nx, c.ny, c.nz, c.nxh, P, ns, part, flags, c.nk);
-    reduce(W);
-    hipLaunchKernelGGL((k_bin<double, 1, 0, false>), dim3(g), dim3(64 * BIN_WAVES), sizeof(double) * ns, c.stream, F,
-                       F, c.nx, c.ny, c.nz, c.nxh, P, ns, part, flags, c.nk);
-    reduce(raw);
+    // reduction order of launch_bin
+    hipLaunchKernelGGL((k_bin<double, 0, 4, true>), dim3(gw), dim3(64), lds, c.stream, F, F, c.nx, c.ny, c.nz, c.nxh,
+                       P, ns, part, flags, c.nk);
+    reduce(gw, W);
+    hipLaunchKernelGGL((k_bin<double, 1, 0, false>), dim3(gv), dim3(64), lds, c.stream, F, F, c.nx, c.ny, c.nz, c.nxh,
+                       P, ns, part, flags, c.nk);
+    reduce(gv, raw);
     const double n3 = (double)c.nvol;
     hipLaunchKernelGGL(k_spec_finish, dim3((ns + 255) / 256), dim3(256), 0, c.stream, raw, W, ns,
                        kind == 0 ? n3 * n3 : 1.0, S_dev);
