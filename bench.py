@@ -194,22 +194,31 @@ def spectrum_error(sa, sb, floor):
 
 
 def corrector_leg(la, args):
-    """The learned corrector, one per coupling: train at --corr-train-n on
-    pairs the plain integrator generates, then run at --corr-n with coarse
-    steps of --corr-cf times dt (leapfrog + predicted residual).  Reported against the
-    plain Yoshida-4 run over the same time: the field error of leapfrog
-    alone, of the always-accepted corrected run, and of two guarded runs, in
-    which a step whose Friedmann-residual change exceeds --corr-tol times
-    the plain run's average change per coarse interval, or the fixed bound
-    --corr-bound, is replaced by --corr-cf Yoshida-4 steps; cost in force
-    evaluations (coarse step 1, Yoshida-4 3).
-    Errors are relative to the fluctuation norm of the plain run's fields
-    and momenta; non-finite entries mean the coarse leapfrog step is beyond
-    its stability limit at that coupling."""
-    g2s = [float(x) for x in args.corr_g2.split(",")]
+    """The learned corrector as config 4 states it: one network trained on
+    --corr-pairs snapshot pairs that the plain integrator generates at
+    --corr-train-n (L scaled so that dx and dt equal the evaluation
+    lattice's: the per-site dynamics are the same, only the box is smaller),
+    along trajectories with g^2/m^2 log-uniform in [1e4, 1e6] and their own
+    seeds, --corr-per-traj pairs each (a coupling whose coarse leapfrog step
+    is not finite gives no pairs and its trajectory is dropped after one);
+    then applied at --corr-n with coarse steps of --corr-cf times dt.
+    Reported: a learning curve (held-out one-step error of the corrected
+    coarse step over that of the leapfrog alone, against the plain
+    Yoshida-4 step, on states the training never saw); and per evaluation
+    coupling, over --corr-steps coarse steps, the trajectory error of the
+    leapfrog alone, of the always-accepted corrector, of the corrector
+    under the 1e-6 constraint bound (plain fallback) and under the judge
+    (every --corr-judge-th step re-taken by the plain integrator, switched
+    off above --corr-err-tol), with force evaluations (coarse step 1,
+    Yoshida-4 step 3) against the plain run.  Errors are relative to the
+    fluctuation norm of the plain run, for fields / momenta."""
     cf = args.corr_cf
+    n, ntr = args.corr_n, args.corr_train_n
+    dt = (10.0 / n) / 40.0
+    ltr = 10.0 * ntr / n
+    rng = np.random.default_rng(args.corr_seed)
     out = {"metric": "learned corrector: error and force evaluations against plain Yoshida-4",
-           "coarse_factor": cf, "tol": args.corr_tol, "runs": []}
+           "coarse_factor": cf, "train_n": ntr, "train_L": ltr, "eval_n": n, "dt": dt}
 
     def fluct_err(st, ref):
         e = []
@@ -219,29 +228,99 @@ def corrector_leg(la, args):
             e.append(float(d / r))
         return e
 
-    n = args.corr_n
-    dt = (10.0 / n) / 40.0
-    ntr = args.corr_train_n
-    dt_tr = (10.0 / ntr) / 40.0
-    for q, g2 in enumerate(g2s):
-        t0 = time.perf_counter()
-        run = {"g2": g2}
-        with la.Corrector(seed=7 + q) as corr:
-            with la.Lattice(ntr, nfld=2, length=10.0, model_params=(1.0, g2)) as lat:
-                init_state(lat, seed=1000 + q, g2=g2)
-                lat.symp4(dt_tr, 8)
-                losses = corr.train(lat, dt_tr, cf, npairs=args.corr_pairs, lr=args.corr_lr)
-            ok = np.isfinite(losses)
-            run["train"] = {"n": ntr, "pairs": args.corr_pairs, "usable_pairs": int(ok.sum()),
-                            "seconds": round(time.perf_counter() - t0, 1)}
-            if not ok.any():
-                # every coarse step was beyond the leapfrog's stability limit
-                run["train"]["note"] = "coarse leapfrog step not finite at this coupling; nothing to learn"
-                out["runs"].append(run)
-                continue
-            good = losses[ok]
-            run["train"].update({"loss_first": float(np.mean(good[:5])), "loss_last": float(np.mean(good[-5:])),
-                                 "ratio": corr.get_norm().tolist()})
+    # held-out states: own seeds, couplings where the coarse step is stable
+    val = [(float(g), 90000 + q, w) for q, (g, w) in enumerate([(1e4, 40), (2e4, 80), (4e4, 40), (1e4, 160)])]
+
+    def one_step_errors(corr):
+        res = []
+        for g2, seed, warm in val:
+            with la.Lattice(ntr, nfld=2, length=ltr, model_params=(1.0, g2)) as lat:
+                init_state(lat, seed=seed, g2=g2)
+                lat.symp4(dt, warm)
+                start = lat.get_state()
+                lat.symp4(dt, cf)
+                fine = lat.get_state()
+                lat.set_state(*start)
+                lat.symp(2, cf * dt, 1)
+                leap = lat.get_state()
+                lat.set_state(*start)
+                corr.step(lat, dt, cf, nsteps=1, threshold=1e300)
+                cor = lat.get_state()
+            el, ec = fluct_err(leap, fine), fluct_err(cor, fine)
+            res.append({"g2": g2, "warm_steps": warm, "err_leapfrog": el, "err_corrected": ec,
+                        "ratio": [ec[0] / el[0], ec[1] / el[1]]})
+        return res
+
+    # the coarse leapfrog step is stable while (k_max^2 + g^2 phi^2) (cf dt)^2
+    # < 4, k_max^2 = 3 (pi / dx)^2 for the spectral Laplacian and |phi| up to
+    # PHI0: couplings above g2max have no usable coarse step
+    g2max = (4.0 - 3.0 * math.pi ** 2 * (cf / 40.0) ** 2) / ((cf * dt) ** 2 * PHI0 ** 2)
+    out["g2_stable_max"] = g2max
+    t0 = time.perf_counter()
+    curve, trajs, used = [], [], 0
+    checkpoints = sorted({int(args.corr_pairs * f) for f in (0.05, 0.1, 0.25, 0.5, 0.75, 1.0)})
+    nxt = 0
+
+    def new_stream(q, g2=None):
+        if g2 is None:
+            g2 = float(np.exp(rng.uniform(np.log(1e4), np.log(min(g2max, 1e6)))))
+        lat = la.Lattice(ntr, nfld=2, length=ltr, model_params=(1.0, g2))
+        init_state(lat, seed=50000 + q, g2=g2)
+        warm = int(rng.integers(0, 400))
+        if warm:
+            lat.symp4(dt, warm)
+        return {"lat": lat, "g2": g2, "warm": warm, "pairs": 0, "losses": []}
+
+    with la.Corrector(seed=args.corr_seed) as corr:
+        # interleaved trajectories, one pair from each in turn; the first
+        # (which fixes the target scale) at the middle of the stable range
+        nq = 0
+        streams = []
+        for k in range(args.corr_streams):
+            streams.append(new_stream(nq, g2=math.sqrt(1e4 * min(g2max, 1e6)) if k == 0 else None))
+            nq += 1
+        while used < args.corr_pairs:
+            for k, st in enumerate(streams):
+                if used >= args.corr_pairs:
+                    break
+                # learning rate decaying exponentially to a tenth over the run
+                lr = args.corr_lr * 0.1 ** (used / args.corr_pairs)
+                loss = corr.train(st["lat"], dt, cf, npairs=1, lr=lr)[0]
+                if np.isfinite(loss):
+                    used += 1
+                    st["pairs"] += 1
+                    st["losses"].append(float(loss))
+                if not np.isfinite(loss) or st["pairs"] >= args.corr_per_traj:
+                    trajs.append({"g2": st["g2"], "warm": st["warm"], "pairs": st["pairs"],
+                                  "loss_first": float(np.mean(st["losses"][:5])) if st["losses"] else None,
+                                  "loss_last": float(np.mean(st["losses"][-5:])) if st["losses"] else None})
+                    st["lat"].close()
+                    streams[k] = new_stream(nq)
+                    nq += 1
+                if used >= nxt:
+                    nxt += 500
+                    print(f"[bench] corrector: {used} pairs, {time.perf_counter() - t0:.0f} s", file=sys.stderr,
+                          flush=True)
+                if checkpoints and used >= checkpoints[0]:
+                    checkpoints.pop(0)
+                    ev = one_step_errors(corr)
+                    curve.append({"pairs": used, "seconds": round(time.perf_counter() - t0, 1),
+                                  "median_ratio_fields": float(np.median([r["ratio"][0] for r in ev])),
+                                  "median_ratio_momenta": float(np.median([r["ratio"][1] for r in ev])),
+                                  "states": ev})
+                    print(f"[bench] corrector: {used} pairs, held-out ratio "
+                          f"{curve[-1]['median_ratio_fields']:.3f} / {curve[-1]['median_ratio_momenta']:.3f}",
+                          file=sys.stderr, flush=True)
+        for st in streams:
+            st["lat"].close()
+        stable = [t["g2"] for t in trajs if t["pairs"] > 0]
+        out["train"] = {"pairs": used, "trajectories": len(trajs), "usable_trajectories": len(stable),
+                        "largest_usable_g2": max(stable) if stable else None,
+                        "seconds": round(time.perf_counter() - t0, 1), "ratio": corr.get_norm().tolist()}
+        out["learning_curve"] = curve
+        runs = []
+        for q, g2 in enumerate(float(x) for x in args.corr_g2.split(",")):
+            run = {"g2": g2}
             with la.Lattice(n, nfld=2, length=10.0, model_params=(1.0, g2)) as lat:
                 init_state(lat, seed=2000 + q, g2=g2)
                 e0 = lat.energies()["resid"]
@@ -255,30 +334,28 @@ def corrector_leg(la, args):
                 lat.set_state(*start)
                 corr.step(lat, dt, cf, nsteps=args.corr_steps, threshold=1e300)
                 always = lat.get_state()
-                d_always = lat.energies()["resid"] - e0
-                lat.set_state(*leap)
-                d_leap = lat.energies()["resid"] - e0
-                thr = args.corr_tol * abs(d_plain) / args.corr_steps
-                lat.set_state(*start)
-                st = corr.step(lat, dt, cf, nsteps=args.corr_steps, threshold=thr)
-                guarded = lat.get_state()
-                # the same guard at a fixed bound on the relative Friedmann
-                # residual (change per coarse step)
                 lat.set_state(*start)
                 st_b = corr.step(lat, dt, cf, nsteps=args.corr_steps, threshold=args.corr_bound)
                 bounded = lat.get_state()
-                d_bound = lat.energies()["resid"] - e0
+                lat.set_state(*start)
+                st_j = corr.step_judged(lat, dt, cf, nsteps=args.corr_steps, threshold=args.corr_bound,
+                                        judge_every=args.corr_judge, err_tol=args.corr_err_tol)
+                judged = lat.get_state()
             plain = 3.0 * cf * args.corr_steps
-            run["eval"] = {"n": n, "coarse_steps": args.corr_steps, "plain_resid_change": d_plain,
-                           "leapfrog_resid_change": d_leap, "corrected_resid_change": d_always,
-                           "threshold": thr, "err_leapfrog": fluct_err(leap, ref),
-                           "err_corrected": fluct_err(always, ref), "err_guarded": fluct_err(guarded, ref),
-                           "fallbacks": st["fallbacks"], "force_evals": st["force_evals"],
-                           "plain_force_evals": plain, "saving": plain / st["force_evals"],
-                           "bound": args.corr_bound, "bound_fallbacks": st_b["fallbacks"],
-                           "bound_force_evals": st_b["force_evals"], "bound_saving": plain / st_b["force_evals"],
-                           "bound_resid_change": d_bound, "err_bound": fluct_err(bounded, ref)}
-        out["runs"].append(run)
+            run.update({"coarse_steps": args.corr_steps, "plain_resid_change": d_plain,
+                        "err_leapfrog": fluct_err(leap, ref), "err_corrected": fluct_err(always, ref),
+                        "bound": args.corr_bound, "bound_fallbacks": st_b["fallbacks"],
+                        "bound_saving": plain / st_b["force_evals"], "err_bound": fluct_err(bounded, ref),
+                        "judge_every": args.corr_judge, "err_tol": args.corr_err_tol,
+                        "judged": st_j["judged"], "max_judged_error": st_j["max_judged_error"],
+                        "max_judged_leapfrog_error": st_j["max_judged_leapfrog_error"],
+                        "switched_off_at": st_j["switched_off_at"], "judged_saving": plain / st_j["force_evals"],
+                        "err_judged": fluct_err(judged, ref)})
+            runs.append(run)
+            print(f"[bench] corrector eval g2={g2:g}: leapfrog {run['err_leapfrog']}, corrected "
+                  f"{run['err_corrected']}, judged {run['err_judged']} saving {run['judged_saving']:.2f}",
+                  file=sys.stderr, flush=True)
+        out["eval"] = runs
     print(json.dumps(out))
 
 
@@ -442,10 +519,18 @@ def main():
     ap.add_argument("--corrector", action="store_true", help="only run the learned-corrector leg")
     ap.add_argument("--corr-train-n", type=int, default=128)
     ap.add_argument("--corr-n", type=int, default=256)
-    ap.add_argument("--corr-pairs", type=int, default=300)
+    ap.add_argument("--corr-pairs", type=int, default=20000, help="training pairs in all")
+    ap.add_argument("--corr-per-traj", type=int, default=100, help="training pairs per trajectory")
+    ap.add_argument("--corr-streams", type=int, default=8, help="trajectories trained on in turn")
+    ap.add_argument("--corr-seed", type=int, default=7)
     ap.add_argument("--corr-steps", type=int, default=20)
-    ap.add_argument("--corr-lr", type=float, default=1e-3)
-    ap.add_argument("--corr-cf", type=int, default=4, help="coarse step = this many fine steps")
+    ap.add_argument("--corr-lr", type=float, default=3e-4)
+    ap.add_argument("--corr-cf", type=int, default=8, help="coarse step = this many fine steps")
+    ap.add_argument("--corr-judge", type=int, default=5,
+                    help="judged run: every this many accepted corrected steps is re-taken by the plain integrator")
+    ap.add_argument("--corr-err-tol", type=float, default=0.5,
+                    help="judged run: switch the corrector off when a judged one-step error exceeds this (or the "
+                         "leapfrog step's)")
     ap.add_argument("--ensemble", action="store_true", help="only run the batched-ensemble leg")
     ap.add_argument("--gw", action="store_true", help="only run the leg with the six GW components")
     ap.add_argument("--config1", action="store_true",
@@ -457,12 +542,8 @@ def main():
     ap.add_argument("--ens-steps", type=int, default=20)
     ap.add_argument("--ens-form", default="mixed", choices=["mixed", "f64"])
     ap.add_argument("--corr-bound", type=float, default=1e-6,
-                    help="second guarded run: fall back when a coarse step moves the relative Friedmann residual "
-                         "by more than this")
-    ap.add_argument("--corr-tol", type=float, default=10.0,
-                    help="accept a corrected step whose constraint change is within this multiple of the plain "
-                         "run's average change per coarse interval")
-    ap.add_argument("--corr-g2", default="1e4,1e5,1e6")
+                    help="fall back when a coarse step moves the relative Friedmann residual by more than this")
+    ap.add_argument("--corr-g2", default="1e4,2e4,3e4,4e4,6e4,1e5,1e6")
     args = ap.parse_args()
     if args.corrector:
         import lattice_amd as la

@@ -279,7 +279,7 @@ int  lat_checkpoint(lat_ctx *ctx, const char *dir, double time, int outframenum)
 int  lat_restore(lat_ctx *ctx, const char *dir, double *time, int *outframenum);
 
 /* ---- learned corrector (no reference counterpart) ----
- * A 4-layer 3x3x3 periodic convolutional network (4 -> 32 -> 32 -> 32 -> 4
+ * A 4-layer 3x3x3 periodic convolutional network (8 -> 32 -> 32 -> 32 -> 4
  * channels, SiLU, BF16 weights and activations, FP32 accumulation on MFMA)
  * that maps the state after one coarse leapfrog step of coarse_factor * dt to
  * the residual of coarse_factor Yoshida-4 steps of dt.  It is trained on the
@@ -289,8 +289,11 @@ int  lat_restore(lat_ctx *ctx, const char *dir, double *time, int *outframenum);
  * Contexts: single, two-field, FP64, cubic with n a multiple of 16.
  * Weights of layer l: cout x 27 cin with k = tap * cin + ci, tap =
  * (dz+1)*9 + (dy+1)*3 + (dx+1) for the neighbour at (x+dx, y+dy, z+dz);
- * channels (phi, chi, pi_phi, pi_chi).  The network sees and corrects
- * fluctuations only, at unit scale: input (v - <v>) / s_v, target
+ * input channels (phi, chi, pi_phi, pi_chi, <phi>, <pi_phi>, log10(g^2/m^2) - 5,
+ * log10(s_chi) / 10), output channels (phi, chi, pi_phi, pi_chi).  The
+ * network sees and corrects fluctuations only, at unit scale (the last four
+ * input channels are uniform: what the normalisation removes and the
+ * dynamics depend on): input (v - <v>) / s_v, target
  * (r - <r>) / (ratio s_v) per channel, with <.> the spatial mean and s_v the
  * spatial standard deviation of the coarse state at hand; a corrected step
  * adds ratio s_v (y - <y>), leaving the homogeneous mode at the coarse step's
@@ -313,6 +316,19 @@ int  lat_corrector_train(lat_corrector *corr, lat_ctx *ctx, double dt, int coars
 /* stats[3] = fallbacks, largest accepted constraint change, force evaluations */
 int  lat_corrector_step(lat_corrector *corr, lat_ctx *ctx, double dt, int coarse_factor, int nsteps,
                         double threshold, double *stats);
+/* lat_corrector_step with the plain integrator as judge of the error: every
+ * judge_every-th accepted corrected step is also taken by the plain
+ * integrator from the same start; the errors of the corrected step and of
+ * the leapfrog step alone against it (per channel, RMS difference over the
+ * plain step's spatial standard deviation, the largest over the four
+ * channels) are measured and the plain step is kept.  Once a judged
+ * corrected step's error exceeds err_tol or the leapfrog step's, the
+ * corrector is switched off for the rest of the call.  stats[7] = fallbacks,
+ * force evaluations, judged steps, largest judged error (corrected), step
+ * index at which it was switched off (-1: never), largest accepted
+ * constraint change, largest judged error (leapfrog alone). */
+int  lat_corrector_step_judged(lat_corrector *corr, lat_ctx *ctx, double dt, int coarse_factor, int nsteps,
+                               double threshold, int judge_every, double err_tol, double *stats);
 
 /* ---- state (module Hamiltonian globals fld, fldp, yscl, ysclp,
  *      hamiltonian_conformal.f90:22-24; hamiltonian_cosmic.f90:27,31) ---- */

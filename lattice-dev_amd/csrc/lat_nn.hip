@@ -1,18 +1,23 @@
 // lat_nn.hip -- the learned corrector: a 3-D convolutional network on MFMA.
 //
-// Network: 4 layers of 3x3x3 periodic convolutions, 4 -> W -> W -> W -> 4
+// Network: 4 layers of 3x3x3 periodic convolutions, 8 -> W -> W -> W -> 4
 // channels (W = 32), SiLU between layers, BF16 weights and activations, FP32
-// accumulation on the matrix cores (v_mfma_f32_16x16x16_bf16).  Input: the
-// normalised (phi, chi, pi_phi, pi_chi) after one coarse leapfrog step;
-// output: the normalised residual to the fine Yoshida-4 trajectory.
+// accumulation on the matrix cores (gfx950's v_mfma_f32_16x16x32_bf16).  Input: the
+// normalised (phi, chi, pi_phi, pi_chi) fluctuations after one coarse
+// leapfrog step and four uniform channels that carry what the normalisation
+// removes and the dynamics depend on (the homogeneous phi and pi_phi, the
+// coupling, the chi fluctuation amplitude); output: the normalised residual
+// to the fine Yoshida-4 trajectory.
 //
 // Layout: activations are site-major with channels fastest, X[site][C]
 // (C a multiple of 4), so the four consecutive reduction indices an MFMA
 // lane supplies are one 8-byte load.  A convolution is an implicit GEMM
 //   Y[s][co] = sum_k W[co][k] X[nbr(s, tap(k))][ci(k)],  k = tap * CIN + ci,
 // with M = output channels (16-row tiles), N = 16 sites per wave tile and
-// K = 27 CIN padded to a multiple of 16.  The weight matrix of a layer is
-// staged in LDS once per workgroup.
+// K = 27 CIN padded to a multiple of 32, 32 reduction indices per MFMA (each
+// lane supplies 8 consecutive ones: 8 channels of one tap, or 4 channels of
+// two taps when CIN = 4).  The weight matrix of a layer is staged in LDS
+// once per workgroup.
 //
 // Backward: the data gradient is the same convolution with the transposed,
 // tap-reversed weights Wt[ci][t * COUT + co] = W[co][(26 - t) * CIN + ci];
@@ -26,6 +31,8 @@ namespace lat {
 namespace nn {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short bf16;
 
@@ -45,7 +52,16 @@ __device__ inline float dsilu(float v)
     return s * (1.0f + v * (1.0f - s));
 }
 
-constexpr int KPAD(int cin) { return (27 * cin + 15) / 16 * 16; }
+constexpr int KPAD(int cin) { return (27 * cin + 31) / 32 * 32; }
+
+// D (16 x 16, FP32) += A (16 x 32) B (32 x 16) in BF16; lane l supplies
+// A[l % 16][8 (l / 16) .. +7] and B[8 (l / 16) .. +7][l % 16] and holds
+// D[4 (l / 16) + e][l % 16], e = 0..3
+__device__ __forceinline__ f32x4 mfma32(s16x8 a, s16x8 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                   0, 0, 0);
+}
 
 // epilogue modes of the convolution
 enum { CONV_SILU = 0,  // Z = acc + b (BF16), Y = silu(Z) (BF16)
@@ -63,38 +79,50 @@ __device__ inline size_t nbr_site(int x, int y, int z, int t, int n)
     return (size_t)xx + (size_t)n * ((size_t)yy + (size_t)n * zz);
 }
 
-// One wave per 16-site tile; 4 waves per workgroup; grid-stride over tiles.
+// One wave per 16-site tile; CONV_WAVES waves per workgroup share the LDS
+// copy of the weights; grid-stride over tiles.
 // W: [MT*16][KP] BF16 (rows >= COUT and k >= 27*CIN are zero).
+constexpr int CONV_WAVES = 8;
 template <int CIN, int COUT, int MODE>
-__global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ X, const bf16 *__restrict__ W,
+__global__ __launch_bounds__(64 * CONV_WAVES) void k_conv(const bf16 *__restrict__ X, const bf16 *__restrict__ W,
                                               const float *__restrict__ bias, bf16 *__restrict__ Y,
                                               float *__restrict__ Yf, bf16 *__restrict__ Z,
                                               const bf16 *__restrict__ Zin, int n)
 {
     constexpr int MT = (COUT + 15) / 16;
     constexpr int KP = KPAD(CIN);
-    __shared__ bf16 ws[MT * 16 * KP];
-    for (int q = threadIdx.x; q < MT * 16 * KP / 4; q += 256)
-        reinterpret_cast<bf16x4 *>(ws)[q] = reinterpret_cast<const bf16x4 *>(W)[q];
+    __shared__ __attribute__((aligned(16))) bf16 ws[MT * 16 * KP];
+    for (int q = threadIdx.x; q < MT * 16 * KP / 8; q += 64 * CONV_WAVES)
+        reinterpret_cast<s16x8 *>(ws)[q] = reinterpret_cast<const s16x8 *>(W)[q];
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int j = lane & 15, kg = lane >> 4;
     const size_t nvol = (size_t)n * n * n, ntile = nvol / 16;
-    for (size_t tile = (size_t)blockIdx.x * 4 + wave; tile < ntile; tile += (size_t)gridDim.x * 4) {
+    for (size_t tile = (size_t)blockIdx.x * CONV_WAVES + wave; tile < ntile;
+         tile += (size_t)gridDim.x * CONV_WAVES) {
         const size_t s = tile * 16 + j;
         const int x = (int)(s % n), y = (int)((s / n) % n), z = (int)(s / ((size_t)n * n));
         f32x4 acc[MT];
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
-        for (int k0 = 0; k0 < KP; k0 += 16) {
-            const int k = k0 + 4 * kg;
-            const int t = k / CIN, ci = k - t * CIN;
-            bf16x4 b = bf16x4{0, 0, 0, 0};
-            if (t < 27) b = *reinterpret_cast<const bf16x4 *>(X + nbr_site(x, y, z, t, n) * CIN + ci);
+        for (int k0 = 0; k0 < KP; k0 += 32) {
+            const int k = k0 + 8 * kg;
+            s16x8 b = s16x8{0, 0, 0, 0, 0, 0, 0, 0};
+            if constexpr (CIN % 8 == 0) {
+                const int t = k / CIN, ci = k - t * CIN;
+                if (t < 27) b = *reinterpret_cast<const s16x8 *>(X + nbr_site(x, y, z, t, n) * CIN + ci);
+            } else {
+                static_assert(CIN == 4, "4 or a multiple of 8 input channels");
+                const int t = k / 4; // taps t and t + 1, channels 0..3 of each
+                bf16x4 lo = bf16x4{0, 0, 0, 0}, hi = bf16x4{0, 0, 0, 0};
+                if (t < 27) lo = *reinterpret_cast<const bf16x4 *>(X + nbr_site(x, y, z, t, n) * 4);
+                if (t + 1 < 27) hi = *reinterpret_cast<const bf16x4 *>(X + nbr_site(x, y, z, t + 1, n) * 4);
+                b = s16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+            }
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
-                const bf16x4 a = *reinterpret_cast<const bf16x4 *>(ws + (mt * 16 + j) * KP + k);
-                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[mt], 0, 0, 0);
+                const s16x8 a = *reinterpret_cast<const s16x8 *>(ws + (mt * 16 + j) * KP + k);
+                acc[mt] = mfma32(a, b, acc[mt]);
             }
         }
         // lane holds D[co][site] for co = mt*16 + 4*kg + e, site = tile*16 + j
@@ -133,7 +161,9 @@ __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ X, const 
 }
 
 // Weight gradient: dW[co][t*CIN + ci] = sum_s G[s][co] X[nbr(s,t)][ci] over a
-// chunk of CHUNK sites per wave; one wave per (co tile, column tile, chunk).
+// chunk of GCHUNK sites per wave, 32 sites per MFMA (a multiple of 32 sites
+// divides nvol, n being a multiple of 16); one wave per (co tile, column
+// tile, chunk).
 // part: [nchunk][MT*16][KP] FP32.  Also db[co] partial in column KP (part has
 // KP + 16 columns: the bias gradient uses a column of ones).
 constexpr int GCHUNK = 4096;
@@ -154,11 +184,11 @@ __global__ __launch_bounds__(64) void k_conv_dw(const bf16 *__restrict__ G, cons
     const bool bias_col = ct == CT - 1;
     const int t = col / CIN, ci = col - t * CIN;
     f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (size_t sb = s0; sb < s0 + GCHUNK && sb < nvol; sb += 16) {
-        bf16x4 a, b;
+    for (size_t sb = s0; sb < s0 + GCHUNK && sb < nvol; sb += 32) {
+        s16x8 a, b;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const size_t s = sb + 4 * kg + q;
+        for (int q = 0; q < 8; ++q) {
+            const size_t s = sb + 8 * kg + q;
             a[q] = (short)((co < COUT) ? G[s * COUT + co] : (bf16)0);
             bf16 bv = 0;
             if (bias_col) {
@@ -169,7 +199,7 @@ __global__ __launch_bounds__(64) void k_conv_dw(const bf16 *__restrict__ G, cons
             }
             b[q] = (short)bv;
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+        acc = mfma32(a, b, acc);
     }
     // D[row = 4*kg + e][col = j]
     float *dst = part + (size_t)chunk * (MT * 16) * LD;
@@ -245,9 +275,11 @@ namespace nn {
 
 constexpr int NL = 4;             // layers
 constexpr int WID = 32;           // hidden channels
-constexpr int CINS[NL] = {4, WID, WID, WID};
+constexpr int NIN = 8;            // input channels: 4 fluctuations, 4 uniform
+constexpr int CINS[NL] = {NIN, WID, WID, WID};
 constexpr int COUTS[NL] = {WID, WID, WID, 4};
 constexpr int CM_BLOCKS = 512;    // blocks of the channel-mean reduction
+constexpr int CM_SLOTS = 40;      // doubles of Net::cm
 
 struct Layer {
     int cin, cout, kp, mp, kpt, mpt; // padded shapes of W [mp][kp] and Wt [mpt][kpt]
@@ -275,9 +307,13 @@ struct Net {
     float *y = nullptr, *tgt = nullptr, *part = nullptr, *lossp = nullptr;
     // channel statistics: [0,4) input mean, [4,8) residual mean, [8,12)
     // output mean, [12,16) input standard deviation, [16,20) residual
-    // standard deviation (first training pair); block partials
+    // standard deviation (first training pair), [20,24) zeros, [24,28) RMS
+    // of a judged step's difference from the plain step, [28,32) mean and
+    // [32,36) standard deviation of the plain step, [36,40) RMS of the
+    // leapfrog step's difference from it; block partials
     double *cm = nullptr, *cpart = nullptr;
     void *save_f = nullptr, *save_p = nullptr, *coarse_f = nullptr, *coarse_p = nullptr;
+    void *leap_f = nullptr, *leap_p = nullptr; // judged steps: the leapfrog step alone
     DevScalars *save_s = nullptr;
     hipStream_t stream = nullptr;
     std::string err;
@@ -309,14 +345,14 @@ static void free_buffers(Net &net)
 {
     void *ps[] = {net.x0, net.a[0], net.a[1], net.a[2], net.z[0], net.z[1], net.z[2], net.g4, net.gA, net.gB,
                   net.y, net.tgt, net.part, net.lossp, net.save_f, net.save_p, net.coarse_f, net.coarse_p,
-                  net.save_s, net.cm, net.cpart};
+                  net.leap_f, net.leap_p, net.save_s, net.cm, net.cpart};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     net.x0 = net.g4 = net.gA = net.gB = nullptr;
     for (int q = 0; q < 3; ++q) net.a[q] = net.z[q] = nullptr;
     net.y = net.tgt = net.part = net.lossp = nullptr;
     net.cm = net.cpart = nullptr;
-    net.save_f = net.save_p = net.coarse_f = net.coarse_p = nullptr;
+    net.save_f = net.save_p = net.coarse_f = net.coarse_p = net.leap_f = net.leap_p = nullptr;
     net.save_s = nullptr;
     net.n = 0;
 }
@@ -329,7 +365,7 @@ static int ensure_buffers(Net &net, const Ctx &c)
     free_buffers(net);
     const size_t nv = c.nvol;
     int rc = 0;
-    rc |= alloc(net, &net.x0, nv * 4, "x0");
+    rc |= alloc(net, &net.x0, nv * NIN, "x0");
     for (int q = 0; q < 3; ++q) {
         rc |= alloc(net, &net.a[q], nv * WID, "act");
         rc |= alloc(net, &net.z[q], nv * WID, "preact");
@@ -346,17 +382,25 @@ static int ensure_buffers(Net &net, const Ctx &c)
     }
     rc |= alloc(net, &net.part, maxpart * nchunks(nv), "dW partials");
     rc |= alloc(net, &net.lossp, 1024, "loss partials");
-    rc |= alloc(net, &net.cm, 20, "channel statistics");
+    rc |= alloc(net, &net.cm, CM_SLOTS, "channel statistics");
     rc |= alloc(net, &net.cpart, 4 * CM_BLOCKS, "channel-mean partials");
     const size_t es = elem_size(c);
     rc |= alloc(net, (char **)&net.save_f, es * c.ms.f, "saved fields");
     rc |= alloc(net, (char **)&net.save_p, es * c.ms.f, "saved momenta");
     rc |= alloc(net, (char **)&net.coarse_f, es * c.ms.f, "coarse fields");
     rc |= alloc(net, (char **)&net.coarse_p, es * c.ms.f, "coarse momenta");
+    rc |= alloc(net, (char **)&net.leap_f, es * c.ms.f, "leapfrog fields");
+    rc |= alloc(net, (char **)&net.leap_p, es * c.ms.f, "leapfrog momenta");
     rc |= alloc(net, &net.save_s, 1, "saved scalars");
     if (rc) {
         free_buffers(net);
         return rc;
+    }
+    const double zero[CM_SLOTS] = {};
+    if (hipMemcpy(net.cm, zero, sizeof(zero), hipMemcpyHostToDevice) != hipSuccess) {
+        free_buffers(net);
+        net.err = "channel statistics upload failed";
+        return -3;
     }
     net.n = c.nx;
     return 0;
@@ -410,7 +454,8 @@ __global__ __launch_bounds__(256) void k_chsum(const double *__restrict__ f, con
 }
 
 // out[c] = sum / nvol, or with root the standard deviation sqrt(sum / nvol)
-// (1 where it is 0: a channel without fluctuations is left unscaled)
+// (root 1: 1 where it is 0, so a channel without fluctuations is left
+// unscaled; root 2: the RMS as it is)
 __global__ __launch_bounds__(64) void k_chmean(const double *__restrict__ part, int nb, size_t nvol, int root,
                                                double *__restrict__ out)
 {
@@ -420,26 +465,35 @@ __global__ __launch_bounds__(64) void k_chmean(const double *__restrict__ part, 
         t /= (double)nvol;
         if (root) {
             t = sqrt(t);
-            if (!(t > 0.0)) t = 1.0;
+            if (root == 1 && !(t > 0.0)) t = 1.0;
         }
         out[threadIdx.x] = t;
     }
 }
 
-// X0[s][c] = (v_c(s) - <v_c>) / s_c for (phi, chi, pi_phi, pi_chi); st =
-// channel statistics (means at 0, standard deviations at 12)
+// X0[s][c] = (v_c(s) - <v_c>) / s_c for (phi, chi, pi_phi, pi_chi), then
+// the uniform channels <phi>, <pi_phi>, lg2 = log10(g^2/m^2) - 5 and
+// log10(s_chi) / 10; st = channel statistics (means at 0, standard
+// deviations at 12)
 __global__ __launch_bounds__(256) void k_pack(const double *__restrict__ f, const double *__restrict__ p,
-                                              size_t nvol, const double *__restrict__ st, bf16 *__restrict__ X)
+                                              size_t nvol, const double *__restrict__ st, float lg2,
+                                              bf16 *__restrict__ X)
 {
     const double4 mean = {st[0], st[1], st[2], st[3]};
     const double4 isd = {1.0 / st[12], 1.0 / st[13], 1.0 / st[14], 1.0 / st[15]};
+    const short u0 = (short)to_bf16((float)st[0]), u1 = (short)to_bf16((float)st[2]);
+    const short u2 = (short)to_bf16(lg2), u3 = (short)to_bf16((float)(log10(st[13]) / 10.0));
     for (size_t s = blockIdx.x * (size_t)256 + threadIdx.x; s < nvol; s += (size_t)gridDim.x * 256) {
-        bf16x4 o;
+        s16x8 o;
         o[0] = (short)to_bf16((float)((f[s] - mean.x) * isd.x));
         o[1] = (short)to_bf16((float)((f[nvol + s] - mean.y) * isd.y));
         o[2] = (short)to_bf16((float)((p[s] - mean.z) * isd.z));
         o[3] = (short)to_bf16((float)((p[nvol + s] - mean.w) * isd.w));
-        *reinterpret_cast<bf16x4 *>(X + s * 4) = o;
+        o[4] = u0;
+        o[5] = u1;
+        o[6] = u2;
+        o[7] = u3;
+        *reinterpret_cast<s16x8 *>(X + s * NIN) = o;
     }
 }
 
@@ -506,15 +560,15 @@ static void conv(Net &net, const bf16 *X, const bf16 *W, const float *bias, bf16
                  const bf16 *Zin)
 {
     const size_t ntile = (size_t)net.n * net.n * net.n / 16;
-    size_t g = (ntile + 3) / 4;
-    if (g > 8192) g = 8192;
-    hipLaunchKernelGGL((k_conv<CIN, COUT, MODE>), dim3((unsigned)g), dim3(256), 0, net.stream, X, W, bias, Y, Yf, Z,
-                       Zin, net.n);
+    size_t g = (ntile + CONV_WAVES - 1) / CONV_WAVES;
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL((k_conv<CIN, COUT, MODE>), dim3((unsigned)g), dim3(64 * CONV_WAVES), 0, net.stream, X, W, bias,
+                       Y, Yf, Z, Zin, net.n);
 }
 
 static void forward(Net &net)
 {
-    conv<4, WID, CONV_SILU>(net, net.x0, net.L[0].wb, net.L[0].b, net.a[0], nullptr, net.z[0], nullptr);
+    conv<NIN, WID, CONV_SILU>(net, net.x0, net.L[0].wb, net.L[0].b, net.a[0], nullptr, net.z[0], nullptr);
     conv<WID, WID, CONV_SILU>(net, net.a[0], net.L[1].wb, net.L[1].b, net.a[1], nullptr, net.z[1], nullptr);
     conv<WID, WID, CONV_SILU>(net, net.a[1], net.L[2].wb, net.L[2].b, net.a[2], nullptr, net.z[2], nullptr);
     conv<WID, 4, CONV_LINEAR>(net, net.a[2], net.L[3].wb, net.L[3].b, nullptr, net.y, nullptr, nullptr);
@@ -549,7 +603,7 @@ static void backward_layers(Net &net)
     conv<WID, WID, CONV_DSILU>(net, net.gA, net.L[2].wt, nullptr, net.gB, nullptr, nullptr, net.z[1]);
     weight_grad<WID, WID>(net, net.L[1], net.gB, net.a[0]);
     conv<WID, WID, CONV_DSILU>(net, net.gB, net.L[1].wt, nullptr, net.gA, nullptr, nullptr, net.z[0]);
-    weight_grad<4, WID>(net, net.L[0], net.gA, net.x0);
+    weight_grad<NIN, WID>(net, net.L[0], net.gA, net.x0);
 }
 
 static void backward(Net &net)
@@ -570,7 +624,7 @@ static void adam(Net &net, float lr)
 {
     ++net.step;
     const float c1 = 1.f - powf(0.9f, (float)net.step), c2 = 1.f - powf(0.999f, (float)net.step);
-    adam_layer<4, WID>(net, net.L[0], lr, c1, c2);
+    adam_layer<NIN, WID>(net, net.L[0], lr, c1, c2);
     adam_layer<WID, WID>(net, net.L[1], lr, c1, c2);
     adam_layer<WID, WID>(net, net.L[2], lr, c1, c2);
     adam_layer<WID, 4>(net, net.L[3], lr, c1, c2);
@@ -619,6 +673,18 @@ static int corr_fail(Net &net, const char *msg)
     return -2;
 }
 
+// A call borrows its context's stream: it ends with the stream drained and
+// the borrow cleared, so the corrector never keeps the stream of a context
+// that may be destroyed before it.
+struct StreamLease {
+    Net &net;
+    ~StreamLease()
+    {
+        if (net.stream) (void)hipStreamSynchronize(net.stream);
+        net.stream = nullptr;
+    }
+};
+
 static int check_ctx(Net &net, Ctx &c)
 {
     if (c.B != 1 || c.view) return corr_fail(net, "the corrector runs on single contexts");
@@ -654,8 +720,10 @@ static void pack(Net &net, Ctx &c, const void *f, const void *p)
                        (const double *)nullptr, (const double *)nullptr, (const float *)nullptr,
                        (const double *)net.cm, c.nvol, net.cpart);
     hipLaunchKernelGGL(k_chmean, dim3(1), dim3(64), 0, c.stream, net.cpart, CM_BLOCKS, c.nvol, 1, net.cm + 12);
+    const double g2 = c.cfg.model_params[1];
+    const float lg2 = g2 > 0.0 ? (float)(std::log10(g2) - 5.0) : 0.0f;
     hipLaunchKernelGGL(k_pack, dim3(grid_sites(c.nvol)), dim3(256), 0, c.stream, (const double *)f,
-                       (const double *)p, c.nvol, (const double *)net.cm, net.x0);
+                       (const double *)p, c.nvol, (const double *)net.cm, lg2, net.x0);
 }
 
 static int save_state(Net &net, Ctx &c)
@@ -782,6 +850,7 @@ int lat_corrector_infer(lat_corrector *h, lat_ctx *ctx, float *out)
 {
     if (!h || !ctx || !out) return -1;
     Net &net = h->net;
+    StreamLease lease{net};
     Ctx &c = ctx->c;
     int rc = check_ctx(net, c);
     if (rc) return rc;
@@ -800,6 +869,7 @@ int lat_corrector_gradients(lat_corrector *h, lat_ctx *ctx, const float *target,
 {
     if (!h || !ctx || !target) return -1;
     Net &net = h->net;
+    StreamLease lease{net};
     Ctx &c = ctx->c;
     int rc = check_ctx(net, c);
     if (rc) return rc;
@@ -839,6 +909,7 @@ int lat_corrector_train(lat_corrector *h, lat_ctx *ctx, double dt, int coarse_fa
 {
     if (!h || !ctx || coarse_factor < 1 || npairs < 0) return -1;
     Net &net = h->net;
+    StreamLease lease{net};
     Ctx &c = ctx->c;
     int rc = check_ctx(net, c);
     if (rc) return rc;
@@ -914,6 +985,7 @@ int lat_corrector_step(lat_corrector *h, lat_ctx *ctx, double dt, int coarse_fac
 {
     if (!h || !ctx || coarse_factor < 1 || nsteps < 0) return -1;
     Net &net = h->net;
+    StreamLease lease{net};
     Ctx &c = ctx->c;
     if (!net.stats) return corr_fail(net, "the corrector has not been trained (no normalisation)");
     int rc = check_ctx(net, c);
@@ -946,6 +1018,114 @@ int lat_corrector_step(lat_corrector *h, lat_ctx *ctx, double dt, int coarse_fac
         stats[0] = fb;
         stats[1] = maxv;
         stats[2] = forces;
+    }
+    return 0;
+}
+
+// As lat_corrector_step, and every judge_every-th accepted corrected step is
+// also taken by the plain integrator (coarse_factor Yoshida-4 steps of dt
+// from the same start): the errors of the corrected step and of the leapfrog
+// step alone against it, per channel (phi, chi, pi_phi, pi_chi) the RMS
+// difference over the plain step's spatial standard deviation, the largest
+// over the channels, are recorded and the plain step is kept.  Once a judged
+// corrected step's error exceeds err_tol, or that of the leapfrog step
+// alone, the corrector is switched off and every remaining step is the plain
+// integrator's.  stats (nullable, 7 doubles): fallbacks (constraint), force
+// evaluations, judged steps, largest judged error of the corrected step,
+// step index at which the corrector was switched off (-1: never), largest
+// accepted constraint change, largest judged error of the leapfrog step.
+int lat_corrector_step_judged(lat_corrector *h, lat_ctx *ctx, double dt, int coarse_factor, int nsteps,
+                              double threshold, int judge_every, double err_tol, double *stats)
+{
+    if (!h || !ctx || coarse_factor < 1 || nsteps < 0 || judge_every < 1) return -1;
+    Net &net = h->net;
+    StreamLease lease{net};
+    Ctx &c = ctx->c;
+    if (!net.stats) return corr_fail(net, "the corrector has not been trained (no normalisation)");
+    int rc = check_ctx(net, c);
+    if (rc) return rc;
+    const size_t bytes = sizeof(double) * c.ms.f;
+    double fb = 0, forces = 0, judged = 0, maxerr = 0, maxleap = 0, off_at = -1, maxv = 0, e[LAT_NE];
+    int accepted = 0;
+    // largest over the channels of RMS(a - plain) / std(plain), the plain
+    // step being the current state and its standard deviations at cm + 32
+    auto judge_err = [&](const void *af, const void *ap, double *out) -> int {
+        hipLaunchKernelGGL(k_chsum, dim3(CM_BLOCKS), dim3(256), 0, c.stream, (const double *)c.fld,
+                           (const double *)c.fldp, (const double *)af, (const double *)ap, (const float *)nullptr,
+                           (const double *)(net.cm + 20), c.nvol, net.cpart);
+        hipLaunchKernelGGL(k_chmean, dim3(1), dim3(64), 0, c.stream, net.cpart, CM_BLOCKS, c.nvol, 2, net.cm + 24);
+        double jd[12];
+        LAT_HIP(&c, hipMemcpyAsync(jd, net.cm + 24, sizeof(jd), hipMemcpyDeviceToHost, c.stream));
+        LAT_HIP(&c, hipStreamSynchronize(c.stream));
+        double err = 0;
+        for (int q = 0; q < 4; ++q) {
+            const double r = jd[q] / jd[8 + q];
+            err = (r > err || !(r == r)) ? r : err;
+        }
+        *out = err;
+        return 0;
+    };
+    for (int it = 0; it < nsteps; ++it) {
+        if (off_at >= 0) { // switched off: the plain integrator
+            if ((rc = lat_symp(ctx, 4, dt, coarse_factor))) return rc;
+            forces += 3.0 * coarse_factor;
+            continue;
+        }
+        if ((rc = lat_energies(ctx, e))) return rc;
+        const double r0 = e[LAT_E_RESID];
+        if ((rc = save_state(net, c))) return rc;
+        if ((rc = lat_symp(ctx, 2, coarse_factor * dt, 1))) return rc;
+        const bool judge = accepted % judge_every == 0; // if this step is accepted
+        if (judge) {
+            LAT_HIP(&c, hipMemcpyAsync(net.leap_f, c.fld, bytes, hipMemcpyDeviceToDevice, c.stream));
+            LAT_HIP(&c, hipMemcpyAsync(net.leap_p, c.fldp, bytes, hipMemcpyDeviceToDevice, c.stream));
+        }
+        pack(net, c, c.fld, c.fldp);
+        forward(net);
+        channel_means(net, c, nullptr, nullptr, nullptr, nullptr, net.y, 2);
+        hipLaunchKernelGGL(k_apply, dim3(grid_sites(c.nvol)), dim3(256), 0, c.stream, (double *)c.fld,
+                           (double *)c.fldp, net.y, c.nvol, (const double *)net.cm, d4(net.ratio));
+        c.ke_valid = false;
+        forces += 1;
+        if ((rc = lat_energies(ctx, e))) return rc;
+        const double v = std::fabs(e[LAT_E_RESID] - r0);
+        if (!(v <= threshold)) {
+            if ((rc = restore_state(net, c))) return rc;
+            if ((rc = lat_symp(ctx, 4, dt, coarse_factor))) return rc;
+            fb += 1;
+            forces += 3.0 * coarse_factor;
+            continue;
+        }
+        maxv = v > maxv ? v : maxv;
+        ++accepted;
+        if (!judge) continue;
+        // judge: the plain step from the same start
+        LAT_HIP(&c, hipMemcpyAsync(net.coarse_f, c.fld, bytes, hipMemcpyDeviceToDevice, c.stream));
+        LAT_HIP(&c, hipMemcpyAsync(net.coarse_p, c.fldp, bytes, hipMemcpyDeviceToDevice, c.stream));
+        if ((rc = restore_state(net, c))) return rc;
+        if ((rc = lat_symp(ctx, 4, dt, coarse_factor))) return rc;
+        forces += 3.0 * coarse_factor;
+        judged += 1;
+        channel_means(net, c, c.fld, c.fldp, nullptr, nullptr, nullptr, 7);
+        hipLaunchKernelGGL(k_chsum, dim3(CM_BLOCKS), dim3(256), 0, c.stream, (const double *)c.fld,
+                           (const double *)c.fldp, (const double *)nullptr, (const double *)nullptr,
+                           (const float *)nullptr, (const double *)(net.cm + 28), c.nvol, net.cpart);
+        hipLaunchKernelGGL(k_chmean, dim3(1), dim3(64), 0, c.stream, net.cpart, CM_BLOCKS, c.nvol, 1, net.cm + 32);
+        double ec = 0, el = 0;
+        if ((rc = judge_err(net.coarse_f, net.coarse_p, &ec))) return rc;
+        if ((rc = judge_err(net.leap_f, net.leap_p, &el))) return rc;
+        maxerr = (ec > maxerr || !(ec == ec)) ? ec : maxerr;
+        maxleap = (el > maxleap || !(el == el)) ? el : maxleap;
+        if (!(ec <= err_tol) || !(ec <= el)) off_at = it;
+    }
+    if (stats) {
+        stats[0] = fb;
+        stats[1] = forces;
+        stats[2] = judged;
+        stats[3] = maxerr;
+        stats[4] = off_at;
+        stats[5] = maxv;
+        stats[6] = maxleap;
     }
     return 0;
 }

@@ -70,7 +70,7 @@ EXPORTS = [
     "lat_checkpoint", "lat_restore",
     "lat_corrector_create", "lat_corrector_destroy", "lat_corrector_last_error", "lat_corrector_set_layer",
     "lat_corrector_get_layer", "lat_corrector_set_norm", "lat_corrector_get_norm", "lat_corrector_infer",
-    "lat_corrector_gradients", "lat_corrector_train", "lat_corrector_step",
+    "lat_corrector_gradients", "lat_corrector_train", "lat_corrector_step", "lat_corrector_step_judged",
 ]
 STAGES = ["x_r2c", "y_fwd", "z_lap", "y_inv", "x_c2r", "scalar", "drift", "stencil", "reduce", "gw_src",
           "sep_x", "sep_y", "sep_z"]
@@ -617,7 +617,10 @@ class Corrector:
     """The learned corrector (lat_corrector_*): a 4-layer BF16 3-D conv net
     on MFMA trained on the GPU on pairs from the plain integrator."""
 
-    SHAPES = [(4, 32), (32, 32), (32, 32), (32, 4)]   # (cin, cout) per layer
+    SHAPES = [(8, 32), (32, 32), (32, 32), (32, 4)]   # (cin, cout) per layer
+    # input channels: the fluctuations (phi, chi, pi_phi, pi_chi) over their
+    # spatial standard deviations, then the uniform <phi>, <pi_phi>,
+    # log10(g^2/m^2) - 5 and log10(s_chi) / 10
 
     def __init__(self, seed=1):
         h = ctypes.c_void_p()
@@ -704,3 +707,17 @@ class Corrector:
         self._check(lib().lat_corrector_step(self._h, lat._h, ctypes.c_double(dt), coarse_factor, nsteps,
                                              ctypes.c_double(threshold), _p(st)), "step")
         return {"fallbacks": int(st[0]), "max_violation": float(st[1]), "force_evals": float(st[2])}
+
+    def step_judged(self, lat, dt, coarse_factor=8, nsteps=1, threshold=1e-6, judge_every=4, err_tol=0.1):
+        """Corrected coarse steps with the plain integrator as judge: every
+        judge_every-th accepted step is also taken by the plain integrator,
+        the errors of the corrected and of the leapfrog step against it are
+        measured and the plain step kept; the corrector is switched off once
+        a corrected step's error exceeds err_tol or the leapfrog step's."""
+        st = np.zeros(7)
+        self._check(lib().lat_corrector_step_judged(self._h, lat._h, ctypes.c_double(dt), coarse_factor, nsteps,
+                                                    ctypes.c_double(threshold), judge_every,
+                                                    ctypes.c_double(err_tol), _p(st)), "step_judged")
+        return {"fallbacks": int(st[0]), "force_evals": float(st[1]), "judged": int(st[2]),
+                "max_judged_error": float(st[3]), "switched_off_at": int(st[4]), "max_violation": float(st[5]),
+                "max_judged_leapfrog_error": float(st[6])}

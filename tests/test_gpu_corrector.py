@@ -31,7 +31,7 @@ def state(n, L, seed, amp=2e3):
 
 def random_layers(rng, scale=0.3):
     layers = []
-    for cin, cout in [(4, 32), (32, 32), (32, 32), (32, 4)]:
+    for cin, cout in [(8, 32), (32, 32), (32, 32), (32, 4)]:
         w = (rng.standard_normal((cout, 27, cin)) * scale / np.sqrt(27 * cin)).astype(np.float32)
         b = (rng.standard_normal(cout) * 0.1).astype(np.float32)
         layers.append((w, b))
@@ -45,7 +45,7 @@ def torch_weight(w):
 
 
 def torch_forward(x0, layers, rounding=True):
-    """x0: (nz, ny, nx, 4) BF16-valued; returns output (nz, ny, nx, 4) and the graph leaves."""
+    """x0: (nz, ny, nx, 8) BF16-valued; returns output (nz, ny, nx, 4) and the graph leaves."""
     F = torch.nn.functional
     h = torch.from_numpy(x0.transpose(3, 0, 1, 2)[None].copy())
     params = []
@@ -66,11 +66,17 @@ def torch_forward(x0, layers, rounding=True):
     return h[0].permute(1, 2, 3, 0), params
 
 
-def packed_input(fld, fldp):
+def packed_input(fld, fldp, g2):
     """Fluctuations about the per-channel spatial mean over the per-channel
-    spatial standard deviation, in BF16."""
+    spatial standard deviation, then the uniform channels <phi>, <pi_phi>,
+    log10(g^2/m^2) - 5 and log10(s_chi) / 10, in BF16."""
     v = np.stack([fld[0], fld[1], fldp[0], fldp[1]], axis=-1)
-    return bf16(((v - v.mean(axis=(0, 1, 2))) / v.std(axis=(0, 1, 2))).astype(np.float32))
+    m, sd = v.mean(axis=(0, 1, 2)), v.std(axis=(0, 1, 2))
+    x = np.empty(v.shape[:3] + (8,), np.float32)
+    x[..., :4] = (v - m) / sd
+    x[..., 4], x[..., 5] = m[0], m[2]
+    x[..., 6], x[..., 7] = np.log10(g2) - 5.0, np.log10(sd[1]) / 10.0
+    return bf16(x)
 
 
 def test_corrector_forward_matches_torch(gpu):
@@ -87,7 +93,7 @@ def test_corrector_forward_matches_torch(gpu):
         out = corr.infer(lat)
         w0, b0 = corr.get_layer(0)
     assert np.array_equal(w0, layers[0][0]) and np.array_equal(b0, layers[0][1])
-    ref, _ = torch_forward(packed_input(fld, fldp), layers)
+    ref, _ = torch_forward(packed_input(fld, fldp, 1e4), layers)
     ref = ref.detach().numpy()
     err = np.abs(out - ref)
     # same BF16 rounding points; only the FP32 summation order differs, which
@@ -111,7 +117,7 @@ def test_corrector_gradients_match_torch(gpu):
             corr.set_layer(li, w, b)
         corr.set_norm(np.ones(4))
         loss, grads = corr.gradients(lat, target)
-    out, params = torch_forward(packed_input(fld, fldp), layers)
+    out, params = torch_forward(packed_input(fld, fldp, 1e4), layers)
     tl = ((out - torch.from_numpy(target)) ** 2).mean()
     tl.backward()
     assert abs(loss - tl.item()) < 1e-2 * tl.item()
