@@ -18,7 +18,8 @@
 // histogram; the runs that cross lanes are joined by one segmented DPP scan
 // per class.  Within one histogram instruction the lanes' bins are distinct,
 // and the instructions run in program order (no atomics, no cross-wave
-// ordering).  The block histograms are reduced in block order by
+// ordering; the barriers fence LDS only, so the next class's loads stay in
+// flight across them).  The block histograms are reduced in block order by
 // k_bin_reduce (groups of 64 blocks, then the groups).  Results are bitwise
 // reproducible run to run.
 #include <mutex>
@@ -155,12 +156,13 @@ __global__ __launch_bounds__(64) void k_bin(const cpx<T> *__restrict__ F1, const
     constexpr int LM = BIN_LANEMODES;
     constexpr int LA = MODE <= 2 ? LM : 1;
     // the next class's loads are issued before the current class is binned
-    // (FP32 data: two register sets fit)
+    // (FP32 data: two register sets fit; FP64 data loads each class when its
+    // turn comes)
     constexpr bool PF = MODE <= 2 && sizeof(T) == 4;
     extern __shared__ double hist[]; // NC * ns
     const int lane = threadIdx.x;
     for (int q = lane; q < NC * ns; q += 64) hist[q] = 0.0;
-    __syncthreads();
+    lds_sync();
     const bool counts = (flags & BIN_COUNTS) != 0;
     // the lane's modes [s, e) of a row: M consecutive ones, and the row's
     // last mode for the lane whose range reaches it
@@ -191,39 +193,51 @@ __global__ __launch_bounds__(64) void k_bin(const cpx<T> *__restrict__ F1, const
         c.mj[3] = ny - c.jj; c.mk[3] = nz - c.kk;
         return c;
     };
-    // The class's values, mode-major, modes sb .. sb + M of each member row:
-    // the lane's own (sb = s) or, for a lane without modes, the row's first
-    // ones; absent members read member 0.  Every load is in bounds and
+    // The class's values, modes sb .. sb + M of each member row: the lane's
+    // own (sb = s) or, for a lane without modes, the row's first ones;
+    // absent members read member 0.  Every load is in bounds and
     // unpredicated; values of modes or members outside the class are never
     // used.
     const int sb = e > s ? s : 0;
     struct Buf {
         cpx<T> a[LA][4], b[LA][4];
     };
+    struct Rows {
+        const cpx<T> *r1[4], *r2[4];
+    };
+    auto rows_of = [&](const Cls &c) {
+        Rows r;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const int mm = m < c.nm ? m : 0;
+            const size_t off = ((size_t)c.mk[mm] * ny + c.mj[mm]) * P + sb;
+            r.r1[m] = F1 + off;
+            r.r2[m] = MODE != 0 ? F2 + off : F1 + off;
+        }
+        return r;
+    };
+    // a whole class: row by row (each row's loads back to back, so the
+    // cache lines a lane range spans are reused while they are in L1)
     auto load = [&](Buf &u, int q) {
         if constexpr (MODE <= 2) {
-            const Cls c = members(q);
-            const cpx<T> *r1[4], *r2[4];
+            const Rows r = rows_of(members(q));
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
-                const int mm = m < c.nm ? m : 0;
-                const size_t off = ((size_t)c.mk[mm] * ny + c.mj[mm]) * P + sb;
-                r1[m] = F1 + off;
-                r2[m] = MODE != 0 ? F2 + off : F1 + off;
-            }
 #pragma unroll
-            for (int t = 0; t < LM; ++t) {
-                if (t > M) continue; // (uniform)
+                for (int t = 0; t < LM; ++t) {
+                    if (t > M) continue; // (uniform)
+                    u.a[t][m] = r.r1[m][t];
+                }
 #pragma unroll
-                for (int m = 0; m < 4; ++m) {
-                    u.a[t][m] = r1[m][t];
-                    if constexpr (MODE != 0) u.b[t][m] = r2[m][t];
+                for (int t = 0; t < LM; ++t) {
+                    if (t > M) continue;
+                    if constexpr (MODE != 0) u.b[t][m] = r.r2[m][t];
                     else u.b[t][m] = cpx<T>{0, 0};
                 }
             }
         }
     };
-    auto bin_class = [&](const Buf &u, int q) {
+    auto bin_class = [&](Buf &u, int q) {
         const Cls c = members(q);
         const int rr2 = c.jj * c.jj + c.kk * c.kk;
         // value channels of mode i summed over the members, in member order
@@ -305,12 +319,12 @@ __global__ __launch_bounds__(64) void k_bin(const cpx<T> *__restrict__ F1, const
 #pragma unroll
                 for (int ch = 0; ch < NC; ++ch) hist[ch * ns + lk] += ra[ch];
             }
-            __syncthreads();
+            lds_sync();
             if (whole && lk + 1 < ns) {
 #pragma unroll
                 for (int ch = 0; ch < NC; ++ch) hist[ch * ns + lk + 1] += rb[ch];
             }
-            __syncthreads();
+            lds_sync();
             const bool first = brk && fk == NOKEY;
             const double kf = first ? 1.0 : 0.0, kr = brk ? 0.0 : 1.0;
 #pragma unroll
@@ -365,7 +379,7 @@ __global__ __launch_bounds__(64) void k_bin(const cpx<T> *__restrict__ F1, const
 #pragma unroll
             for (int ch = 0; ch < NC; ++ch) hist[ch * ns + lk] += ra[ch];
         }
-        __syncthreads();
+        lds_sync();
         if (ta && fk + 1 < ns) {
 #pragma unroll
             for (int ch = 0; ch < NC; ++ch) hist[ch * ns + fk + 1] += fb[ch];
@@ -374,7 +388,7 @@ __global__ __launch_bounds__(64) void k_bin(const cpx<T> *__restrict__ F1, const
 #pragma unroll
             for (int ch = 0; ch < NC; ++ch) hist[ch * ns + lk + 1] += rb[ch];
         }
-        __syncthreads();
+        lds_sync();
     };
     if constexpr (PF) {
         Buf u0, u1;
