@@ -452,6 +452,31 @@ static int bin_grid(const Ctx &c, K kern, size_t lds)
     return g < ncls ? g : ncls;
 }
 
+// The weight channel W of (ns, flags): its cache slot (W at spec_out +
+// (5 + slot) * spec_cap), binned by the weight-only pass on a miss.
+template <typename T>
+static int weight_slot(Ctx &c, int ns, int flags)
+{
+    Ctx &owner = (c.view && c.parent) ? *c.parent : c;
+    const long long key = ((long long)flags << 32) | (unsigned)ns;
+    for (int q = 0; q < Ctx::WCACHE; ++q)
+        if (owner.wkey[q] == key) return q;
+    const int slot = owner.wnext;
+    owner.wnext = (owner.wnext + 1) % Ctx::WCACHE;
+    owner.wkey[slot] = -1;
+    const size_t lw = sizeof(double) * ns;
+    const int g = bin_grid(c, k_bin<T, 0, 4, true>, lw);
+    // the weight-only pass reads no transform
+    const cpx<T> *none = nullptr;
+    hipLaunchKernelGGL((k_bin<T, 0, 4, true>), dim3(g), dim3(64), lw, c.stream, none, none, c.nx, c.ny, c.nz, c.nxh,
+                       c.P, ns, c.spec_part, flags, c.nk);
+    LAT_HIP(&c, hipGetLastError());
+    bin_reduce(c, g, ns, c.spec_out + (5 + slot) * (size_t)c.spec_cap);
+    LAT_HIP(&c, hipGetLastError());
+    owner.wkey[slot] = key;
+    return slot;
+}
+
 template <typename T>
 int launch_bin(Ctx &c, const cpx<T> *F1, const cpx<T> *F2, int mode, int nbins, int flags)
 {
@@ -469,26 +494,9 @@ int launch_bin(Ctx &c, const cpx<T> *F1, const cpx<T> *F2, int mode, int nbins, 
         c.last_error = "spectrum rows longer than the binning kernel's row buffer";
         return -1;
     }
-    // weight channel: from the cache slot of (ns, flags), binned on a miss
-    Ctx &owner = (c.view && c.parent) ? *c.parent : c;
-    const long long key = ((long long)flags << 32) | (unsigned)ns;
-    int slot = -1;
-    for (int q = 0; q < Ctx::WCACHE; ++q)
-        if (owner.wkey[q] == key) slot = q;
+    const int slot = weight_slot<T>(c, ns, flags);
+    if (slot < 0) return slot;
     const size_t cap = (size_t)c.spec_cap;
-    if (slot < 0) {
-        slot = owner.wnext;
-        owner.wnext = (owner.wnext + 1) % Ctx::WCACHE;
-        owner.wkey[slot] = -1;
-        const size_t lw = sizeof(double) * ns;
-        const int g = bin_grid(c, k_bin<T, 0, 4, true>, lw);
-        hipLaunchKernelGGL((k_bin<T, 0, 4, true>), dim3(g), dim3(64), lw, c.stream, F1, F2, c.nx, c.ny, c.nz, c.nxh,
-                           c.P, ns, c.spec_part, flags, c.nk);
-        LAT_HIP(&c, hipGetLastError());
-        bin_reduce(c, g, ns, c.spec_out + (5 + slot) * cap);
-        LAT_HIP(&c, hipGetLastError());
-        owner.wkey[slot] = key;
-    }
     const size_t lds = sizeof(double) * nch * ns;
     int g = 0;
     if (mode == 0) {

@@ -76,6 +76,38 @@ __global__ __launch_bounds__(TPB) void k_drift(T *__restrict__ f, const T *__res
     }
 }
 
+// The same drift, 16 bytes per lane and one lane per vector: grid (vectors of
+// one field / TPB, members, fields), so the field index (for the re-centring
+// shift) is blockIdx.z and no lane divides.  Needs nvol a multiple of the
+// vector width and 16-byte aligned field and member offsets.
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_drift_v(T *__restrict__ f, const T *__restrict__ p, size_t nvol,
+                                                 const DevScalars *__restrict__ dsc, int recenter, size_t ms)
+{
+    constexpr int V = 16 / sizeof(T);
+    using Vec = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+    const size_t i = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (i >= nvol / V) return;
+    const size_t off = blockIdx.y * ms + blockIdx.z * nvol;
+    dsc += blockIdx.y;
+    const int nd = dsc->nd;
+    const T sh = (recenter && dsc->mixed) ? T(dsc->nu[blockIdx.z]) : T(0);
+    Vec x = reinterpret_cast<const Vec *>(f + off)[i];
+    const Vec pv = reinterpret_cast<const Vec *>(p + off)[i];
+    T *xs = reinterpret_cast<T *>(&x);
+    const T *ps = reinterpret_cast<const T *>(&pv);
+    for (int q = 0; q < nd; ++q) {
+        const T d = T(dsc->dc[q]);
+#pragma unroll
+        for (int e = 0; e < V; ++e) xs[e] = xs[e] + d * ps[e];
+    }
+    if (recenter && dsc->mixed) {
+#pragma unroll
+        for (int e = 0; e < V; ++e) xs[e] = xs[e] - sh;
+    }
+    reinterpret_cast<Vec *>(f + off)[i] = x;
+}
+
 template <typename T>
 int launch_drift(Ctx &c)
 {
@@ -83,12 +115,22 @@ int launch_drift(Ctx &c)
     for (int part = 0; part < (c.gw ? 2 : 1); ++part) {
         const int nf = part == 0 ? c.nfld : 6;
         const size_t n = (size_t)nf * c.nvol;
-        int g = (int)((n + TPB - 1) / TPB);
-        if (g > 8192) g = 8192;
         T *f = (T *)(part == 0 ? c.fld : c.gwu);
         const T *p = (const T *)(part == 0 ? c.fldp : c.gwp);
-        hipLaunchKernelGGL((k_drift<T>), dim3(g, c.B), dim3(TPB), 0, c.stream, f, p, c.nvol, nf, c.dsc,
-                           part == 0 ? 1 : 0, part == 0 ? c.ms.f : (size_t)6 * c.nvol);
+        const size_t ms = part == 0 ? c.ms.f : (size_t)6 * c.nvol;
+        constexpr size_t V = 16 / sizeof(T);
+        const bool vec = c.nvol % V == 0 && ms % V == 0 && ((uintptr_t)f | (uintptr_t)p) % 16 == 0 &&
+                         c.nvol / V / TPB < 0x7fffffffu;
+        if (vec) {
+            const unsigned gx = (unsigned)((c.nvol / V + TPB - 1) / TPB);
+            hipLaunchKernelGGL((k_drift_v<T>), dim3(gx, c.B, nf), dim3(TPB), 0, c.stream, f, p, c.nvol, c.dsc,
+                               part == 0 ? 1 : 0, ms);
+        } else {
+            int g = (int)((n + TPB - 1) / TPB);
+            if (g > 8192) g = 8192;
+            hipLaunchKernelGGL((k_drift<T>), dim3(g, c.B), dim3(TPB), 0, c.stream, f, p, c.nvol, nf, c.dsc,
+                               part == 0 ? 1 : 0, ms);
+        }
     }
     prof_end(c, pi);
     LAT_HIP(&c, hipGetLastError());
