@@ -13,7 +13,11 @@ metric is site-steps/s = N^3 * steps / s, output spectra of both fields
 included at the configured cadence.  Both forms run from identical IC:
   plain FP64, and mixed (FP32 fields/FFT, FP64 background and reductions);
 `value` is the mixed form, the FP64 form and the mixed-vs-FP64 spectrum
-error are reported beside it.
+error are reported beside it.  After the headline (and outside its timed
+region) the default run measures the other configurations into `extras`:
+the 256^3 drop-in configuration (spectral and 7-point FD, both forms), the
+96-member ensemble of uneven sizes and the 1024^3 run with the six GW
+components (--no-extras skips them).
 """
 from __future__ import annotations
 
@@ -420,7 +424,7 @@ def ensemble_leg(la, args):
     best = "graph" if out["autotuner_picked_single_launch"] else "graph_per_class"
     out["value"] = out[best]["site_steps_per_s"]
     out["value_mode"] = best
-    print(json.dumps(out))
+    return out
 
 
 def config1_leg(la, args):
@@ -446,7 +450,7 @@ def config1_leg(la, args):
                                 "model_bytes_per_site_step": model, "hbm_frac_vs_8TBs": v * model / 8e12,
                                 "stages_ms": {k: round(x[0], 3) for k, x in r["prof"].items() if x[1]}})
             print(f"[bench] {n}^3 {lap_name} {form}: {v:.4e} site-steps/s", file=sys.stderr, flush=True)
-    print(json.dumps(out))
+    return out
 
 
 def gw_leg(la, args):
@@ -497,7 +501,7 @@ def gw_leg(la, args):
                             "energy": abs(m["e"]["rho"] - d["e"]["rho"]) / abs(d["e"]["rho"])},
            "after_steps": args.warmup + args.gw_steps}
     out["value"] = out["mixed"]["site_steps_per_s"]
-    print(json.dumps(out))
+    return out
 
 
 def main():
@@ -544,6 +548,8 @@ def main():
     ap.add_argument("--corr-bound", type=float, default=1e-6,
                     help="fall back when a coarse step moves the relative Friedmann residual by more than this")
     ap.add_argument("--corr-g2", default="1e4,2e4,3e4,4e4,6e4,1e5,1e6")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="default run: skip the 256^3 drop-in, ensemble and GW legs after the headline")
     args = ap.parse_args()
     if args.corrector:
         import lattice_amd as la
@@ -553,17 +559,17 @@ def main():
     if args.ensemble:
         import lattice_amd as la
 
-        ensemble_leg(la, args)
+        print(json.dumps(ensemble_leg(la, args)))
         return
     if args.gw:
         import lattice_amd as la
 
-        gw_leg(la, args)
+        print(json.dumps(gw_leg(la, args)))
         return
     if args.config1:
         import lattice_amd as la
 
-        config1_leg(la, args)
+        print(json.dumps(config1_leg(la, args)))
         return
     if args.copy_calibration:
         # the library's plain copy kernel (k_copy16) over 2 GiB, 1 + 3 launches:
@@ -646,6 +652,23 @@ def main():
         line["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_steps)
     else:
         line["cpu_baseline"] = None
+    if not args.no_extras and n == 1024:
+        # the other configurations, measured in the same run (not the headline)
+        extras = {}
+        c1 = config1_leg(la, args)
+        extras["config1_256"] = {"steps": c1["steps"], "runs": [
+            {k: r[k] for k in ("laplacian", "form", "site_steps_per_s", "ms_per_step", "hbm_frac_vs_8TBs")}
+            for r in c1["runs"]]}
+        ens = ensemble_leg(la, args)
+        extras["ensemble_96"] = {k: ens[k] for k in ("classes", "form", "steps", "value", "value_mode",
+                                                      "autotuner_picked_single_launch",
+                                                      "spectrum_rel_err_vs_single_runs")}
+        extras["ensemble_96"]["modes"] = {m: ens[m]["site_steps_per_s"]
+                                          for m in ("graph", "eager", "graph_per_class", "eager_per_class")}
+        gw = gw_leg(la, args)
+        extras["gw_1024"] = {k: gw[k] for k in ("steps", "mixed", "f64", "mixed_over_fp64",
+                                                 "hbm_frac_mixed_vs_8TBs", "mixed_vs_f64", "after_steps")}
+        line["extras"] = extras
     print(json.dumps(line))
 
 
