@@ -21,8 +21,9 @@
 //
 // Backward: the data gradient is the same convolution with the transposed,
 // tap-reversed weights Wt[ci][t * COUT + co] = W[co][(26 - t) * CIN + ci];
-// the weight gradient is a GEMM over sites (K = sites) accumulated per chunk
-// of sites into partials that are reduced in a fixed order (deterministic).
+// the weight gradient is a GEMM over sites (K = sites), staged through LDS
+// per row segment and accumulated per workgroup into partials that are
+// reduced in a fixed order (deterministic).
 #include <cmath>
 
 #include "lat_internal.h"
@@ -160,55 +161,134 @@ __global__ __launch_bounds__(64 * CONV_WAVES) void k_conv(const bf16 *__restrict
     }
 }
 
-// Weight gradient: dW[co][t*CIN + ci] = sum_s G[s][co] X[nbr(s,t)][ci] over a
-// chunk of GCHUNK sites per wave, 32 sites per MFMA (a multiple of 32 sites
-// divides nvol, n being a multiple of 16); one wave per (co tile, column
-// tile, chunk).
-// part: [nchunk][MT*16][KP] FP32.  Also db[co] partial in column KP (part has
-// KP + 16 columns: the bias gradient uses a column of ones).
+// Weight gradient partials: one per block, nchunks(nvol) of them at most
+// (GCHUNK sites per partial), each [MT*16][KP + 16] FP32 with the bias
+// gradient in column KP.
 constexpr int GCHUNK = 4096;
+
+// Weight gradient from LDS tiles.  A block of 9 waves walks chunks of CL
+// sites along x (one row segment; CL = 64, 32 or 16, dividing n).  Per chunk
+// the X rows of the 3 x 3 (dy, dz) neighbour rows with one site of halo on
+// each side in x, and the chunk's G, are staged in LDS channel-major
+// (xs[ci][row][x], gs[co][x]), so that the 8 consecutive sites an MFMA lane
+// supplies are 8 consecutive LDS entries.  Wave w takes the tap row (dy, dz)
+// = (w % 3 - 1, w / 3 - 1) and its three dx taps: per 32 sites it loads G
+// once (A = G^T, rows co) and X shifted by dx (B, columns ci) and keeps the
+// 3 x MT x NTL accumulator tiles of its taps in registers across its chunks.
+// The block writes its partial dW (and db in column KP) as [MT*16][KP + 16];
+// the partials are summed in block order by k_sum_chunks.
+constexpr int DW2_WAVES = 9;
+constexpr int DW2_CLMAX = 64;
+constexpr int DW2_RP = DW2_CLMAX + 8;   // LDS row pitch of the X tile (halo + 16-byte multiple)
+constexpr int DW2_BLOCKS = 512;          // persistent blocks (2 per CU)
+
+__device__ inline int dw2_chunk(int n) { return n % 64 == 0 ? 64 : (n % 32 == 0 ? 32 : 16); }
+
 template <int CIN, int COUT>
-__global__ __launch_bounds__(64) void k_conv_dw(const bf16 *__restrict__ G, const bf16 *__restrict__ X,
-                                                float *__restrict__ part, int n)
+__global__ __launch_bounds__(64 * DW2_WAVES) void k_conv_dw2(const bf16 *__restrict__ G, const bf16 *__restrict__ X,
+                                                             float *__restrict__ part, int n)
 {
-    constexpr int MT = (COUT + 15) / 16;
-    constexpr int KP = KPAD(CIN);
-    constexpr int CT = KP / 16 + 1; // the last column tile carries the bias column
-    constexpr int LD = KP + 16;
-    const int lane = threadIdx.x, j = lane & 15, kg = lane >> 4;
-    const int ct = blockIdx.x % CT, mt = (blockIdx.x / CT) % MT, chunk = blockIdx.x / (CT * MT);
+    constexpr int MT = (COUT + 15) / 16, NTL = (CIN + 15) / 16;
+    constexpr int KP = KPAD(CIN), LD = KP + 16;
+    constexpr int NT = 64 * DW2_WAVES;
+    // channel rows padded to whole 16-row tiles; rows >= CIN / COUT stay zero
+    __shared__ __attribute__((aligned(16))) bf16 xs[NTL * 16][9][DW2_RP];
+    __shared__ __attribute__((aligned(16))) bf16 gs[MT * 16][DW2_CLMAX];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int j = lane & 15, kg = lane >> 4;
+    const int dy = wave % 3 - 1, dz = wave / 3 - 1;
+    const int row = (dz + 1) * 3 + (dy + 1);
+    const int CL = dw2_chunk(n);
+    const int KS = (CL + 31) / 32;
     const size_t nvol = (size_t)n * n * n;
-    const size_t s0 = (size_t)chunk * GCHUNK;
-    const int co = mt * 16 + j; // A row supplied by this lane
-    const int col = ct * 16 + j; // B column supplied by this lane
-    const bool bias_col = ct == CT - 1;
-    const int t = col / CIN, ci = col - t * CIN;
-    f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (size_t sb = s0; sb < s0 + GCHUNK && sb < nvol; sb += 32) {
-        s16x8 a, b;
+    const int nchunk = (int)(nvol / CL);
+    for (int q = threadIdx.x; q < NTL * 16 * 9 * DW2_RP; q += NT) (&xs[0][0][0])[q] = 0;
+    for (int q = threadIdx.x; q < MT * 16 * DW2_CLMAX; q += NT) (&gs[0][0])[q] = 0;
+    f32x4 acc[3][MT][NTL];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const size_t s = sb + 8 * kg + q;
-            a[q] = (short)((co < COUT) ? G[s * COUT + co] : (bf16)0);
-            bf16 bv = 0;
-            if (bias_col) {
-                bv = (j == 0) ? to_bf16(1.0f) : (bf16)0;
-            } else if (t < 27) {
-                const int x = (int)(s % n), y = (int)((s / n) % n), z = (int)(s / ((size_t)n * n));
-                bv = X[nbr_site(x, y, z, t, n) * CIN + ci];
-            }
-            b[q] = (short)bv;
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < NTL; ++nt) acc[d][mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float db = 0.f; // bias gradient of channel co = threadIdx.x (< COUT)
+    __syncthreads();
+    for (int ch = blockIdx.x; ch < nchunk; ch += gridDim.x) {
+        const size_t s0 = (size_t)ch * CL;
+        const int x0 = (int)(s0 % n), y = (int)((s0 / n) % n), z = (int)(s0 / ((size_t)n * n));
+        // stage X: 9 rows x (CL + 2) sites x CIN channels, 4 channels (8 bytes) per item
+        constexpr int C4 = CIN / 4;
+        for (int q = threadIdx.x; q < 9 * (CL + 2) * C4; q += NT) {
+            const int c4 = q % C4, rest = q / C4;
+            const int xi = rest % (CL + 2), r = rest / (CL + 2);
+            int xx = x0 + xi - 1, yy = y + r % 3 - 1, zz = z + r / 3 - 1;
+            xx = xx < 0 ? xx + n : (xx >= n ? xx - n : xx);
+            yy = yy < 0 ? yy + n : (yy >= n ? yy - n : yy);
+            zz = zz < 0 ? zz + n : (zz >= n ? zz - n : zz);
+            const size_t site = (size_t)xx + (size_t)n * ((size_t)yy + (size_t)n * zz);
+            const bf16x4 v = *reinterpret_cast<const bf16x4 *>(X + site * CIN + 4 * c4);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) xs[4 * c4 + e][r][xi] = (bf16)v[e];
         }
-        acc = mfma32(a, b, acc);
-    }
-    // D[row = 4*kg + e][col = j]
-    float *dst = part + (size_t)chunk * (MT * 16) * LD;
+        // stage G: CL sites x COUT channels
+        constexpr int O4 = COUT / 4;
+        for (int q = threadIdx.x; q < CL * O4; q += NT) {
+            const int o4 = q % O4, xi = q / O4;
+            const bf16x4 v = *reinterpret_cast<const bf16x4 *>(G + (s0 + xi) * COUT + 4 * o4);
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        const int r = mt * 16 + 4 * kg + e;
-        const int cidx = bias_col ? KP + j : ct * 16 + j;
-        dst[(size_t)r * LD + cidx] = acc[e];
+            for (int e = 0; e < 4; ++e) gs[4 * o4 + e][xi] = (bf16)v[e];
+        }
+        __syncthreads();
+        if (threadIdx.x < COUT) {
+            for (int xi = 0; xi < CL; ++xi) db += from_bf16(gs[threadIdx.x][xi]);
+        }
+        for (int ks = 0; ks < KS; ++ks) {
+            // first of the lane's 8 sites within the chunk; at CL = 16 the upper
+            // half of the K step holds no sites and supplies zeros
+            const int xk = 32 * ks + 8 * kg;
+            s16x8 a[MT];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+                a[mt] = xk < CL ? *reinterpret_cast<const s16x8 *>(&gs[mt * 16 + j][xk]) : s16x8{0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+#pragma unroll
+                for (int nt = 0; nt < NTL; ++nt) {
+                    s16x8 b;
+                    const bf16 *src = &xs[nt * 16 + j][row][xk + d]; // site x0 + xk + q + (d - 1), halo at 0
+#pragma unroll
+                    for (int qq = 0; qq < 8; ++qq) b[qq] = xk < CL ? (short)src[qq] : (short)0;
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt) acc[d][mt][nt] = mfma32(a[mt], b, acc[d][mt][nt]);
+                }
+            }
+        }
+        __syncthreads();
     }
+    // D[row co = mt*16 + 4*kg + e][col ci = nt*16 + j] of tap t = (dz+1)*9 + (dy+1)*3 + d
+    float *dst = part + (size_t)blockIdx.x * (MT * 16) * LD;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int t = (dz + 1) * 9 + (dy + 1) * 3 + d;
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < NTL; ++nt) {
+                const int ci = nt * 16 + j;
+                if (ci >= CIN) continue;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int co = mt * 16 + 4 * kg + e;
+                    dst[(size_t)co * LD + t * CIN + ci] = co < COUT ? acc[d][mt][nt][e] : 0.f;
+                }
+            }
+    }
+    // padding columns 27 CIN .. KP-1 and the bias block KP .. KP+15
+    for (int q = threadIdx.x; q < MT * 16 * (LD - 27 * CIN); q += NT) {
+        const int co = q / (LD - 27 * CIN), k = 27 * CIN + q % (LD - 27 * CIN);
+        if (!(k == KP && co < COUT)) dst[(size_t)co * LD + k] = 0.f;
+    }
+    if (threadIdx.x < COUT) dst[(size_t)threadIdx.x * LD + KP] = db;
 }
 
 // out[q] = sum over chunks (in order) of part[chunk][q]
@@ -579,8 +659,12 @@ static void weight_grad(Net &net, Layer &l, const bf16 *G, const bf16 *X)
 {
     const size_t nv = (size_t)net.n * net.n * net.n;
     const int MT = (COUT + 15) / 16, CT = KPAD(CIN) / 16 + 1;
-    const size_t nch = nchunks(nv);
-    hipLaunchKernelGGL((k_conv_dw<CIN, COUT>), dim3((unsigned)(nch * MT * CT)), dim3(64), 0, net.stream, G, X,
+    // one partial per block: at most the nchunks(nv) partials net.part holds
+    size_t nch = nchunks(nv);
+    if (nch > DW2_BLOCKS) nch = DW2_BLOCKS;
+    (void)MT;
+    (void)CT;
+    hipLaunchKernelGGL((k_conv_dw2<CIN, COUT>), dim3((unsigned)nch), dim3(64 * DW2_WAVES), 0, net.stream, G, X,
                        net.part, net.n);
     const int nq = l.mp * (l.kp + 16);
     hipLaunchKernelGGL(k_sum_chunks, dim3((nq + 255) / 256), dim3(256), 0, net.stream, net.part, (int)nch, nq,
