@@ -78,8 +78,10 @@ def algorithmic_bytes(stage, n, nf, w):
     }.get(stage)
 
 
-def run_form(la, n, precision, steps, warmup, out_every, dt, sep=False, laplacian=None, tune=False):
+def run_form(la, n, precision, steps, warmup, out_every, dt, sep=False, laplacian=None, tune=False, korder=None):
     lat = make_lattice(la, n, precision, laplacian)
+    if korder is not None:
+        lat.set_kspace_order(korder == "blocked")
     if sep:
         lat.set_laplacian_form(True)
     if tune:
@@ -548,6 +550,8 @@ def main():
     ap.add_argument("--corr-bound", type=float, default=1e-6,
                     help="fall back when a coarse step moves the relative Friedmann residual by more than this")
     ap.add_argument("--corr-g2", default="1e4,2e4,3e4,4e4,6e4,1e5,1e6")
+    ap.add_argument("--kspace-order", default=None, choices=["blocked", "fftw"],
+                    help="row order of the force pipeline's k-space arrays (default: the library's)")
     ap.add_argument("--no-extras", action="store_true",
                     help="default run: skip the 256^3 drop-in, ensemble and GW legs after the headline")
     args = ap.parse_args()
@@ -599,7 +603,7 @@ def main():
         prec = la.F64 if form == "f64" else la.MIXED
         print(f"[bench] {form} {n}^3: warmup {args.warmup}, timing {args.steps} steps", file=sys.stderr, flush=True)
         res[form] = run_form(la, n, prec, args.steps, args.warmup, args.output_every, dt,
-                             sep=(form == "mixed" and args.lap_form == "sep"))
+                             sep=(form == "mixed" and args.lap_form == "sep"), korder=args.kspace_order)
         r = res[form]
         print(f"[bench] {form}: {r['elapsed']:.3f} s, {n ** 3 * args.steps / r['elapsed']:.4e} site-steps/s "
               f"(init {r['t_init']:.1f} s)", file=sys.stderr, flush=True)

@@ -114,6 +114,14 @@ int  lat_copy_bandwidth(lat_ctx *ctx, size_t bytes, int reps, double *gbps);
  * 512 or 1024 (error otherwise); default 0. */
 int  lat_set_laplacian_form(lat_ctx *ctx, int form);
 int  lat_laplacian_form(lat_ctx *ctx);
+/* Row order of the k-space arrays inside the field force pipeline (no
+ * reference counterpart; transforms handed to the caller are always in
+ * FFTW order): 1 = blocked (the rows of 64 consecutive kz planes of one ky
+ * adjacent, which shortens the z pass's stride and lengthens the y pass's;
+ * cubic 256 and 1024), 0 = FFTW order (default).  The transforms agree
+ * bitwise; the reductions are grouped differently, so the results agree to
+ * rounding. */
+int  lat_set_kspace_order(lat_ctx *ctx, int blocked);
 int  lat_autotune(lat_ctx *ctx);
 int  lat_tuning(lat_ctx *ctx, int *out);
 int  lat_set_tuning(lat_ctx *ctx, const int *in);   /* in[4] of 0/1, as lat_tuning reports */

@@ -141,9 +141,17 @@ struct StageTimer {
 };
 
 // ---------------------------------------------------------------- Laplacian pipeline
+// The blocked k-space order (sweeps.h) in the field force pipeline: cubic
+// 256 and 1024 when switched on (lat_set_kspace_order).
+static int kblk_active(const Ctx &c)
+{
+    return c.kblk && c.nx == c.ny && c.nx == c.nz && (c.nx == 256 || c.nx == 1024) ? 1 : 0;
+}
+
 // Forward R2C (x, y, z) of the nfld state fields into Fk; drift fused if asked.
+// layout 1: the blocked order (zmode 2 only: the transforms stay internal).
 template <typename T>
-static int forward_state(Ctx &c, bool drift, int zmode)
+static int forward_state(Ctx &c, bool drift, int zmode, int layout = 0)
 {
     const cpx<T> *twx = (const cpx<T> *)(sizeof(T) == 8 ? (void *)c.twx64 : (void *)c.twx32);
     const cpx<T> *twy = (const cpx<T> *)(sizeof(T) == 8 ? (void *)c.twy64 : (void *)c.twy32);
@@ -155,15 +163,15 @@ static int forward_state(Ctx &c, bool drift, int zmode)
     for (int l = 0; l < c.nfld; ++l) {
         StageTimer st(c, LAT_ST_X_R2C);
         rc = launch_x_r2c<T>(c, twx, fld + (size_t)l * c.nvol, fldp + (size_t)l * c.nvol, Fk + (size_t)l * c.nk,
-                             drift ? 1 : 0, l);
+                             drift ? 1 : 0, l, layout);
         if (rc) return rc;
     }
     {
         StageTimer st(c, LAT_ST_Y_FWD);
-        if ((rc = launch_col<T>(c, twy, Fk, 1, 0, c.nfld))) return rc;
+        if ((rc = launch_col<T>(c, twy, Fk, 1, 0, c.nfld, layout))) return rc;
     }
     StageTimer st(c, LAT_ST_Z_LAP);
-    if ((rc = launch_col<T>(c, twz, Fk, 2, zmode, c.nfld))) return rc;
+    if ((rc = launch_col<T>(c, twz, Fk, 2, zmode, c.nfld, layout))) return rc;
     return 0;
 }
 
@@ -242,18 +250,19 @@ static int state_force(Ctx &c, bool drift, int epi)
         }
         return g;
     }
-    int rc = forward_state<T>(c, drift, 2);
+    const int layout = kblk_active(c);
+    int rc = forward_state<T>(c, drift, 2, layout);
     if (rc) return rc;
     const cpx<T> *twx = (const cpx<T> *)(sizeof(T) == 8 ? (void *)c.twx64 : (void *)c.twx32);
     const cpx<T> *twy = (const cpx<T> *)(sizeof(T) == 8 ? (void *)c.twy64 : (void *)c.twy32);
     {
         StageTimer st(c, LAT_ST_Y_INV);
-        if ((rc = launch_col<T>(c, twy, (cpx<T> *)c.Fk, 1, 1, c.nfld))) return rc;
+        if ((rc = launch_col<T>(c, twy, (cpx<T> *)c.Fk, 1, 1, c.nfld, layout))) return rc;
     }
     int g;
     {
         StageTimer st(c, LAT_ST_X_C2R);
-        g = launch_x_c2r<T>(c, twx, (cpx<T> *)c.Fk, c.nfld, epi, nullptr, (const T *)c.fld, (T *)c.fldp);
+        g = launch_x_c2r<T>(c, twx, (cpx<T> *)c.Fk, c.nfld, epi, nullptr, (const T *)c.fld, (T *)c.fldp, layout);
     }
     if (g < 0) return g;
     if (gw && (rc = gw_kick<T>(c, drift))) return rc;
@@ -717,6 +726,7 @@ int lat_member(lat_ctx *h, int m, lat_ctx **out)
     w.stream = c.stream;
     w.device = c.device;
     w.cus = c.cus;
+    w.kblk = c.kblk;
     w.fld = (char *)c.fld + es * c.ms.f * m;
     w.fldp = (char *)c.fldp + es * c.ms.f * m;
     w.Fk = c.Fk ? (void *)((char *)c.Fk + 2 * es * c.ms.k * m) : nullptr;
@@ -1830,6 +1840,16 @@ int lat_autotune(lat_ctx *h)
         return -2;
     }
     return autotune(c, 3);
+}
+
+// 1: the field force pipeline keeps its k-space arrays in the blocked row
+// order where it is compiled in (cubic 256 and 1024); 0: FFTW order (default)
+int lat_set_kspace_order(lat_ctx *h, int blocked)
+{
+    CTX_OR_FAIL(h);
+    if (blocked != 0 && blocked != 1) return -1;
+    c.kblk = blocked == 1;
+    return 0;
 }
 
 // 0: the five-sweep 3-D pipeline; 1: the separable three-sweep Laplacian

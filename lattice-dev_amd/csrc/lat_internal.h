@@ -139,6 +139,7 @@ struct Ctx {
     int ns;
     int spec_cap;           // bins the histogram buffers hold
     int cus = 0;            // compute units of the device (sizes the binning grid)
+    bool kblk = false;      // force pipeline in the blocked k-space order where compiled in (kblk_active)
     // The weight channel W(b) of the binning depends on the lattice geometry,
     // the bin count and the weighting flags only, so launch_bin computes it
     // once per (bin count, flags) and keeps it in one of WCACHE slots of
@@ -165,12 +166,14 @@ size_t elem_size(const Ctx &c);
 // If drift != 0, first applies the pending drifts of dsc to f in place using pi
 // (field >= 0: a field, re-centred in the mixed form; field < 0: a tensor
 // component, never re-centred).
+// layout 1 (x R2C, column passes, x C2R): the blocked k-space row order of the
+// force pipeline (sweeps.h), for cubic lattices of a size with it compiled in.
 template <typename T>
-int launch_x_r2c(Ctx &c, const cpx<T> *tw, T *f, const T *pi, cpx<T> *Fk, int drift, int field = 0);
+int launch_x_r2c(Ctx &c, const cpx<T> *tw, T *f, const T *pi, cpx<T> *Fk, int drift, int field = 0, int layout = 0);
 // Column FFT along y (axis 1) or z (axis 2) of a half-complex array.
 // mode 0: forward; 1: inverse; 2: forward, multiply by -k^2 dk^2/nvol, inverse.
 template <typename T>
-int launch_col(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int axis, int mode, int n_arrays);
+int launch_col(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int axis, int mode, int n_arrays, int layout = 0);
 // x-direction C2R of nfld half-complex arrays followed by an epilogue:
 //   EPI_STORE:  write the real rows into out (one array per field)
 //   EPI_KICK:   kick pi with the device coefficients; reduce GE, PE, KE'
@@ -179,7 +182,8 @@ int launch_col(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int axis, int mode, int n_a
 //               fld = S and fldp = pi_u of those components; no reductions
 enum { EPI_STORE = 0, EPI_KICK = 1, EPI_REDUCE = 2, EPI_GWKICK = 3 };
 template <typename T>
-int launch_x_c2r(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int nf, int epi, T *out, const T *fld, T *fldp);
+int launch_x_c2r(Ctx &c, const cpx<T> *tw, cpx<T> *Fk, int nf, int epi, T *out, const T *fld, T *fldp,
+                 int layout = 0);
 
 // ---- separable spectral Laplacian (lat_sep.hip), mixed form ----
 // Kick of all fields with the three-sweep Laplacian (x: drift + D_xx; y:

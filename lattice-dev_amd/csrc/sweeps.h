@@ -51,6 +51,34 @@ __device__ __forceinline__ void tile_range(int T, const Blk &bk, int &start, int
 
 __device__ __forceinline__ int wmag(int j0, int n) { return (j0 <= n / 2) ? j0 : n - j0; }
 
+// ---------------------------------------------------------------- blocked k-space order
+// The force pipeline of cubic lattices may keep its k-space arrays in a
+// blocked row order: the KB rows (ky, kz) with kz in one block of KB planes
+// and the same ky are adjacent,
+//   row(ky, kz) = ((kz / KB) * N + ky) * KB + kz % KB      (FFTW order: kz * N + ky),
+// which shortens the z pass's stride from N rows to one row inside a block
+// (the y pass's stride grows from 1 to KB rows).  The x sweeps walk the
+// real-space rows y + N z in order and map each to its k-space row.  KB = 64
+// matches the first radix-16 pass of a 1024-point column (positions j + 64 q:
+// block q, row j), which keeps the z pass's addressing in compile-time parts.
+constexpr int KBLK = 64;
+// lattice sizes with the blocked order compiled in (the force pipeline of
+// cubic lattices of these sizes uses it when switched on)
+template <int N>
+constexpr bool kblk_size() { return N == 256 || N == 1024; }
+template <int N, int KB>
+__device__ __forceinline__ int kspace_row(int row)
+{
+    if constexpr (KB == 0) {
+        return row;
+    } else {
+        static_assert((KB & (KB - 1)) == 0 && N % KB == 0, "power-of-two block dividing N");
+        const unsigned r = (unsigned)row;
+        const unsigned ky = r % N, kz = r / N;
+        return (int)(((kz / KB) * N + ky) * KB + kz % KB);
+    }
+}
+
 // 16-byte vector of reals (4 floats or 2 doubles) for vector loads/stores.
 template <typename T>
 struct alignas(16) Vec16 {
@@ -113,7 +141,7 @@ constexpr int col_lds() { return col_buf<T, N, TC>() * (PP ? 2 : 1) + (TWL ? N :
 template <typename T, int N, int L, bool TWL>
 constexpr int c2r_lds() { return c2r_buf<T, N, L>() + (TWL ? N : 0); }
 
-template <typename T, int N, int L, bool DRIFT, int MR = 4, bool TWL = false>
+template <typename T, int N, int L, bool DRIFT, int MR = 4, bool TWL = false, int KB = 0>
 __device__ __forceinline__ void x_r2c_body(T *__restrict__ f, const T *__restrict__ pi, cpx<T> *__restrict__ Fk,
                                            const cpx<T> *__restrict__ tw, int rows, int P,
                                            const DevScalars *__restrict__ dsc, int field, MemberStride ms,
@@ -196,7 +224,7 @@ __device__ __forceinline__ void x_r2c_body(T *__restrict__ f, const T *__restric
                 const int line = idx / (M / 2 + 1), kp = idx - line * (M / 2 + 1);
                 const int row = r0 + line;
                 if (row >= rows) continue;
-                cpx<T> *dst = Fk + (size_t)row * P + 2 * kp;
+                cpx<T> *dst = Fk + (size_t)kspace_row<N, KB>(row) * P + 2 * kp;
                 const cpx<T> *s = buf + line * LS + 2 * kp;
                 if (kp < M / 2) *reinterpret_cast<VT *>(dst) = VT{{s[0].x, s[0].y, s[1].x, s[1].y}};
                 else *dst = s[0];
@@ -205,7 +233,7 @@ __device__ __forceinline__ void x_r2c_body(T *__restrict__ f, const T *__restric
             for (int idx = threadIdx.x; idx < L * (M + 1); idx += TPB) {
                 const int line = idx / (M + 1), k = idx - line * (M + 1);
                 const int row = r0 + line;
-                if (row < rows) Fk[(size_t)row * P + k] = buf[line * LS + k];
+                if (row < rows) Fk[(size_t)kspace_row<N, KB>(row) * P + k] = buf[line * LS + k];
             }
         }
         lds_sync();
@@ -218,13 +246,27 @@ __device__ __forceinline__ void x_r2c_body(T *__restrict__ f, const T *__restric
 // the inverse (radices in reverse order) run in registers in one step, which
 // saves one LDS write + read of the tile and a barrier pair.
 // PP (with FUSE): the LDS passes run out of place between two tile buffers.
+// BLK (blocked k-space order, KBLK planes per block; s_blk = N * KBLK * P):
+//   0: FFTW order, line offsets n * s_axis, tile offsets o * s_other;
+//   1: y pass on the blocked order: the tile's kz = o at (o / KBLK) s_blk +
+//      (o % KBLK) s_other (s_other = P), lines along ky with s_axis = KBLK P;
+//   2: z pass on the blocked order: kz = n at (n / KBLK) s_blk + (n % KBLK)
+//      s_axis (s_axis = P), tiles along ky with s_other = KBLK P.
 template <typename T, int N, int TC, int MODE, int MR = 16, bool PF = true, int MINW = 1, int NT = TPB,
-          bool TWL = false, bool FUSE = false, bool PP = false>
+          bool TWL = false, bool FUSE = false, bool PP = false, int BLK = 0>
 __device__ __forceinline__ void col_body(cpx<T> *__restrict__ Fk, const cpx<T> *__restrict__ tw, int nxh, int n_other,
                                          size_t s_axis, size_t s_other, size_t s_arr, int n_arr, int ny, double kfac,
                                          size_t ms_k, const Blk bk, cpx<T> *__restrict__ buf,
-                                         cpx<T> *__restrict__ buf2, cpx<T> *__restrict__ twl)
+                                         cpx<T> *__restrict__ buf2, cpx<T> *__restrict__ twl, size_t s_blk = 0)
 {
+    auto line_off = [&](int n) -> size_t {
+        if constexpr (BLK == 2) return (size_t)((unsigned)n / KBLK) * s_blk + (size_t)((unsigned)n % KBLK) * s_axis;
+        else return (size_t)n * s_axis;
+    };
+    auto tile_off = [&](int o) -> size_t {
+        if constexpr (BLK == 1) return (size_t)((unsigned)o / KBLK) * s_blk + (size_t)((unsigned)o % KBLK) * s_other;
+        else return (size_t)o * s_other;
+    };
     LAT_TWIDDLES_TO_LDS(N, NT, twl)
     assume_blk(bk);
     Fk += bk.y * ms_k;
@@ -249,7 +291,7 @@ __device__ __forceinline__ void col_body(cpx<T> *__restrict__ Fk, const cpx<T> *
         o = rest % n_other;
         const int arr = rest / n_other;
         ncol = min(TC, nxh - ct * TC);
-        return Fk + (size_t)arr * s_arr + (size_t)o * s_other + (size_t)ct * TC;
+        return Fk + (size_t)arr * s_arr + tile_off(o) + (size_t)ct * TC;
     };
     auto load = [&](int t) {
         int ncol, ct, o;
@@ -261,7 +303,7 @@ __device__ __forceinline__ void col_body(cpx<T> *__restrict__ Fk, const cpx<T> *
 #pragma unroll
             for (int q = 0; q < R0; ++q) {
                 cpx<T> a = {0, 0};
-                if ((TOT0 % NT == 0 || g < TOT0) && line < ncol) a = base[(size_t)(j + q * NR0) * s_axis + line];
+                if ((TOT0 % NT == 0 || g < TOT0) && line < ncol) a = base[line_off(j + q * NR0) + line];
                 v[b][q] = a;
             }
         }
@@ -284,7 +326,7 @@ __device__ __forceinline__ void col_body(cpx<T> *__restrict__ Fk, const cpx<T> *
         if (PF && t + ts < t1) load(t + ts);
         lds_sync();
         auto store = [&](int line, int pos, cpx<T> a) {
-            if (line < ncol) base[(size_t)pos * s_axis + line] = a;
+            if (line < ncol) base[line_off(pos) + line] = a;
         };
         if constexpr (MODE == 2 && FUSE && PP && NP >= 2) {
             // pass 0 left the tile in buf; forward passes 1 .. NP-2, fused,
@@ -353,7 +395,8 @@ __device__ __forceinline__ void col_body(cpx<T> *__restrict__ Fk, const cpx<T> *
 }
 
 // ---------------------------------------------------------------- x C2R + epilogue
-template <typename T, int N, int L, int EPI, int MR = 4, bool PRE = true, bool TWL = false, int MINW = 1>
+template <typename T, int N, int L, int EPI, int MR = 4, bool PRE = true, bool TWL = false, int MINW = 1,
+          int KB = 0>
 __device__ __forceinline__ void x_c2r_body(const cpx<T> *__restrict__ Fk, const cpx<T> *__restrict__ tw, int rows,
                                            int P, size_t nk, size_t nvol, int nf, T *__restrict__ out,
                                            const T *__restrict__ fld, T *__restrict__ fldp,
@@ -413,7 +456,7 @@ __device__ __forceinline__ void x_c2r_body(const cpx<T> *__restrict__ Fk, const 
             const int row = r0 + line;
             fr[q] = VT{};
             if (idx < nf * L * U && row < rows) {
-                const cpx<T> *src = Fk + (size_t)fi * nk + (size_t)row * P;
+                const cpx<T> *src = Fk + (size_t)fi * nk + (size_t)kspace_row<N, KB>(row) * P;
                 if (sizeof(T) == 8 || u < M / 2) {
                     fr[q] = *reinterpret_cast<const VT *>(src + (sizeof(T) == 4 ? 2 * u : u));
                 } else {

@@ -66,7 +66,7 @@ EXPORTS = [
     "lat_ensemble_member", "lat_ensemble_use_graph", "lat_ensemble_single_launch", "lat_ensemble_symp",
     "lat_ensemble_synchronize", "lat_ensemble_set_tuning", "lat_ensemble_tuning",
     "lat_autotune", "lat_tuning", "lat_set_tuning", "lat_copy_bandwidth", "lat_make_output",
-    "lat_set_laplacian_form", "lat_laplacian_form",
+    "lat_set_laplacian_form", "lat_laplacian_form", "lat_set_kspace_order",
     "lat_checkpoint", "lat_restore",
     "lat_corrector_create", "lat_corrector_destroy", "lat_corrector_last_error", "lat_corrector_set_layer",
     "lat_corrector_get_layer", "lat_corrector_set_norm", "lat_corrector_get_norm", "lat_corrector_infer",
@@ -171,6 +171,11 @@ class Lattice:
 
     def laplacian_form(self):
         return lib().lat_laplacian_form(self._h)
+
+    def set_kspace_order(self, blocked):
+        """Row order of the force pipeline's k-space arrays: True = blocked
+        (cubic 256 and 1024), False = FFTW order (default)."""
+        self._check(lib().lat_set_kspace_order(self._h, int(bool(blocked))), "lat_set_kspace_order")
 
     def set_tuning(self, x_r2c=0, y=0, z=0, x_c2r=0):
         vals = (ctypes.c_int * 4)(x_r2c, y, z, x_c2r)
