@@ -78,7 +78,8 @@ def algorithmic_bytes(stage, n, nf, w):
     }.get(stage)
 
 
-def run_form(la, n, precision, steps, warmup, out_every, dt, sep=False, laplacian=None, tune=False, korder=None):
+def run_form(la, n, precision, steps, warmup, out_every, dt, sep=False, laplacian=None, tune=False, korder=None,
+             place=0):
     lat = make_lattice(la, n, precision, laplacian)
     if korder is not None:
         lat.set_kspace_order(korder == "blocked")
@@ -90,6 +91,12 @@ def run_form(la, n, precision, steps, warmup, out_every, dt, sep=False, laplacia
     init_state(lat)
     lat.synchronize()
     t_init = time.time() - t0
+    placement = None
+    if place > 1:
+        # keep the fastest of `place` placements of the state and k-space
+        # arrays in HBM (the state is carried over unchanged)
+        kept, ms = lat.place_tune(place)
+        placement = {"candidates": place, "kept": kept, "force_eval_ms": [round(x, 3) for x in ms]}
     lat.symp4(dt, warmup)
     lat.synchronize()
     lat.profile(True)
@@ -107,7 +114,7 @@ def run_form(la, n, precision, steps, warmup, out_every, dt, sep=False, laplacia
     lat.profile(False)
     energies = lat.energies()
     lat.close()
-    return dict(elapsed=elapsed, prof=prof, spectra=spectra, energies=energies, t_init=t_init)
+    return dict(elapsed=elapsed, prof=prof, spectra=spectra, energies=energies, t_init=t_init, placement=placement)
 
 
 def roofline(prof, n, nf, w):
@@ -552,6 +559,9 @@ def main():
     ap.add_argument("--corr-g2", default="1e4,2e4,3e4,4e4,6e4,1e5,1e6")
     ap.add_argument("--kspace-order", default=None, choices=["blocked", "fftw"],
                     help="row order of the force pipeline's k-space arrays (default: the library's)")
+    ap.add_argument("--place-tune", type=int, default=4,
+                    help="1024^3 headline forms: keep the fastest of this many HBM placements of the state and "
+                         "k-space arrays (lat_place_tune; 1 = as allocated)")
     ap.add_argument("--no-extras", action="store_true",
                     help="default run: skip the 256^3 drop-in, ensemble and GW legs after the headline")
     args = ap.parse_args()
@@ -603,7 +613,8 @@ def main():
         prec = la.F64 if form == "f64" else la.MIXED
         print(f"[bench] {form} {n}^3: warmup {args.warmup}, timing {args.steps} steps", file=sys.stderr, flush=True)
         res[form] = run_form(la, n, prec, args.steps, args.warmup, args.output_every, dt,
-                             sep=(form == "mixed" and args.lap_form == "sep"), korder=args.kspace_order)
+                             sep=(form == "mixed" and args.lap_form == "sep"), korder=args.kspace_order,
+                             place=args.place_tune)
         r = res[form]
         print(f"[bench] {form}: {r['elapsed']:.3f} s, {n ** 3 * args.steps / r['elapsed']:.4e} site-steps/s "
               f"(init {r['t_init']:.1f} s)", file=sys.stderr, flush=True)
@@ -628,7 +639,9 @@ def main():
                                f"Yoshida-4, L=10/m, dt=dx/40, spectra every {args.output_every} steps",
                    "form": head,
                    "laplacian_form": ("separable, 3 sweeps" if args.lap_form == "sep" and head == "mixed"
-                                      else "3-D, 5 sweeps")},
+                                      else "3-D, 5 sweeps"),
+                   "placement": (f"fastest of {args.place_tune} HBM placements (lat_place_tune)"
+                                 if args.place_tune > 1 else "as allocated")},
         "roofline": roofline(r["prof"], n, 2, w),
         "hbm_copy_GBs": round(copy_gbs, 1),
     }
@@ -651,6 +664,8 @@ def main():
             line["friedmann_resid"] = {"mixed": r["energies"]["resid"], "f64": f["energies"]["resid"]}
     for form in res:
         line.setdefault("stages_ms", {})[form] = {k: round(v[0], 3) for k, v in res[form]["prof"].items() if v[1]}
+        if res[form]["placement"]:
+            line.setdefault("placement", {})[form] = res[form]["placement"]
     if not args.no_cpu:
         print("[bench] cpu baseline (oracle)", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_steps)

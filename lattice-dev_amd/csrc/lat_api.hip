@@ -1830,6 +1830,155 @@ static int autotune(Ctx &c, int reps)
     return c.prec == LAT_F64 ? autotune_t<double>(c, reps) : autotune_t<float>(c, reps);
 }
 
+// ---------------------------------------------------------------- placement tuner
+// How fast the sweeps stream depends on where the field, momentum and k-space
+// arrays land in HBM: the x R2C of a 1024^3 mixed state, timed on four such
+// allocations in one process, ran 3.07, 3.21, 3.49 and 3.55 ms per launch,
+// each allocation keeping its own figure on every later call.  The tuner
+// allocates further sets of the three arrays, times one force evaluation's
+// sweeps on each (one pending drift of size zero, so that the x R2C reads the
+// momenta as it does in a step; kick coefficients zero; GW kick left out),
+// keeps the fastest set holding the state unchanged and frees the others.
+struct Placement {
+    void *fld = nullptr, *fldp = nullptr, *Fk = nullptr;
+};
+
+template <typename T>
+static int place_time(Ctx &c, int reps, float *ms)
+{
+    auto run = [&]() -> int {
+        const int g = state_force<T>(c, true, EPI_KICK);
+        return g < 0 ? g : 0;
+    };
+    hipEvent_t e0, e1;
+    LAT_HIP(&c, hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return -3;
+    }
+    int rc = run();
+    if (!rc && hipEventRecord(e0, c.stream) != hipSuccess) rc = -3;
+    for (int r = 0; r < reps && !rc; ++r) rc = run();
+    if (!rc && (hipEventRecord(e1, c.stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)) rc = -3;
+    if (!rc) (void)hipEventElapsedTime(ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+static int place_tune(Ctx &c, int cand, int reps)
+{
+    const size_t es = elem_size(c);
+    const size_t bf = es * c.ms.f, bk = c.Fk ? 2 * es * c.ms.k : 0;
+    std::vector<Placement> pl(1);
+    pl[0] = {c.fld, c.fldp, c.Fk};
+    for (int k = 1; k < cand; ++k) {
+        // a further set only while it leaves 8 GiB of device memory free
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess || fr < 2 * bf + bk + (size_t(8) << 30)) {
+            (void)hipGetLastError();
+            break;
+        }
+        Placement q;
+        if (hipMalloc(&q.fld, bf) != hipSuccess || hipMalloc(&q.fldp, bf) != hipSuccess ||
+            (bk && hipMalloc(&q.Fk, bk) != hipSuccess)) {
+            (void)hipGetLastError(); // out of memory: tune over the sets allocated so far
+            for (void *p : {q.fld, q.fldp, q.Fk})
+                if (p) (void)hipFree(p);
+            break;
+        }
+        pl.push_back(q);
+    }
+    const int K = (int)pl.size();
+    int rc = 0;
+    DevScalars *tdsc = nullptr;
+    DevScalars *const dsc0 = c.dsc;
+    void *const Fk64_0 = c.Fk64;
+    const bool gw0 = c.gw, prof0 = c.prof.on;
+    std::vector<float> t(K, 0.f);
+    auto use = [&](const Placement &q) {
+        c.fld = q.fld;
+        c.fldp = q.fldp;
+        c.Fk = q.Fk;
+        if (Fk64_0 && Fk64_0 == pl[0].Fk) c.Fk64 = q.Fk; // the FP64 context's Fk64 is its Fk
+    };
+    auto copy_state = [&](const Placement &to, const Placement &from) -> int {
+        if (hipMemcpyAsync(to.fld, from.fld, bf, hipMemcpyDeviceToDevice, c.stream) != hipSuccess ||
+            hipMemcpyAsync(to.fldp, from.fldp, bf, hipMemcpyDeviceToDevice, c.stream) != hipSuccess)
+            return -3;
+        return 0;
+    };
+    if (K > 1) {
+        // timing scalars: the context's own with one zero drift pending, no
+        // re-centering and a zero kick
+        DevScalars ts;
+        if (hipMalloc((void **)&tdsc, sizeof(DevScalars)) != hipSuccess ||
+            hipMemcpyAsync(&ts, dsc0, sizeof(DevScalars), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+            hipStreamSynchronize(c.stream) != hipSuccess) {
+            rc = -3;
+        } else {
+            ts.nd = 1;
+            for (double &d : ts.dc) d = 0.0;
+            for (int l = 0; l < 2; ++l) ts.nu[l] = ts.mu[l] = 0.0;
+            ts.hA = ts.hB = 0.0;
+            if (hipMemcpy(tdsc, &ts, sizeof(DevScalars), hipMemcpyHostToDevice) != hipSuccess) rc = -3;
+        }
+        c.gw = false;
+        c.prof.on = false;
+        c.dsc = tdsc;
+        // set 1 keeps a copy of the state while set 0 (the context's own
+        // arrays) is timed; set 0 is then restored from it and set 1 timed last
+        bool backed = false;
+        if (!rc) backed = (rc = copy_state(pl[1], pl[0])) == 0;
+        if (!rc) rc = c.prec == LAT_F64 ? place_time<double>(c, reps, &t[0]) : place_time<float>(c, reps, &t[0]);
+        for (int k = 2; k < K && !rc; ++k) {
+            if ((rc = copy_state(pl[k], pl[1]))) break;
+            use(pl[k]);
+            rc = c.prec == LAT_F64 ? place_time<double>(c, reps, &t[k]) : place_time<float>(c, reps, &t[k]);
+            use(pl[0]);
+        }
+        use(pl[0]);
+        // set 0 gets the state back from set 1 (once the copy was made),
+        // whatever happened since
+        if (backed) {
+            const int rr = copy_state(pl[0], pl[1]);
+            if (!rc) rc = rr;
+        }
+        if (!rc) {
+            use(pl[1]);
+            rc = c.prec == LAT_F64 ? place_time<double>(c, reps, &t[1]) : place_time<float>(c, reps, &t[1]);
+            use(pl[0]);
+        }
+        int w = 0;
+        for (int k = 1; k < K && !rc; ++k)
+            if (t[k] < t[w]) w = k;
+        if (!rc && w != 0) rc = copy_state(pl[w], pl[0]);
+        if (!rc && hipStreamSynchronize(c.stream) != hipSuccess) rc = -3;
+        if (rc) w = 0; // on any failure the context keeps its own (restored) arrays
+        c.dsc = dsc0;
+        c.gw = gw0;
+        c.prof.on = prof0;
+        use(pl[w]);
+        (void)hipStreamSynchronize(c.stream);
+        for (int k = 0; k < K; ++k) {
+            if (k == w) continue;
+            for (void *p : {pl[k].fld, pl[k].fldp, pl[k].Fk})
+                if (p) (void)hipFree(p);
+        }
+        c.place_ms.resize(K);
+        for (int k = 0; k < K; ++k) c.place_ms[k] = t[k] / (float)reps; // per force evaluation
+        c.place_pick = w;
+    } else {
+        c.place_ms.clear(); // no further set could be allocated
+        c.place_pick = 0;
+    }
+    if (tdsc) (void)hipFree(tdsc);
+    // the context's device scalars (and with them the cached kinetic sum)
+    // were not touched: the timing ran on a copy of them
+    if (rc) c.last_error = "placement tuning failed";
+    return rc;
+}
+
 extern "C" {
 
 int lat_autotune(lat_ctx *h)
@@ -1840,6 +1989,26 @@ int lat_autotune(lat_ctx *h)
         return -2;
     }
     return autotune(c, 3);
+}
+
+// Placement tuner (no reference counterpart): `candidates` sets of the field,
+// momentum and k-space arrays (the context's own and candidates - 1 new
+// ones) timed on one force evaluation each; the fastest is kept with the
+// state, the others are freed.  Single contexts only; device pointers handed
+// out before (lat_device_state) are invalid afterwards.
+int lat_place_tune(lat_ctx *h, int candidates, float *ms_out)
+{
+    CTX_OR_FAIL(h);
+    if (c.view || c.B != 1) {
+        c.last_error = "placement tuning takes a single context";
+        return -2;
+    }
+    if (candidates < 1 || candidates > 8) return -1;
+    if (candidates == 1) return 0;
+    const int rc = place_tune(c, candidates, 3);
+    if (!rc && ms_out)
+        for (int k = 0; k < candidates; ++k) ms_out[k] = k < (int)c.place_ms.size() ? c.place_ms[k] : -1.f;
+    return rc ? rc : c.place_pick;
 }
 
 // 1: the field force pipeline keeps its k-space arrays in the blocked row

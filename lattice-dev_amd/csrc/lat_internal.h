@@ -140,6 +140,8 @@ struct Ctx {
     int spec_cap;           // bins the histogram buffers hold
     int cus = 0;            // compute units of the device (sizes the binning grid)
     bool kblk = false;      // force pipeline in the blocked k-space order where compiled in (kblk_active)
+    std::vector<float> place_ms; // placement tuner: ms per force evaluation per candidate set (0: the arrays it had)
+    int place_pick = 0;          // placement tuner: the candidate kept
     // The weight channel W(b) of the binning depends on the lattice geometry,
     // the bin count and the weighting flags only, so launch_bin computes it
     // once per (bin count, flags) and keeps it in one of WCACHE slots of

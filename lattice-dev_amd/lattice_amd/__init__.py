@@ -66,7 +66,7 @@ EXPORTS = [
     "lat_ensemble_member", "lat_ensemble_use_graph", "lat_ensemble_single_launch", "lat_ensemble_symp",
     "lat_ensemble_synchronize", "lat_ensemble_set_tuning", "lat_ensemble_tuning",
     "lat_autotune", "lat_tuning", "lat_set_tuning", "lat_copy_bandwidth", "lat_make_output",
-    "lat_set_laplacian_form", "lat_laplacian_form", "lat_set_kspace_order",
+    "lat_set_laplacian_form", "lat_laplacian_form", "lat_set_kspace_order", "lat_place_tune",
     "lat_checkpoint", "lat_restore",
     "lat_corrector_create", "lat_corrector_destroy", "lat_corrector_last_error", "lat_corrector_set_layer",
     "lat_corrector_get_layer", "lat_corrector_set_norm", "lat_corrector_get_norm", "lat_corrector_infer",
@@ -156,6 +156,17 @@ class Lattice:
         """Pick the faster launch shape of each sweep (sizes up to 256)."""
         self._check(lib().lat_autotune(self._h), "lat_autotune")
         return self.tuning()
+
+    def place_tune(self, candidates=4):
+        """Keep the fastest of `candidates` placements of the field, momentum
+        and k-space arrays in HBM, the state unchanged (lat_place_tune).
+        Returns (index of the set kept, ms per force evaluation of each set;
+        set 0 = the arrays the context had, -1 = not allocated)."""
+        ms = (ctypes.c_float * max(int(candidates), 1))()
+        r = lib().lat_place_tune(self._h, int(candidates), ms)
+        if r < 0:
+            self._check(r, "lat_place_tune")
+        return r, [float(x) for x in ms]
 
     def copy_bandwidth(self, nbytes=2 << 30, reps=5):
         """Measured device-copy bandwidth in GB/s (read + write)."""
