@@ -128,15 +128,18 @@ int  lat_set_tuning(lat_ctx *ctx, const int *in);   /* in[4] of 0/1, as lat_tuni
 /* Placement tuner (no reference counterpart): how fast the sweeps stream
  * depends on where the field, momentum and k-space arrays land in HBM (the
  * x R2C of a 1024^3 mixed state ran 3.07-3.55 ms per launch over four
- * allocations in one process, each keeping its figure).  Allocates
- * candidates - 1 further sets of these arrays (fewer if memory runs out),
- * times the sweeps of one force evaluation on each and on the context's own
- * (on copies of the state), keeps the fastest set holding the state
- * unchanged and frees the others.  candidates in 1..8; ms_out (nullable)
- * receives `candidates` times, in ms per force evaluation, one per set (0:
- * the context's own arrays; -1: not allocated).  Returns the index of the set kept (>= 0) or
- * an error (< 0).  Single contexts only (not batches or member views); device
- * pointers from lat_device_state taken before are invalid afterwards. */
+ * allocations in one process, each keeping its figure; the distance between
+ * the field and momentum arrays sets it).  Allocates candidates - 1 further
+ * sets of these arrays (fewer if memory runs out), each with a spacer of its
+ * own size between its field and momentum arrays, times the sweeps of one
+ * force evaluation on each and on the context's own (on copies of the
+ * state), keeps the fastest set holding the state unchanged and frees the
+ * others.  candidates in 1..8; ms_out (nullable) receives `candidates`
+ * times, in ms per force evaluation, one per set (0: the context's own
+ * arrays; -1: not allocated).  Returns the index of the set kept (>= 0) or
+ * an error (< 0).  Single contexts only (not batches or member views);
+ * device pointers from lat_device_state taken before are invalid
+ * afterwards. */
 int  lat_place_tune(lat_ctx *ctx, int candidates, float *ms_out);
 
 /* Ensembles of uneven lattice sizes (no reference counterpart): nclass cubic
