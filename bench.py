@@ -475,6 +475,10 @@ def gw_leg(la, args):
         with la.Lattice(n, nfld=2, length=10.0, precision=prec, laplacian=la.LAP_SPECTRAL, time_var=la.COSMIC,
                         model=la.MODEL_TWOFIELD, model_params=(1.0, G2), gw=True, gw_mpl=1.0) as lat:
             init_state(lat)
+            placement = None
+            if args.place_tune > 1:
+                kept, ms = lat.place_tune(args.place_tune)
+                placement = {"candidates": args.place_tune, "kept": kept, "force_eval_ms": [round(x, 3) for x in ms]}
             lat.symp4(dt, args.warmup)
             lat.synchronize()
             t0 = time.perf_counter()
@@ -484,7 +488,7 @@ def gw_leg(la, args):
             res[form] = {"site_steps_per_s": float(n) ** 3 * args.gw_steps / el,
                          "ms_per_step": 1e3 * el / args.gw_steps,
                          "spectra": [lat.output_spectra(0), lat.output_spectra(1)],
-                         "gw": lat.gw_spectrum(), "e": lat.energies()}
+                         "gw": lat.gw_spectrum(), "e": lat.energies(), "placement": placement}
         print(f"[bench] gw {form} {n}^3: {res[form]['site_steps_per_s']:.4e} site-steps/s", file=sys.stderr,
               flush=True)
 
@@ -497,8 +501,8 @@ def gw_leg(la, args):
     w = 4
     out = {"metric": f"site-steps/s at {n}^3 with the six GW components, mixed vs plain FP64",
            "steps": args.gw_steps, "warmup": args.warmup,
-           "mixed": {k: m[k] for k in ("site_steps_per_s", "ms_per_step")},
-           "f64": {k: d[k] for k in ("site_steps_per_s", "ms_per_step")},
+           "mixed": {k: m[k] for k in ("site_steps_per_s", "ms_per_step", "placement")},
+           "f64": {k: d[k] for k in ("site_steps_per_s", "ms_per_step", "placement")},
            "mixed_over_fp64": m["site_steps_per_s"] / d["site_steps_per_s"],
            # model bytes per site-step with GW (two fields): 3 [2 (7w + 8wc) + 6 (6w + 8wc) + 2w]
            "hbm_frac_mixed_vs_8TBs": m["site_steps_per_s"] * 3 * (2 * (7 * w + 8 * w) + 6 * (6 * w + 8 * w)

@@ -1835,12 +1835,14 @@ static int autotune(Ctx &c, int reps)
 // arrays land in HBM: the x R2C of a 1024^3 mixed state, timed on four such
 // allocations in one process, ran 3.07, 3.21, 3.49 and 3.55 ms per launch,
 // each allocation keeping its own figure on every later call.  The tuner
-// allocates further sets of the three arrays, times one force evaluation's
-// sweeps on each (one pending drift of size zero, so that the x R2C reads the
-// momenta as it does in a step; kick coefficients zero; GW kick left out),
+// allocates further sets of these arrays (with the metric perturbations,
+// their momenta and source scratch where the context has them), times one
+// force evaluation's sweeps on each (one pending drift of size zero, so that
+// the x R2C reads the momenta as it does in a step; kick coefficients zero),
 // keeps the fastest set holding the state unchanged and frees the others.
 struct Placement {
     void *fld = nullptr, *fldp = nullptr, *Fk = nullptr;
+    void *gwu = nullptr, *gwp = nullptr, *gwsrc = nullptr; // contexts with the metric perturbations
 };
 
 template <typename T>
@@ -1870,30 +1872,36 @@ static int place_tune(Ctx &c, int cand, int reps)
 {
     const size_t es = elem_size(c);
     const size_t bf = es * c.ms.f, bk = c.Fk ? 2 * es * c.ms.k : 0;
+    const size_t bg = c.gw ? es * 6 * c.nvol : 0, bs = c.gwsrc ? es * 2 * c.nvol : 0;
     std::vector<Placement> pl(1);
-    pl[0] = {c.fld, c.fldp, c.Fk};
+    pl[0] = {c.fld, c.fldp, c.Fk, c.gwu, c.gwp, c.gwsrc};
     // Set k puts a spacer of its own size between its field and momentum
-    // arrays, so that even where consecutive allocations are consecutive in
-    // memory each set has the two at another distance (a spacer of 66 MB or
-    // 1026 MB made the 1024^3 mixed x R2C 3.20 ms per launch, none or 130 MB
-    // 3.49 ms, in each of several processes).
+    // arrays (and between the metric perturbations and their momenta), so
+    // that even where consecutive allocations are consecutive in memory each
+    // set has the two at another distance (a spacer of 66 MB or 1026 MB made
+    // the 1024^3 mixed x R2C 3.20 ms per launch, none or 130 MB 3.49 ms, in
+    // each of several processes).
     static const size_t spacer_mb[7] = {66, 1026, 34, 258, 18, 514, 2050};
     for (int k = 1; k < cand; ++k) {
         const size_t sp = spacer_mb[(k - 1) % 7] << 20;
         // a further set only while it leaves 8 GiB of device memory free
         size_t fr = 0, tot = 0;
-        if (hipMemGetInfo(&fr, &tot) != hipSuccess || fr < 2 * bf + bk + sp + (size_t(8) << 30)) {
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess || fr < 2 * bf + bk + 2 * bg + bs + sp + (size_t(8) << 30)) {
             (void)hipGetLastError();
             break;
         }
         Placement q;
-        void *spacer = nullptr;
-        const bool ok = hipMalloc(&q.fld, bf) == hipSuccess && hipMalloc(&spacer, sp) == hipSuccess &&
-                        hipMalloc(&q.fldp, bf) == hipSuccess && (!bk || hipMalloc(&q.Fk, bk) == hipSuccess);
-        if (spacer) (void)hipFree(spacer);
+        void *spacer = nullptr, *spacer2 = nullptr;
+        bool ok = hipMalloc(&q.fld, bf) == hipSuccess && hipMalloc(&spacer, sp) == hipSuccess &&
+                  hipMalloc(&q.fldp, bf) == hipSuccess && (!bk || hipMalloc(&q.Fk, bk) == hipSuccess);
+        if (ok && bg)
+            ok = hipMalloc(&q.gwu, bg) == hipSuccess && hipMalloc(&spacer2, sp) == hipSuccess &&
+                 hipMalloc(&q.gwp, bg) == hipSuccess && (!bs || hipMalloc(&q.gwsrc, bs) == hipSuccess);
+        for (void *p : {spacer, spacer2})
+            if (p) (void)hipFree(p);
         if (!ok) {
             (void)hipGetLastError(); // out of memory: tune over the sets allocated so far
-            for (void *p : {q.fld, q.fldp, q.Fk})
+            for (void *p : {q.fld, q.fldp, q.Fk, q.gwu, q.gwp, q.gwsrc})
                 if (p) (void)hipFree(p);
             break;
         }
@@ -1904,17 +1912,23 @@ static int place_tune(Ctx &c, int cand, int reps)
     DevScalars *tdsc = nullptr;
     DevScalars *const dsc0 = c.dsc;
     void *const Fk64_0 = c.Fk64;
-    const bool gw0 = c.gw, prof0 = c.prof.on;
+    const bool prof0 = c.prof.on;
     std::vector<float> t(K, 0.f);
     auto use = [&](const Placement &q) {
         c.fld = q.fld;
         c.fldp = q.fldp;
         c.Fk = q.Fk;
         if (Fk64_0 && Fk64_0 == pl[0].Fk) c.Fk64 = q.Fk; // the FP64 context's Fk64 is its Fk
+        c.gwu = q.gwu;
+        c.gwp = q.gwp;
+        c.gwsrc = q.gwsrc;
     };
     auto copy_state = [&](const Placement &to, const Placement &from) -> int {
         if (hipMemcpyAsync(to.fld, from.fld, bf, hipMemcpyDeviceToDevice, c.stream) != hipSuccess ||
             hipMemcpyAsync(to.fldp, from.fldp, bf, hipMemcpyDeviceToDevice, c.stream) != hipSuccess)
+            return -3;
+        if (bg && (hipMemcpyAsync(to.gwu, from.gwu, bg, hipMemcpyDeviceToDevice, c.stream) != hipSuccess ||
+                   hipMemcpyAsync(to.gwp, from.gwp, bg, hipMemcpyDeviceToDevice, c.stream) != hipSuccess))
             return -3;
         return 0;
     };
@@ -1933,7 +1947,6 @@ static int place_tune(Ctx &c, int cand, int reps)
             ts.hA = ts.hB = 0.0;
             if (hipMemcpy(tdsc, &ts, sizeof(DevScalars), hipMemcpyHostToDevice) != hipSuccess) rc = -3;
         }
-        c.gw = false;
         c.prof.on = false;
         c.dsc = tdsc;
         // set 1 keeps a copy of the state while set 0 (the context's own
@@ -1966,13 +1979,12 @@ static int place_tune(Ctx &c, int cand, int reps)
         if (!rc && hipStreamSynchronize(c.stream) != hipSuccess) rc = -3;
         if (rc) w = 0; // on any failure the context keeps its own (restored) arrays
         c.dsc = dsc0;
-        c.gw = gw0;
         c.prof.on = prof0;
         use(pl[w]);
         (void)hipStreamSynchronize(c.stream);
         for (int k = 0; k < K; ++k) {
             if (k == w) continue;
-            for (void *p : {pl[k].fld, pl[k].fldp, pl[k].Fk})
+            for (void *p : {pl[k].fld, pl[k].fldp, pl[k].Fk, pl[k].gwu, pl[k].gwp, pl[k].gwsrc})
                 if (p) (void)hipFree(p);
         }
         c.place_ms.resize(K);

@@ -55,8 +55,9 @@ def _same(x, y):
 
 @pytest.mark.parametrize("precision,lap,gw", [("f64", "spectral", False), ("mixed", "spectral", False),
                                               ("f64", "fd7", False), ("mixed", "fd7", False),
-                                              ("mixed", "spectral", True)],
-                         ids=["f64-spectral", "mixed-spectral", "f64-fd7", "mixed-fd7", "mixed-spectral-gw"])
+                                              ("mixed", "spectral", True), ("f64", "fd7", True)],
+                         ids=["f64-spectral", "mixed-spectral", "f64-fd7", "mixed-fd7", "mixed-spectral-gw",
+                              "f64-fd7-gw"])
 def test_place_tune_is_invisible(gpu, precision, lap, gw):
     print("case place tune 64", precision, lap, gw, flush=True)
     prec = gpu.F64 if precision == "f64" else gpu.MIXED
