@@ -1870,8 +1870,8 @@ static int place_time(Ctx &c, int reps, float *ms)
 
 static int place_tune(Ctx &c, int cand, int reps)
 {
-    const size_t es = elem_size(c);
-    const size_t bf = es * c.ms.f, bk = c.Fk ? 2 * es * c.ms.k : 0;
+    const size_t es = elem_size(c), B = (size_t)c.B;
+    const size_t bf = es * B * c.ms.f, bk = c.Fk ? 2 * es * B * c.ms.k : 0;
     const size_t bg = c.gw ? es * 6 * c.nvol : 0, bs = c.gwsrc ? es * 2 * c.nvol : 0;
     std::vector<Placement> pl(1);
     pl[0] = {c.fld, c.fldp, c.Fk, c.gwu, c.gwp, c.gwsrc};
@@ -1933,19 +1933,21 @@ static int place_tune(Ctx &c, int cand, int reps)
         return 0;
     };
     if (K > 1) {
-        // timing scalars: the context's own with one zero drift pending, no
-        // re-centering and a zero kick
-        DevScalars ts;
-        if (hipMalloc((void **)&tdsc, sizeof(DevScalars)) != hipSuccess ||
-            hipMemcpyAsync(&ts, dsc0, sizeof(DevScalars), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+        // timing scalars: the context's own (per member) with one zero drift
+        // pending, no re-centering and a zero kick
+        std::vector<DevScalars> ts(B);
+        if (hipMalloc((void **)&tdsc, sizeof(DevScalars) * B) != hipSuccess ||
+            hipMemcpyAsync(ts.data(), dsc0, sizeof(DevScalars) * B, hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
             hipStreamSynchronize(c.stream) != hipSuccess) {
             rc = -3;
         } else {
-            ts.nd = 1;
-            for (double &d : ts.dc) d = 0.0;
-            for (int l = 0; l < 2; ++l) ts.nu[l] = ts.mu[l] = 0.0;
-            ts.hA = ts.hB = 0.0;
-            if (hipMemcpy(tdsc, &ts, sizeof(DevScalars), hipMemcpyHostToDevice) != hipSuccess) rc = -3;
+            for (DevScalars &m : ts) {
+                m.nd = 1;
+                for (double &d : m.dc) d = 0.0;
+                for (int l = 0; l < 2; ++l) m.nu[l] = m.mu[l] = 0.0;
+                m.hA = m.hB = 0.0;
+            }
+            if (hipMemcpy(tdsc, ts.data(), sizeof(DevScalars) * B, hipMemcpyHostToDevice) != hipSuccess) rc = -3;
         }
         c.prof.on = false;
         c.dsc = tdsc;
@@ -2413,7 +2415,9 @@ int lat_ensemble_create(const lat_config *base, int nclass, const int *n, const 
         (void)hipStreamDestroy(c.stream);
         c.stream = e->stream;
         c.owns_stream = false;
-        if (autotune(c, 3)) {
+        // launch shapes, then the fastest of four placements of the class's
+        // arrays (before any graph or member view holds their addresses)
+        if (autotune(c, 3) || place_tune(c, 4, 3)) {
             e->last_error = c.last_error;
             lat_destroy(h);
             lat_ensemble_destroy(e);
