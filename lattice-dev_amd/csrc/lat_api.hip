@@ -2028,7 +2028,10 @@ int lat_place_tune(lat_ctx *h, int candidates, float *ms_out)
         return -2;
     }
     if (candidates < 1 || candidates > 8) return -1;
-    if (candidates == 1) return 0;
+    if (candidates == 1) { // nothing to compare: the context keeps its arrays, untimed
+        if (ms_out) ms_out[0] = -1.f;
+        return 0;
+    }
     const int rc = place_tune(c, candidates, 3);
     if (!rc && ms_out)
         for (int k = 0; k < candidates; ++k) ms_out[k] = k < (int)c.place_ms.size() ? c.place_ms[k] : -1.f;

@@ -108,4 +108,4 @@ def test_place_tune_rejects_batches_views_and_bad_counts(gpu):
         for k in (0, 9):
             with pytest.raises(gpu.LatticeError):
                 lat.place_tune(k)
-        assert lat.place_tune(1)[0] == 0
+        assert lat.place_tune(1) == (0, [-1.0])
