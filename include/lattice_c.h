@@ -153,7 +153,7 @@ int  lat_place_tune(lat_ctx *ctx, int candidates, float *ms_out);
  * classes of spectral, cubic lattices of 16, 32, 48, 64, 96, 128, 192 or 256
  * sites per side), or one launch per sweep per class.  At creation the
  * autotuner times both and keeps the faster, and each class's arrays are
- * put on the fastest of four HBM placements (as lat_place_tune does for a
+ * put on the fastest of eight HBM placements (as lat_place_tune does for a
  * single context).  lat_ensemble_single_launch(ens,
  * 1 or 0) forces a mode, (ens, -1) only queries; it returns 1 when the single
  * launch is in use (0 if not, or if the classes do not allow it).  Members

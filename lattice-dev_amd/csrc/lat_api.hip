@@ -2418,9 +2418,9 @@ int lat_ensemble_create(const lat_config *base, int nclass, const int *n, const 
         (void)hipStreamDestroy(c.stream);
         c.stream = e->stream;
         c.owns_stream = false;
-        // launch shapes, then the fastest of four placements of the class's
+        // launch shapes, then the fastest of eight placements of the class's
         // arrays (before any graph or member view holds their addresses)
-        if (autotune(c, 3) || place_tune(c, 4, 3)) {
+        if (autotune(c, 3) || place_tune(c, 8, 3)) {
             e->last_error = c.last_error;
             lat_destroy(h);
             lat_ensemble_destroy(e);
